@@ -678,16 +678,19 @@ orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
     }
     /* closure faces read lmd+1 points inward plus the face point (mpigo,
        src/Subroutines3D.f90:39; deriv :107-108): lines need >= lmd+2 points */
-    for (int nn = 0; nn < 3; nn++)
-      if ((B->ndf[nn][0] || B->ndf[nn][1]) && L[nn] + 1 < LMD + 2) {
-        fprintf(stderr, "oracle: block %d direction %d has %d points; closure faces need >= %d\n",
-                b, nn + 1, L[nn] + 1, LMD + 2);
+    for (int nn = 0; nn < 3; nn++) {
+      int need = 7;
+      for (int s = 0; s < 2; s++) {
+        int np = B->nbc[nn][s];
+        if (np == 40 && need < LMD + 1) need = LMD + 1;
+        if ((np == 35 || np == 45) && need < LMD + 2) need = LMD + 2;
+      }
+      if (L[nn] + 1 < need) {
+        fprintf(stderr, "oracle: block %d direction %d has %d points; its faces need >= %d\n",
+                b, nn + 1, L[nn] + 1, need);
         abort();
       }
-      else if (L[nn] + 1 < 7) {
-        fprintf(stderr, "oracle: block %d direction %d has %d points; need >= 7\n", b, nn + 1, L[nn] + 1);
-        abort();
-      }
+    }
     B->xbc_ra0 = c->cbca_ra0;
     B->lsz = -1;
   }
