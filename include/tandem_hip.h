@@ -1,0 +1,138 @@
+/*
+ * tandem_hip.h -- C ABI of the MI355X FP64 compact-scheme time stepper.
+ *
+ * The reference's hot path is a set of Fortran module procedures working on
+ * module-global arrays of one MPI rank (src/Subroutines3D.f90, the stage body
+ * inline in src/Main3D.f90:552-965).  This ABI is the drop-in for that path:
+ * a context owns device mirrors of those arrays for every sub-block of every
+ * case, and each entry point below replaces one reference routine (or one
+ * block of inline statements) and acts on all sub-blocks at once, with the
+ * MPI exchanges performed as device-side copies.
+ *
+ * Conventions
+ *   - every function returns 0 on success, a nonzero TP_E_* code otherwise;
+ *     tp_last_error() gives the message (per thread);
+ *   - arrays use the reference's layout: a(0:lmx, var) in Fortran order with
+ *     l = (k*(let+1)+j)*(lxi+1)+i, faces face(0:F-1, c, 0:1);
+ *   - directions nn = 1 (xi), 2 (eta), 3 (zeta) and columns nz, m are 1-based
+ *     exactly as in the reference;
+ *   - calls are asynchronous on the context's HIP stream; tp_download and
+ *     tp_sync synchronise.
+ *   - the library never falls back to a CPU path: without a usable GPU,
+ *     tp_create fails.
+ */
+#ifndef TANDEM_HIP_H
+#define TANDEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tp_context tp_context;
+
+enum {
+  TP_OK = 0, TP_E_ARG = 1, TP_E_STATE = 2, TP_E_HIP = 3, TP_E_SIZE = 4, TP_E_LAYOUT = 5
+};
+
+/* integer case parameters, iparams[] (inputo.dat lines 5-10, 20; src/Main3D.f90:39-56) */
+enum { TP_I_NKRK = 0, TP_I_NVISCOUS, TP_I_NSMF, TP_I_LES, TP_I_NEXTRABC, TP_I_NEXTGCIC, TP_I_NPARAM = 16 };
+/* double case parameters, dparams[] (inputo.dat, inputp.dat; fltk/fltkbc in units of pi) */
+enum {
+  TP_D_REOO = 0, TP_D_TEMPOO, TP_D_AMACH1, TP_D_AMACH2, TP_D_AMACH3, TP_D_WTEMP, TP_D_CFL,
+  TP_D_TMAX, TP_D_TIMF, TP_D_FLTK, TP_D_FLTKBC, TP_D_DTO, TP_D_AOA0, TP_D_AOA1, TP_D_TPS,
+  TP_D_TP, TP_D_SMAGO1, TP_D_SMAGO2, TP_D_DOMLEN, TP_D_SZTH1, TP_D_SZTH2, TP_D_SZXT,
+  TP_D_SZCO, TP_D_SPAN, TP_D_WLEW, TP_D_WLEA, TP_D_NPARAM = 32
+};
+/* arrays reachable by tp_upload / tp_download (names of src/MainVar3D.f90:78-102) */
+enum {
+  TP_QA = 0, TP_QO, TP_DE, TP_RR, TP_SS, TP_XIM, TP_ETM, TP_ZEM, TP_P, TP_YACO, TP_TV,
+  TP_DRVA1, TP_DRVA2, TP_DRVA3, TP_DRVB1, TP_DRVB2, TP_DRVB3, TP_SEND1, TP_SEND2, TP_SEND3,
+  TP_RECV1, TP_RECV2, TP_RECV3, TP_CM1, TP_CM2, TP_CM3, TP_XU, TP_XL, TP_YU, TP_YL, TP_RPEX,
+  TP_NARRAY
+};
+/* scalars of a case (tp_get_scalars / tp_set_scalars), src/MainVar3D.f90:139-141 */
+enum {
+  TP_S_N = 0, TP_S_NDT, TP_S_DT, TP_S_DTS, TP_S_DTE, TP_S_TIMO, TP_S_UMF1, TP_S_UMF2, TP_S_UMF3,
+  TP_S_DUDT1, TP_S_DUDT2, TP_S_DUDT3, TP_S_DTK, TP_S_DTKO, TP_S_AOA, TP_S_NSCALAR = 16
+};
+/* kernel classes reported by tp_profile_read */
+enum {
+  TP_K_DERIV = 0, TP_K_FILTER, TP_K_EXCH, TP_K_POINT, TP_K_FLUX, TP_K_VISC, TP_K_GCBC, TP_K_RK,
+  TP_K_MISC, TP_K_NCLASS
+};
+
+/* ---- context ---- */
+int32_t tp_create(tp_context **ctx);            /* on the current HIP device */
+int32_t tp_destroy(tp_context *ctx);
+const char *tp_last_error(void);
+
+/* A case is one reference run: its sub-blocks (MPI ranks of the reference),
+ * their boundary codes and neighbours (src/Main3D.f90:92-178,
+ * src/SingleTurbulence3D.f90:121-164), its corner junctions
+ * (src/SingleTurbulence3D.f90:378-443) and its input-deck parameters.
+ *   dims[b*3+d]             lxi, let, lze of sub-block b
+ *   nbc/ncd/nsz[b*6+d*2+ip] boundary code, neighbour sub-block (case-local
+ *                           index, -1 none), sponge flag of face ip of direction d
+ *   jct[j*12+mbr*3+{0,1,2}] (sub-block, ip, jp) of corner member mbr of junction j
+ *   jvel[j*2+{0,1}]         members whose velocities are averaged
+ * Several cases in one context are advanced together (ensembles). */
+int32_t tp_add_case(tp_context *ctx, int32_t nblocks, const int32_t *dims, const int32_t *nbc,
+                    const int32_t *ncd, const int32_t *nsz, int32_t njct, const int32_t *jct,
+                    const int32_t *jvel, const int32_t *iparams, const double *dparams,
+                    int32_t *case_id);
+/* allocate device state, compute the scheme coefficients and LU factors
+ * (src/Main3D.f90:293-332), build the launch tables */
+int32_t tp_finalize(tp_context *ctx);
+
+/* grid of one sub-block, ss(0:lmx,1:3) = x, y, z (src/rpt.f90:673-696) */
+int32_t tp_set_grid(tp_context *ctx, int32_t case_id, int32_t block, const double *xyz);
+/* metrics, Jacobian, face normals, wall-slot flags and sponge coefficients
+ * for every case from its grid (src/Main3D.f90:365-480, src/SingleTurbulence3D.f90:209-249) */
+int32_t tp_setup_geometry(tp_context *ctx);
+/* uniform initial state (src/SingleTurbulence3D.f90:179-205) and n = 0, t = 0 */
+int32_t tp_initialo(tp_context *ctx, int32_t case_id);
+
+int32_t tp_upload(tp_context *ctx, int32_t case_id, int32_t block, int32_t array_id,
+                  const double *src, int64_t count);
+int32_t tp_download(tp_context *ctx, int32_t case_id, int32_t block, int32_t array_id,
+                    double *dst, int64_t count);
+int64_t tp_array_count(tp_context *ctx, int32_t case_id, int32_t block, int32_t array_id);
+int32_t tp_get_scalars(tp_context *ctx, int32_t case_id, double *s16);
+int32_t tp_set_scalars(tp_context *ctx, int32_t case_id, const double *s16);
+
+/* ---- per-call drop-ins, all sub-blocks of all cases ---- */
+/* mpigo(nt,nrt,n45,itag): src/Subroutines3D.f90:16-80 (closure sums to the neighbours' recv) */
+int32_t tp_mpigo(tp_context *ctx, int32_t nt, int32_t nrt, int32_t n45);
+/* deriv(nn,nz,m): src/Subroutines3D.f90:84-141 (rr(:,nn) = d rr(:,nz)/d index_nn, drva capture) */
+int32_t tp_deriv(tp_context *ctx, int32_t nn, int32_t nz, int32_t m);
+/* filte(nn,nz): src/Subroutines3D.f90:145-206 (rr(:,nz) += filter increment) */
+int32_t tp_filte(tp_context *ctx, int32_t nn, int32_t nz);
+
+/* ---- coarse entry points ---- */
+/* filtering of qa and qo = qa: src/Main3D.f90:554-561 */
+int32_t tp_filter_step(tp_context *ctx);
+/* one Runge-Kutta stage nk: src/Main3D.f90:567-958 */
+int32_t tp_stage(tp_context *ctx, int32_t nk);
+/* nsteps whole time steps (filter + nkrk stages + n, t advance): src/Main3D.f90:552-965 */
+int32_t tp_step(tp_context *ctx, int32_t nsteps);
+/* the same, replaying one step captured in a HIP graph */
+int32_t tp_step_graph(tp_context *ctx, int32_t nsteps);
+int32_t tp_sync(tp_context *ctx);
+
+/* ---- measurement ---- */
+/* record HIP events around every launch (on the context's stream) */
+int32_t tp_profile(tp_context *ctx, int32_t on);
+/* accumulated device time (ms) and launch count of a kernel class since the
+ * last reset; algorithmic HBM bytes those launches move (bytes per point of
+ * the class times points); reset != 0 clears the counters afterwards */
+int32_t tp_profile_read(tp_context *ctx, int32_t kclass, double *ms, int64_t *launches,
+                        double *bytes, int32_t reset);
+/* out[0] points, out[1] sub-blocks, out[2] cases, out[3] device bytes held */
+int32_t tp_info(tp_context *ctx, int64_t *out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

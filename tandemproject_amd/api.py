@@ -1,0 +1,202 @@
+"""Python host over the C ABI of libtandem_hip.so.
+
+``Solver`` mirrors the reference's operator interface for the hot path
+(src/Subroutines3D.f90 ``mpigo``/``deriv``/``filte`` and the time loop of
+src/Main3D.f90:552-965) on device-resident state.  The HIP library is the
+only compute path: importing this module on a machine where the library
+cannot be loaded, or creating a ``Solver`` without a GPU, raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtandem_hip.so")
+
+I_NAMES = ["nkrk", "nviscous", "nsmf", "les", "nextrabc", "nextgcic"]
+D_NAMES = ["reoo", "tempoo", "amach1", "amach2", "amach3", "wtemp", "cfl", "tmax",
+           "timf", "fltk", "fltkbc", "dto", "aoa0", "aoa1", "tps", "tp", "smago1",
+           "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea"]
+ARRAYS = ["qa", "qo", "de", "rr", "ss", "xim", "etm", "zem", "p", "yaco", "tv",
+          "drva1", "drva2", "drva3", "drvb1", "drvb2", "drvb3", "send1", "send2", "send3",
+          "recv1", "recv2", "recv3", "cm1", "cm2", "cm3", "xu", "xl", "yu", "yl", "rpex"]
+SCALARS = ["n", "ndt", "dt", "dts", "dte", "timo", "umf1", "umf2", "umf3", "dudt1",
+           "dudt2", "dudt3", "dtk", "dtko", "aoa"]
+KCLASS = ["deriv", "filter", "exchange", "point", "flux", "viscous", "gcbc", "rk", "misc"]
+EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finalize",
+           "tp_set_grid", "tp_setup_geometry", "tp_initialo", "tp_upload", "tp_download",
+           "tp_array_count", "tp_get_scalars", "tp_set_scalars", "tp_mpigo", "tp_deriv",
+           "tp_filte", "tp_filter_step", "tp_stage", "tp_step", "tp_step_graph", "tp_sync",
+           "tp_profile", "tp_profile_read", "tp_info"]
+
+_lib = None
+
+
+class TPError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TPError(f"{path} missing: build it with `python -m tandemproject_amd.build` "
+                      "(there is no CPU fallback)")
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    pi32 = ctypes.POINTER(ctypes.c_int32)
+    pd = ctypes.POINTER(ctypes.c_double)
+    L.tp_last_error.restype = ctypes.c_char_p
+    sig = {
+        "tp_create": [ctypes.POINTER(P)], "tp_destroy": [P],
+        "tp_add_case": [P, i32, pi32, pi32, pi32, pi32, i32, pi32, pi32, pi32, pd, pi32],
+        "tp_finalize": [P], "tp_set_grid": [P, i32, i32, pd], "tp_setup_geometry": [P],
+        "tp_initialo": [P, i32], "tp_upload": [P, i32, i32, i32, pd, i64],
+        "tp_download": [P, i32, i32, i32, pd, i64], "tp_get_scalars": [P, i32, pd],
+        "tp_set_scalars": [P, i32, pd], "tp_mpigo": [P, i32, i32, i32],
+        "tp_deriv": [P, i32, i32, i32], "tp_filte": [P, i32, i32], "tp_filter_step": [P],
+        "tp_stage": [P, i32], "tp_step": [P, i32], "tp_step_graph": [P, i32], "tp_sync": [P],
+        "tp_profile": [P, i32],
+        "tp_profile_read": [P, i32, pd, ctypes.POINTER(i64), pd, i32],
+        "tp_info": [P, ctypes.POINTER(i64)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = i32
+    L.tp_array_count.argtypes = [P, i32, i32, i32]
+    L.tp_array_count.restype = i64
+    _lib = L
+    return L
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc):
+    if rc != 0:
+        raise TPError(load().tp_last_error().decode())
+
+
+class Solver:
+    """Device-resident state of one or more cases (each a list of sub-blocks)."""
+
+    def __init__(self, cases):
+        """cases: list of (layout, params) pairs."""
+        L = load()
+        self.h = ctypes.c_void_p()
+        _check(L.tp_create(ctypes.byref(self.h)))
+        self.layouts = []
+        self._keep = []
+        for layout, params in cases:
+            ipar = np.zeros(16, np.int32)
+            for i, k in enumerate(I_NAMES):
+                ipar[i] = int(params.get(k, {"nkrk": 4, "nviscous": 1}.get(k, 0)))
+            dpar = np.zeros(32)
+            for i, k in enumerate(D_NAMES):
+                dpar[i] = float(params.get(k, 0.0))
+            arrs = [np.ascontiguousarray(a, np.int32) for a in
+                    (layout.dims, layout.nbc, layout.ncd, layout.nsz, layout.jct, layout.jvel)]
+            self._keep.append((arrs, ipar, dpar))
+            cid = ctypes.c_int32()
+            d, b, n, s, j, v = arrs
+            _check(L.tp_add_case(self.h, layout.nb, _ip(d), _ip(b), _ip(n), _ip(s), layout.njct,
+                                 _ip(j), _ip(v), _ip(ipar), _dp(dpar), ctypes.byref(cid)))
+            self.layouts.append(layout)
+        _check(L.tp_finalize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().tp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- data ----
+    def set_grid(self, case, block, xyz):
+        x = np.ascontiguousarray(xyz, np.float64)
+        _check(load().tp_set_grid(self.h, case, block, _dp(x)))
+
+    def setup_geometry(self):
+        _check(load().tp_setup_geometry(self.h))
+
+    def initialo(self, case=0):
+        _check(load().tp_initialo(self.h, case))
+
+    def count(self, case, block, name):
+        return int(load().tp_array_count(self.h, case, block, ARRAYS.index(name)))
+
+    def upload(self, case, block, name, values):
+        v = np.ascontiguousarray(values, np.float64).ravel()
+        _check(load().tp_upload(self.h, case, block, ARRAYS.index(name), _dp(v), v.size))
+
+    def download(self, case, block, name):
+        n = self.count(case, block, name)
+        out = np.empty(max(n, 0), np.float64)
+        _check(load().tp_download(self.h, case, block, ARRAYS.index(name), _dp(out), n))
+        return out
+
+    def scalars(self, case=0):
+        s = np.zeros(16)
+        _check(load().tp_get_scalars(self.h, case, _dp(s)))
+        return {k: float(s[i]) for i, k in enumerate(SCALARS)}
+
+    def set_scalars(self, case, values):
+        s = np.zeros(16)
+        _check(load().tp_get_scalars(self.h, case, _dp(s)))
+        for k, v in values.items():
+            s[SCALARS.index(k)] = v
+        _check(load().tp_set_scalars(self.h, case, _dp(s)))
+
+    # ---- reference routines (all sub-blocks) ----
+    def mpigo(self, nt, nrt, n45):
+        _check(load().tp_mpigo(self.h, nt, nrt, n45))
+
+    def deriv(self, nn, nz, m):
+        _check(load().tp_deriv(self.h, nn, nz, m))
+
+    def filte(self, nn, nz):
+        _check(load().tp_filte(self.h, nn, nz))
+
+    def filter_step(self):
+        _check(load().tp_filter_step(self.h))
+
+    def stage(self, nk):
+        _check(load().tp_stage(self.h, nk))
+
+    def step(self, n=1, graph=False):
+        if graph:
+            _check(load().tp_step_graph(self.h, n))
+        else:
+            _check(load().tp_step(self.h, n))
+
+    def sync(self):
+        _check(load().tp_sync(self.h))
+
+    # ---- measurement ----
+    def profile(self, on=True):
+        _check(load().tp_profile(self.h, 1 if on else 0))
+
+    def profile_read(self, kclass, reset=False):
+        ms = ctypes.c_double(); n = ctypes.c_int64(); by = ctypes.c_double()
+        _check(load().tp_profile_read(self.h, KCLASS.index(kclass), ctypes.byref(ms), ctypes.byref(n),
+                                      ctypes.byref(by), 1 if reset else 0))
+        return ms.value, n.value, by.value
+
+    def info(self):
+        out = (ctypes.c_int64 * 4)()
+        _check(load().tp_info(self.h, out))
+        return dict(points=out[0], blocks=out[1], cases=out[2], device_bytes=out[3])

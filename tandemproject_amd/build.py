@@ -1,0 +1,56 @@
+"""Builds the HIP library libtandem_hip.so in-tree for gfx950.
+
+hipcc compiles the kernels and the C ABI directly; no torch extension
+machinery is involved.  FP contraction is off so that every line solve rounds
+like the reference's sequential Fortran; packed-FP32 instructions are
+disabled as required on these GPUs (the kernels are FP64 anyway).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+SOURCES = ["tp_line.hip", "tp_stage.hip", "tp_capi.hip"]
+LIB = os.path.join(HERE, "libtandem_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("TP_OFFLOAD_ARCH", "gfx950")
+
+FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-munsafe-fp-atomics", "-Wall", "-Wno-unused-variable", "-Wno-unused-function"]
+
+
+def build(verbose=False, jobs=3):
+    objs = []
+    procs = []
+    for s in SOURCES:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(CSRC, s.replace(".hip", ".o"))
+        objs.append(obj)
+        cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for s, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append((s, out))
+        elif verbose and out.strip():
+            print(out)
+    if failed:
+        for s, out in failed:
+            sys.stderr.write(f"---- {s} ----\n{out}\n")
+        raise RuntimeError("hipcc failed on " + ", ".join(s for s, _ in failed))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    subprocess.check_call(cmd)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

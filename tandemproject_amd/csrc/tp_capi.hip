@@ -1,0 +1,1165 @@
+// tp_capi.hip -- host side of the MI355X time stepper: context, device memory,
+// scheme coefficients, launch tables, stage sequencing, and the C ABI of
+// include/tandem_hip.h.
+//
+// The scheme coefficients are computed here from their definitions
+// (penta: banded LU without pivoting of the pentadiagonal LHS; fcint/fcbcm:
+// cut-off-wavenumber filter coefficients; sbcco: quasi-disjoint closure rows
+// from the inverse of the coupled two-sided system), following
+// src/Subroutineso.f90:15-188 and src/Main3D.f90:293-332 so that the device
+// consumes the same tables the reference's setup produces.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tandem_hip.h"
+#include "tp_internal.hpp"
+
+using namespace tp;
+
+// ----------------------------------------------------------------------------
+// errors
+// ----------------------------------------------------------------------------
+static thread_local std::string g_err;
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(TP_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+// ----------------------------------------------------------------------------
+// scheme coefficients (host, once per case)
+// ----------------------------------------------------------------------------
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Kim's interior derivative scheme and its one-sided closures, src/MainVar3D.f90:34-68
+constexpr double kAlpha = 0.5862704032801503, kBeta = 0.09549533555017055;
+const double kBnd[3][5] = {{0, 0, 1, 5.912678614078549, 3.775623951744012},
+                           {0, 0.08360703307833438, 1, 2.058102869495757, 0.9704052014790193},
+                           {0.03250008295108466, 0.3998040493524358, 1, 0.7719261277615860,
+                            0.1626635931256900}};
+const double kAbc[3][6] = {{-3.456878182643609, 5.839043358834730, 1.015886726041007,
+                            -0.2246526470654333, 0.08564940889936562, -0.01836710059356763},
+                           {-0.3177447290722621, -0.02807631929593225, 1.593461635747659,
+                            0.2533027046976367, -0.03619652460174756, 0.004080281419108407},
+                           {-0.1219006056449124, -0.6301651351188667, 0.6521195063966084,
+                            0.3938843551210350, 0.01904944407973912, -0.001027260523947668}};
+constexpr double kAA = 0.6431406736919156, kAB = 0.2586011023495066, kAC = 0.007140953479797375;
+
+struct FiltCoef { double alpha, beta, a, b, c; };
+
+// filter coefficients from the cut-off wavenumber fk and fltr = 1/2
+FiltCoef filter_coef(double fk, double fr) {
+  const double c1 = std::cos(fk), c2 = std::cos(2 * fk), c3 = std::cos(3 * fk);
+  const double f = 1 / (30 + 5 * (7 - 16 * fr) * c1 + 2 * (1 + 8 * fr) * c2 - 3 * c3);
+  const double h = std::cos(0.5 * fk), h2 = h * h;
+  FiltCoef o;
+  o.alpha = f * (20 * (2 * fr - 1) - 30 * c1 + 12 * (2 * fr - 1) * c2 - 2 * c3);
+  o.beta = 0.5 * f * (2 * (13 - 8 * fr) + (33 - 48 * fr) * c1 + 6 * c2 - c3);
+  o.a = 60 * (1 - fr) * (h2 * h2) * f;
+  o.b = -2 * o.a / 5;
+  o.c = o.a / 15;
+  return o;
+}
+
+// LHS rows 0..2 of the filter next to a boundary (cut-off ramped to fltkbc)
+void filter_boundary(double fk, double fkbc, double rows[3][5], double fam[3], double fbm[3],
+                     double fcm[3]) {
+  const double step = (fkbc - fk) / 9;
+  const double cut[3] = {fk + 9 * step, fk + 4 * step, fk + step};
+  for (int r = 0; r < 3; r++) {
+    FiltCoef z = filter_coef(cut[r], 0.5);
+    if (r == 0) {
+      const double s = 1 / (1 + z.alpha / 2 + z.beta / 4);
+      const double v[5] = {0, 0, 1, z.alpha * s, z.beta * s};
+      std::copy(v, v + 5, rows[0]);
+      fam[0] = z.a * s; fbm[0] = z.b * s; fcm[0] = z.c * s;
+    } else if (r == 1) {
+      const double v[5] = {0, z.alpha + z.beta / 2, 1, z.alpha, z.beta};
+      std::copy(v, v + 5, rows[1]);
+      fam[1] = z.a; fbm[1] = z.b; fcm[1] = z.c;
+    } else {
+      const double v[5] = {z.beta, z.alpha, 1, z.alpha, z.beta};
+      std::copy(v, v + 5, rows[2]);
+      fam[2] = z.a; fbm[2] = z.b; fcm[2] = z.c;
+    }
+  }
+}
+
+// banded LU (no pivoting) of the pentadiagonal matrix whose first three rows
+// are `head`, last three rows the mirror of `tail` and interior (beta, alpha,
+// 1, alpha, beta); l1/l2 multipliers, u1 = 1/pivot, u2/u3 scaled upper band
+void band_lu(int n, const double head[3][5], const double tail[3][5], double al, double be, double *l1,
+             double *l2, double *u1, double *u2, double *u3) {
+  for (int i = 0; i < n; i++) { l1[i] = l2[i] = 1; u1[i] = u2[i] = u3[i] = 1; }
+  auto pivot = [&](int i, double lo2, double lo1, double up1, double up2, bool set_up2) {
+    // row i: lo2 at i-2, lo1 at i-1, 1 at i, up1 at i+1, up2 at i+2
+    if (i >= 2) {
+      l1[i] = lo2 * u1[i - 2];
+      l2[i] = (lo1 - u2[i - 2] * l1[i]) * u1[i - 1];
+      u1[i] = 1 / (1 - u3[i - 2] * l1[i] - u2[i - 1] * l2[i]);
+    } else if (i == 1) {
+      l2[i] = lo1 * u1[i - 1];
+      u1[i] = 1 / (1 - u2[i - 1] * l2[i]);
+    } else {
+      u1[i] = 1;
+    }
+    if (i >= 1) u2[i] = up1 - u3[i - 1] * l2[i];
+    else u2[i] = up1;
+    if (set_up2) u3[i] = up2;
+  };
+  pivot(0, 0, 0, head[0][3], head[0][4], true);
+  pivot(1, 0, head[1][1], head[1][3], head[1][4], true);
+  pivot(2, head[2][0], head[2][1], head[2][3], head[2][4], true);
+  for (int i = 3; i <= n - 4; i++) pivot(i, be, al, al, be, true);
+  // mirrored tail rows: row n-1-r has entries tail[r][4-c] at column n-1-r+(c-2)
+  pivot(n - 3, tail[2][4], tail[2][3], tail[2][1], tail[2][0], true);
+  pivot(n - 2, tail[1][4], tail[1][3], tail[1][1], 0, false);
+  // last row: only the pivot; the upper entries stay unset as in the reference
+  {
+    const int i = n - 1;
+    l1[i] = tail[0][4] * u1[i - 2];
+    l2[i] = (tail[0][3] - u2[i - 2] * l1[i]) * u1[i - 1];
+    u1[i] = 1 / (1 - u3[i - 2] * l1[i] - u2[i - 1] * l2[i]);
+  }
+  for (int i = 0; i < n; i++) { u2[i] = u2[i] * u1[i]; u3[i] = u3[i] * u1[i]; }
+}
+
+// Gauss-Jordan inverse with row pivoting (row-major n x n); returns the
+// reciprocal of the last pivot
+double gj_inverse(int n, const std::vector<double> &a, std::vector<double> &inv) {
+  std::vector<double> w(a);
+  std::vector<int> perm(n);
+  std::vector<double> col(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  double rp = 0;
+  for (int i = 0; i < n; i++) {
+    int m = i;
+    double best = -1;
+    for (int r = i; r < n; r++) {
+      double v = std::fabs(w[r * n + i]);
+      if (v > best) { best = v; m = r; }
+    }
+    if (m != i) {
+      std::swap(perm[m], perm[i]);
+      for (int j = 0; j < n; j++) std::swap(w[m * n + j], w[i * n + j]);
+    }
+    rp = 1 / w[i * n + i];
+    for (int r = 0; r < n; r++) col[r] = w[r * n + i];
+    for (int j = 0; j < n; j++) {
+      const double f = rp * w[i * n + j];
+      for (int r = 0; r < n; r++) w[r * n + j] = w[r * n + j] - f * col[r];
+      w[i * n + j] = f;
+    }
+    for (int r = 0; r < n; r++) w[r * n + i] = -rp * col[r];
+    w[i * n + i] = rp;
+  }
+  inv.assign(n * n, 0.0);
+  for (int r = 0; r < n; r++)
+    for (int j = 0; j < n; j++) inv[r * n + perm[j]] = w[r * n + j];
+  return rp;
+}
+
+std::vector<double> matmul(int n, const std::vector<double> &a, const std::vector<double> &b) {
+  std::vector<double> c(n * n);
+  for (int r = 0; r < n; r++)
+    for (int j = 0; j < n; j++) {
+      double s = 0;
+      for (int k = 0; k < n; k++) s = s + a[r * n + k] * b[k * n + j];
+      c[r * n + j] = s;
+    }
+  return c;
+}
+
+// quasi-disjoint closure rows for a line split into two halves: build the
+// coupled 2(ll+1) system A f' = B f, decouple the two halves in A (R), and
+// read the first two rows of the right half of R A^-1 B
+void closure_rows(double fk, CaseCoef &c) {
+  std::memset(c.pbci, 0, sizeof(c.pbci));
+  std::memset(c.pbco, 0, sizeof(c.pbco));
+  for (int nt = 0; nt <= 1; nt++) {
+    const int ll = nt == 0 ? LMD : LMF, n = 2 * (ll + 1), sgn = 2 * nt - 1;
+    std::vector<double> A(n * n, 0.0), B(n * n, 0.0);
+    auto at = [&](std::vector<double> &M, int r, int col) -> double & { return M[(r - 1) * n + col - 1]; };
+    if (nt == 0) {
+      double s[3];
+      for (int r = 0; r < 3; r++) { s[r] = 0; for (int q = 0; q < 6; q++) s[r] = s[r] + kAbc[r][q]; }
+      for (int q = 0; q < 3; q++) at(A, 1, 1 + q) = kBnd[0][2 + q];
+      at(B, 1, 1) = -s[0];
+      for (int q = 0; q < 6; q++) at(B, 1, 2 + q) = kAbc[0][q];
+      for (int q = 0; q < 4; q++) at(A, 2, 1 + q) = kBnd[1][1 + q];
+      at(B, 2, 1) = kAbc[1][0]; at(B, 2, 2) = -s[1];
+      for (int q = 1; q < 6; q++) at(B, 2, 2 + q) = kAbc[1][q];
+      for (int q = 0; q < 5; q++) at(A, 3, 1 + q) = kBnd[2][q];
+      at(B, 3, 1) = kAbc[2][0]; at(B, 3, 2) = kAbc[2][1]; at(B, 3, 3) = -s[2];
+      for (int q = 2; q < 6; q++) at(B, 3, 2 + q) = kAbc[2][q];
+      for (int i = 4; i <= n - 3; i++) {
+        const double va[5] = {kBeta, kAlpha, 1, kAlpha, kBeta};
+        const double vb[7] = {-kAC, -kAB, -kAA, 0, kAA, kAB, kAC};
+        for (int q = 0; q < 5; q++) at(A, i, i - 2 + q) = va[q];
+        for (int q = 0; q < 7; q++) at(B, i, i - 3 + q) = vb[q];
+      }
+    } else {
+      FiltCoef z = filter_coef(fk, 0.5);
+      const double al = z.alpha, be = z.beta, al2 = al * al, al3 = al2 * al, be2 = be * be, be3 = be2 * be;
+      const double zv[5] = {z.b + 5 * z.c, z.a - 10 * z.c, z.a - 5 * z.c, z.b + z.c, z.c};
+      double f = 1 / (1 + 5 * al3 + al2 * (8 + 22 * be) + be * (5 + 4 * be + 60 * be2) +
+                      5 * al * (1 + be * (3 + 10 * be)));
+      at(A, 1, 1) = 1;
+      at(A, 1, 2) = (al * (1 + al) * (1 + 4 * al) + 2 * al * (7 + 3 * al) * be + 24 * (1 - al) * be2 - 80 * be3) * f;
+      at(A, 1, 3) = (al3 + be + 14 * al2 * be + 60 * be3 + al * be * (3 + 46 * be)) * f;
+      f = 1 / (1 + al2 * (9 - 5 * be) + al * (5 + (35 - 29 * be) * be) + be * (5 + 6 * (9 - 10 * be) * be));
+      at(A, 2, 1) = (9 * al3 + 10 * be2 * (8 * be - 1) + 5 * al2 * (1 + 8 * be) + al * (1 + be * (4 + 81 * be))) * f;
+      at(A, 2, 2) = 1;
+      at(A, 2, 3) = (al * (1 + al * (5 + 9 * al)) + al * (5 + 36 * al) * be + (55 * al - 1) * be2 + 10 * be3) * f;
+      at(A, 2, 4) = be * (1 + al * (5 + 9 * al) + 5 * be + 35 * al * be + 50 * be2) * f;
+      const double r3[5] = {be, al, 1, al, be};
+      for (int q = 0; q < 5; q++) at(A, 3, 1 + q) = r3[q];
+      double zs = 0;
+      for (int q = 0; q < 5; q++) zs = zs + zv[q];
+      const double b3[6] = {zv[0], zv[1], -zs, zv[2], zv[3], zv[4]};
+      for (int q = 0; q < 6; q++) at(B, 3, 1 + q) = b3[q];
+      for (int i = 4; i <= n - 3; i++) {
+        const double vb[7] = {z.c, z.b, z.a, -2 * (z.a + z.b + z.c), z.a, z.b, z.c};
+        for (int q = 0; q < 5; q++) at(A, i, i - 2 + q) = r3[q];
+        for (int q = 0; q < 7; q++) at(B, i, i - 3 + q) = vb[q];
+      }
+    }
+    for (int col = 1; col <= n; col++) {
+      for (int r = 1; r <= 3; r++) {
+        at(A, n + 1 - r, n + 1 - col) = at(A, r, col);
+        at(B, n + 1 - r, n + 1 - col) = sgn * at(B, r, col);
+      }
+    }
+    std::vector<double> Ai;
+    gj_inverse(n, A, Ai);
+    std::vector<double> R(A);
+    int i = n / 2 - 1;
+    at(R, i, i + 2) = 0; at(R, i + 1, i + 2) = 0; at(R, i + 1, i + 3) = 0;
+    i = n / 2 + 2;
+    at(R, i, i - 2) = 0; at(R, i - 1, i - 2) = 0; at(R, i - 1, i - 3) = 0;
+    std::vector<double> M = matmul(n, R, matmul(n, Ai, B));
+    for (int row = 0; row <= 1; row++) {
+      const int r = n / 2 + 1 + row;
+      for (int q = 0; q <= ll; q++) {
+        c.pbco[nt][row][ll - q] = M[(r - 1) * n + q];
+        c.pbci[nt][row][q] = M[(r - 1) * n + ll + 1 + q];
+      }
+    }
+  }
+  for (int nt = 0; nt <= 1; nt++)
+    for (int row = 0; row <= 1; row++) {
+      double s = 0;
+      for (int q = 0; q <= (nt == 0 ? LMD : LMF); q++) s = s + c.pbco[nt][row][q];
+      c.pbcot[nt][row] = s;
+    }
+}
+
+// GCBC spreading weights over the MBCI points inside a characteristic face
+double gcbc_weights(double sbci[MBCI]) {
+  std::vector<double> m = {kBnd[0][3], kBnd[0][4], 0, kBnd[1][2], kBnd[1][3], kBnd[1][4],
+                           kBnd[2][1], kBnd[2][2], kBnd[2][3]};
+  const double rhs[3] = {1, kBnd[1][1], kBnd[2][0]};
+  std::vector<double> inv;
+  double last = gj_inverse(3, m, inv);
+  for (int i = 0; i < MBCI; i++) {
+    double s = 0;
+    for (int k = 0; k < 3; k++) s = s + inv[i * 3 + k] * rhs[k];
+    sbci[i] = -s;
+  }
+  const double f = kPi / (MBCI + 1);
+  double r = 0;
+  for (int i = 0; i < MBCI; i++) {
+    r = r + 1;
+    sbci[i] = 0.5 * sbci[i] * (1 + std::cos(r * f));
+  }
+  return last;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// context
+// ----------------------------------------------------------------------------
+struct HostCase {
+  int nb = 0, block0 = 0, njct = 0;
+  std::vector<int> dims, nbc, ncd, nsz, jct, jvel;
+  int ip[TP_I_NPARAM] = {0};
+  double dp[TP_D_NPARAM] = {0};
+  CaseCoef coef;
+  double cbca_ra0 = 0;
+  double albed[2][3][5], albef[2][3][5], alphf = 0, betf = 0;
+};
+
+struct DevList {
+  Work *d = nullptr;
+  int n = 0;
+};
+
+struct ProfRec {
+  hipEvent_t a, b;
+  double bytes;
+};
+
+struct tp_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool finalized = false;
+  std::vector<HostCase> cases;
+  std::vector<Block> blocks;          // host copies of the device views
+  std::vector<std::vector<double>> grids;
+  std::vector<int> sponge_n;
+  Block *d_blocks = nullptr;
+  CaseCoef *d_coef = nullptr;
+  double *d_sc = nullptr, *d_part = nullptr;
+  int *d_wr = nullptr, *d_jct = nullptr, *d_jvel = nullptr;
+  int njct = 0, maxk = 0;
+  std::vector<int> wr;                // point-work ranges per case
+  std::vector<void *> allocs;
+  size_t dev_bytes = 0;
+  DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  int nlp[3] = {0, 0, 0};
+  size_t lds[3] = {0, 0, 0};
+  long long npts = 0;
+  bool any_visc = false;
+  // measurement
+  bool prof = false;
+  std::vector<ProfRec> pend[K_NCLASS];
+  std::vector<hipEvent_t> pool;
+  double ms[K_NCLASS] = {0}, bytes[K_NCLASS] = {0};
+  int64_t cnt[K_NCLASS] = {0};
+  // graph of one step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+int dalloc(tp_context *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  HIPCHK(hipMalloc(p, bytes));
+  c->allocs.push_back(*p);
+  c->dev_bytes += bytes;
+  return 0;
+}
+
+template <class T>
+int upload_vec(tp_context *c, T **dptr, const std::vector<T> &v) {
+  int rc = dalloc(c, (void **)dptr, sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpy(*dptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int upload_list(tp_context *c, DevList &L, const std::vector<Work> &v) {
+  L.n = (int)v.size();
+  return upload_vec(c, &L.d, v);
+}
+
+hipEvent_t take_event(tp_context *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct Timed {
+  tp_context *c;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr;
+  Timed(tp_context *cc, int k, double by) : c(cc), cls(k), bytes(by) {
+    if (c->prof) {
+      a = take_event(c);
+      if (a) hipEventRecord(a, c->stream);
+    }
+  }
+  ~Timed() {
+    if (c->prof && a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        hipEventRecord(b, c->stream);
+        c->pend[cls].push_back({a, b, bytes});
+      }
+    }
+  }
+};
+
+// ---------------- launch helpers ----------------
+void run_points(tp_context *c, int which, StageArgs a, int cls, double bytes_per_point) {
+  if (!c->points.n) return;
+  Timed t(c, cls, bytes_per_point * (double)c->npts);
+  a.work = c->points.d;
+  launch_point(which, c->points.n, c->stream, a);
+}
+
+void run_faces(tp_context *c, const DevList &L, int which, StageArgs a, int cls) {
+  if (!L.n) return;
+  Timed t(c, cls, 0.0);
+  a.work = L.d;
+  launch_face(which, L.n, c->stream, a);
+}
+
+StageArgs base_args(tp_context *c) {
+  StageArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.blocks = c->d_blocks;
+  a.coef = c->d_coef;
+  a.sc = c->d_sc;
+  a.part = c->d_part;
+  return a;
+}
+
+void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0) {
+  if (!c->exch.n) return;
+  Timed t(c, K_EXCH, 0.0);
+  ExchArgs a;
+  a.blocks = c->d_blocks; a.work = c->exch.d; a.coef = c->d_coef;
+  a.nt = nt; a.nrt = nrt; a.n45 = n45; a.arr = arr; a.col0 = col0;
+  launch_exchange(c->exch.n, c->stream, a);
+}
+
+void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_arr, int out_col, int m,
+              int acc) {
+  if (!c->lines[d].n) return;
+  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, 16.0 * (double)c->npts);
+  LineArgs a;
+  a.blocks = c->d_blocks; a.work = c->lines[d].d; a.coef = c->d_coef;
+  a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
+  a.m = m; a.acc = acc; a.nlp = c->nlp[d];
+  if (kind == 0) launch_deriv(d, c->lines[d].n, c->lds[d], c->stream, a);
+  else launch_filter(d, c->lines[d].n, c->lds[d], c->stream, a);
+}
+
+void enqueue_filter_step(tp_context *c) {
+  // src/Main3D.f90:555-561
+  for (int m = 1; m <= 5; m++) {
+    run_exchange(c, 1, 1, 0, F_QA, m - 1);
+    for (int d = 0; d < 3; d++) run_line(c, 1, d, F_QA, m - 1, F_QA, m - 1, 0, 0);
+  }
+  run_points(c, PK_COPY5, base_args(c), K_MISC, 80.0);
+}
+
+void enqueue_stage(tp_context *c, int nk) {
+  const int ncase = (int)c->cases.size();
+  launch_sc_stage(c->d_coef, c->d_sc, ncase, nk, c->stream);
+  StageArgs a = base_args(c);
+  a.nk = nk;
+  run_points(c, PK_PRIM, a, K_POINT, 96.0);
+  if (nk == 1) {
+    run_points(c, PK_DTRED, a, K_POINT, 0.0);
+    launch_sc_dt(c->d_coef, c->d_sc, c->d_part, c->d_wr, ncase, 1, c->stream);
+  }
+  if (c->any_visc) {
+    // src/Main3D.f90:627-651: gradients of u, v, w, T combined with the metrics
+    // tv slots: 0 txx 1 tyy 2 tzz 3 txy 4 tyz 5 tzx 6 hxx 7 hyy 8 hzz; -1..-3 ss
+    const int slots[4][3] = {{0, 8, 5}, {3, 1, 6}, {7, 4, 2}, {-1, -2, -3}};
+    for (int f = 0; f < 4; f++) {
+      run_exchange(c, 0, 1, 0, F_DE, f + 1);
+      run_line(c, 0, 2, F_DE, f + 1, F_RR, 2, 0, 0);
+      run_line(c, 0, 1, F_DE, f + 1, F_RR, 1, 0, 0);
+      run_line(c, 0, 0, F_DE, f + 1, F_RR, 0, 0, 0);
+      StageArgs g = a;
+      g.o0 = slots[f][0]; g.o1 = slots[f][1]; g.o2 = slots[f][2];
+      run_points(c, PK_GRAD, g, K_VISC, 120.0);
+    }
+    run_points(c, PK_STRESS, a, K_VISC, 200.0);
+  }
+  run_points(c, PK_CONTRA, a, K_FLUX, 120.0);
+  for (int m = 1; m <= 5; m++) {
+    StageArgs f = a;
+    f.m = m;
+    run_points(c, PK_FLUX, f, K_FLUX, 150.0);
+    run_exchange(c, 0, 0, 0, F_RR, 0);
+    // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
+    run_line(c, 0, 0, F_RR, 0, F_DE, m - 1, m, 0);
+    run_line(c, 0, 1, F_RR, 1, F_DE, m - 1, m, 1);
+    run_line(c, 0, 2, F_RR, 2, F_DE, m - 1, m, 1);
+  }
+  if (c->sponge.n) {
+    Timed t(c, K_MISC, 0.0);
+    StageArgs s = a;
+    s.work = c->sponge.d;
+    launch_point(PK_SPONGE, c->sponge.n, c->stream, s);
+  }
+  run_faces(c, c->gprep, FK_GCBC_PREP, a, K_GCBC);
+  run_faces(c, c->gcopy, FK_COPY, a, K_GCBC);
+  for (int d = 0; d < 3; d++) run_faces(c, c->gapply[d], FK_GCBC_APPLY, a, K_GCBC);
+  launch_sc_rk(c->d_coef, c->d_sc, ncase, nk, c->stream);
+  run_points(c, PK_RK, a, K_RK, 128.0);
+  run_faces(c, c->wall, FK_WALL, a, K_GCBC);
+  if (c->njct) launch_junction(c->d_blocks, c->d_jct, c->d_jvel, c->njct, c->maxk, c->stream);
+  run_faces(c, c->acap, FK_AVG_CAP, a, K_GCBC);
+  run_faces(c, c->acap, FK_COPY, a, K_GCBC);
+  for (int d = 0; d < 3; d++) run_faces(c, c->aapply[d], FK_AVG_APPLY, a, K_GCBC);
+}
+
+void enqueue_step(tp_context *c) {
+  enqueue_filter_step(c);
+  const int nkrk = c->cases[0].ip[TP_I_NKRK];
+  for (int nk = 1; nk <= nkrk; nk++) enqueue_stage(c, nk);
+  launch_sc_end(c->d_sc, (int)c->cases.size(), nullptr, c->stream);
+}
+
+int check_ready(tp_context *c) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  if (!c->finalized) return fail(TP_E_STATE, "context not finalized (call tp_finalize)");
+  return 0;
+}
+
+double *array_ptr(const Block &b, int id, int64_t *count) {
+  const int64_t N = b.N;
+  switch (id) {
+    case TP_QA: *count = 5 * N; return b.qa;
+    case TP_QO: *count = 5 * N; return b.qo;
+    case TP_DE: *count = 5 * N; return b.de;
+    case TP_RR: *count = 3 * N; return b.rr;
+    case TP_SS: *count = 3 * N; return b.ss;
+    case TP_XIM: *count = 3 * N; return b.xim;
+    case TP_ETM: *count = 3 * N; return b.etm;
+    case TP_ZEM: *count = 3 * N; return b.zem;
+    case TP_P: *count = N; return b.p;
+    case TP_YACO: *count = N; return b.yaco;
+    case TP_TV: *count = 9 * N; return b.tv;
+    case TP_DRVA1: case TP_DRVA2: case TP_DRVA3: *count = 10LL * b.F[id - TP_DRVA1]; return b.drva[id - TP_DRVA1];
+    case TP_DRVB1: case TP_DRVB2: case TP_DRVB3: *count = 10LL * b.F[id - TP_DRVB1]; return b.drvb[id - TP_DRVB1];
+    case TP_RECV1: case TP_RECV2: case TP_RECV3: *count = 6LL * b.F[id - TP_RECV1]; return b.recv[id - TP_RECV1];
+    case TP_CM1: case TP_CM2: case TP_CM3: *count = 6LL * b.F[id - TP_CM1]; return b.cm[id - TP_CM1];
+    case TP_XL: *count = 2LL * b.nlu; return b.lud;
+    case TP_XU: *count = 3LL * b.nlu; return b.lud + 2 * b.nlu;
+    case TP_YL: *count = 2LL * b.nlu; return b.luf;
+    case TP_YU: *count = 3LL * b.nlu; return b.luf + 2 * b.nlu;
+    case TP_RPEX: {
+      int n = 0;
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++)
+          if (b.nbc[d][s] == 20 || b.nbc[d][s] == 25) n += b.F[d];
+      *count = n;
+      return b.rpex;
+    }
+  }
+  *count = 0;
+  return nullptr;
+}
+
+int block_index(tp_context *c, int cs, int b) {
+  if (cs < 0 || cs >= (int)c->cases.size()) return -1;
+  if (b < 0 || b >= c->cases[cs].nb) return -1;
+  return c->cases[cs].block0 + b;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------
+extern "C" {
+
+const char *tp_last_error(void) { return g_err.c_str(); }
+
+int32_t tp_create(tp_context **out) {
+  if (!out) return fail(TP_E_ARG, "null output pointer");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(TP_E_HIP, "no HIP device available: the MI355X time stepper has no CPU path");
+  tp_context *c = new tp_context();
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return fail(TP_E_HIP, "hipGetDevice failed"); }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(TP_E_HIP, "hipStreamCreate failed");
+  }
+  *out = c;
+  return 0;
+}
+
+int32_t tp_destroy(tp_context *c) {
+  if (!c) return 0;
+  hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  for (int k = 0; k < K_NCLASS; k++)
+    for (auto &r : c->pend[k]) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  for (auto e : c->pool) hipEventDestroy(e);
+  for (void *p : c->allocs) hipFree(p);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_t *nbc, const int32_t *ncd,
+                    const int32_t *nsz, int32_t njct, const int32_t *jct, const int32_t *jvel,
+                    const int32_t *iparams, const double *dparams, int32_t *case_id) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  if (c->finalized) return fail(TP_E_STATE, "cases must be added before tp_finalize");
+  if (nb <= 0 || !dims || !nbc || !ncd || !nsz || !iparams || !dparams)
+    return fail(TP_E_ARG, "tp_add_case: missing arrays");
+  if ((int)c->cases.size() >= MAX_CASE) return fail(TP_E_SIZE, "too many cases");
+  HostCase h;
+  h.nb = nb;
+  h.njct = njct;
+  h.dims.assign(dims, dims + 3 * nb);
+  h.nbc.assign(nbc, nbc + 6 * nb);
+  h.ncd.assign(ncd, ncd + 6 * nb);
+  h.nsz.assign(nsz, nsz + 6 * nb);
+  if (njct > 0) {
+    if (!jct || !jvel) return fail(TP_E_ARG, "junction arrays missing");
+    h.jct.assign(jct, jct + 12 * njct);
+    h.jvel.assign(jvel, jvel + 2 * njct);
+  }
+  std::memcpy(h.ip, iparams, sizeof(h.ip));
+  std::memcpy(h.dp, dparams, sizeof(h.dp));
+  // validate the topology: line lengths the stencils and closures need, and
+  // neighbours that exist and match
+  for (int b = 0; b < nb; b++) {
+    for (int d = 0; d < 3; d++) {
+      const int L = dims[3 * b + d] + 1;
+      int need = 7;
+      for (int s = 0; s < 2; s++) {
+        const int np = nbc[6 * b + 2 * d + s];
+        if (np != 10 && np != 20 && np != 25 && np != 30 && np != 35 && np != 40 && np != 45)
+          return fail(TP_E_LAYOUT, "block " + std::to_string(b) + ": unknown boundary code " + std::to_string(np));
+        if (np == 40) need = std::max(need, LMD + 1);
+        if (np == 35 || np == 45) need = std::max(need, LMD + 2);
+        const int nbr = ncd[6 * b + 2 * d + s];
+        const bool exchanges = np == 30 || np == 35 || np == 40 || np == 45 ||
+                               ((np == 20 || np == 25) && iparams[TP_I_NEXTGCIC] == 1);
+        if (exchanges) {
+          if (nbr < 0 || nbr >= nb)
+            return fail(TP_E_LAYOUT, "block " + std::to_string(b) + " face " + std::to_string(2 * d + s) +
+                                         " needs a neighbour");
+          for (int e = 0; e < 3; e++)
+            if (e != d && dims[3 * nbr + e] != dims[3 * b + e])
+              return fail(TP_E_LAYOUT, "block " + std::to_string(b) + " and neighbour " + std::to_string(nbr) +
+                                           " faces differ in size");
+        }
+      }
+      if (L < need)
+        return fail(TP_E_LAYOUT, "block " + std::to_string(b) + " direction " + std::to_string(d + 1) + " has " +
+                                     std::to_string(L) + " points; its faces need >= " + std::to_string(need));
+    }
+  }
+  if (!c->cases.empty() && c->cases[0].ip[TP_I_NKRK] != iparams[TP_I_NKRK])
+    return fail(TP_E_ARG, "all cases of a context must use the same number of RK stages");
+  if (iparams[TP_I_NKRK] < 1) return fail(TP_E_ARG, "nkrk must be >= 1");
+  h.block0 = c->cases.empty() ? 0 : c->cases.back().block0 + c->cases.back().nb;
+  // coefficients, src/Main3D.f90:60-63, 71-73, 295-312
+  CaseCoef &k = h.coef;
+  std::memset(&k, 0, sizeof(k));
+  std::memcpy(k.abc, kAbc, sizeof(kAbc));
+  const double fltk = kPi * dparams[TP_D_FLTK], fltkbc = kPi * dparams[TP_D_FLTKBC];
+  for (int r = 0; r < 3; r++)
+    for (int q = 0; q < 5; q++) {
+      h.albed[0][r][q] = kBnd[r][q];
+      const double inner[3][5] = {{0, 0, 1, kAlpha, kBeta}, {0, kAlpha, 1, kAlpha, kBeta},
+                                  {kBeta, kAlpha, 1, kAlpha, kBeta}};
+      h.albed[1][r][q] = inner[r][q];
+    }
+  filter_boundary(fltk, fltkbc, h.albef[0], k.fam, k.fbm, k.fcm);
+  FiltCoef fi = filter_coef(fltk, 0.5);
+  k.fa = fi.a; k.fb = fi.b; k.fc = fi.c;
+  h.alphf = fi.alpha; h.betf = fi.beta;
+  {
+    const double inner[3][5] = {{0, 0, 1, fi.alpha, fi.beta}, {0, fi.alpha, 1, fi.alpha, fi.beta},
+                                {fi.beta, fi.alpha, 1, fi.alpha, fi.beta}};
+    std::memcpy(h.albef[1], inner, sizeof(inner));
+  }
+  closure_rows(fltk, k);
+  h.cbca_ra0 = gcbc_weights(k.sbci);
+  k.fex[0] = 45 / 30.0; k.fex[1] = -9 / 30.0; k.fex[2] = 1 / 30.0;
+  const double l2 = 8 * std::log(2.0);
+  k.ffx[0] = 4 / l2; k.ffx[1] = 6 / l2; k.ffx[2] = 7 / l2;
+  const double gam = 1.4, poo = 1 / gam, aoo = std::sqrt(gam * poo / 1.0);
+  k.srefoo = 111 / dparams[TP_D_TEMPOO];
+  k.srefp1dre = (k.srefoo + 1) / dparams[TP_D_REOO];
+  k.cfl = dparams[TP_D_CFL]; k.dto = dparams[TP_D_DTO]; k.wtemp = dparams[TP_D_WTEMP];
+  k.smago1 = dparams[TP_D_SMAGO1]; k.smago2 = dparams[TP_D_SMAGO2];
+  k.timf = dparams[TP_D_TIMF]; k.tp = dparams[TP_D_TP]; k.tps = dparams[TP_D_TPS];
+  k.aoa0 = dparams[TP_D_AOA0]; k.aoa1 = dparams[TP_D_AOA1];
+  const double m1 = dparams[TP_D_AMACH1], m2 = dparams[TP_D_AMACH2], m3 = dparams[TP_D_AMACH3];
+  k.amachoo = std::sqrt(m1 * m1 + m2 * m2 + m3 * m3);
+  k.uoo[0] = m1 * aoo; k.uoo[1] = m2 * aoo; k.uoo[2] = m3 * aoo;
+  k.nkrk = iparams[TP_I_NKRK]; k.nviscous = iparams[TP_I_NVISCOUS]; k.nsmf = iparams[TP_I_NSMF];
+  k.les = iparams[TP_I_LES]; k.nextgcic = iparams[TP_I_NEXTGCIC];
+  if (case_id) *case_id = (int)c->cases.size();
+  c->cases.push_back(std::move(h));
+  return 0;
+}
+
+int32_t tp_finalize(tp_context *c) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  if (c->finalized) return fail(TP_E_STATE, "already finalized");
+  if (c->cases.empty()) return fail(TP_E_STATE, "no case added");
+  HIPCHK(hipSetDevice(c->device));
+  const int ncase = (int)c->cases.size();
+  // ---- blocks and their device memory ----
+  for (int cs = 0; cs < ncase; cs++) {
+    HostCase &h = c->cases[cs];
+    if (h.ip[TP_I_NVISCOUS] == 1) c->any_visc = true;
+    for (int lb = 0; lb < h.nb; lb++) {
+      Block b;
+      std::memset(&b, 0, sizeof(b));
+      b.nx = h.dims[3 * lb] + 1; b.ny = h.dims[3 * lb + 1] + 1; b.nz = h.dims[3 * lb + 2] + 1;
+      b.N = b.nx * b.ny * b.nz;
+      b.L[0] = b.nx; b.L[1] = b.ny; b.L[2] = b.nz;
+      b.F[0] = b.ny * b.nz; b.F[1] = b.nz * b.nx; b.F[2] = b.nx * b.ny;
+      b.cs = cs;
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++) {
+          b.nbc[d][s] = h.nbc[6 * lb + 2 * d + s];
+          const int nbr = h.ncd[6 * lb + 2 * d + s];
+          b.ncd[d][s] = nbr >= 0 ? h.block0 + nbr : -1;
+          const int np = b.nbc[d][s];
+          b.ndf[d][s] = (np == 35 || np == 40 || np == 45) ? 1 : 0;
+        }
+      b.nlu = b.nx + b.ny + b.nz;
+      b.lu_off[0] = 0; b.lu_off[1] = b.nx; b.lu_off[2] = b.nx + b.ny;
+      int nwall = 0;
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++) {
+          if (b.nbc[d][s] == 20 || b.nbc[d][s] == 25) { b.wall_off[d][s] = nwall; nwall += b.F[d]; }
+          else b.wall_off[d][s] = -1;
+        }
+      // one allocation per block, 256-byte aligned sub-arrays
+      size_t off = 0;
+      auto take = [&](size_t n) { size_t o = off; off += ((n * sizeof(double) + 255) / 256) * 256; return o; };
+      const size_t N = b.N;
+      size_t o_qa = take(5 * N), o_qo = take(5 * N), o_de = take(5 * N), o_rr = take(3 * N), o_ss = take(3 * N);
+      size_t o_xi = take(3 * N), o_et = take(3 * N), o_ze = take(3 * N), o_p = take(N), o_ya = take(N);
+      size_t o_tv = take(9 * N);
+      size_t o_f[3][4];
+      for (int d = 0; d < 3; d++) {
+        o_f[d][0] = take(10 * (size_t)b.F[d]); o_f[d][1] = take(10 * (size_t)b.F[d]);
+        o_f[d][2] = take(6 * (size_t)b.F[d]); o_f[d][3] = take(6 * (size_t)b.F[d]);
+      }
+      size_t o_lud = take(5 * (size_t)b.nlu), o_luf = take(5 * (size_t)b.nlu), o_rp = take(std::max(nwall, 1));
+      char *base = nullptr;
+      int rc = dalloc(c, (void **)&base, off);
+      if (rc) return rc;
+      HIPCHK(hipMemset(base, 0, off));
+      auto P = [&](size_t o) { return (double *)(base + o); };
+      b.qa = P(o_qa); b.qo = P(o_qo); b.de = P(o_de); b.rr = P(o_rr); b.ss = P(o_ss);
+      b.xim = P(o_xi); b.etm = P(o_et); b.zem = P(o_ze); b.p = P(o_p); b.yaco = P(o_ya); b.tv = P(o_tv);
+      for (int d = 0; d < 3; d++) {
+        b.drva[d] = P(o_f[d][0]); b.drvb[d] = P(o_f[d][1]); b.recv[d] = P(o_f[d][2]); b.cm[d] = P(o_f[d][3]);
+      }
+      b.lud = P(o_lud); b.luf = P(o_luf); b.rpex = P(o_rp);
+      // LU factors of the derivative and filter matrices, src/Main3D.f90:316-332
+      std::vector<double> lud(5 * b.nlu), luf(5 * b.nlu);
+      for (int d = 0; d < 3; d++) {
+        const int n = b.L[d], o = b.lu_off[d], nl = b.nlu;
+        const int ns = b.ndf[d][0], ne = b.ndf[d][1];
+        band_lu(n, h.albed[ns], h.albed[ne], kAlpha, kBeta, &lud[0 * nl + o], &lud[1 * nl + o], &lud[2 * nl + o],
+                &lud[3 * nl + o], &lud[4 * nl + o]);
+        band_lu(n, h.albef[ns], h.albef[ne], h.alphf, h.betf, &luf[0 * nl + o], &luf[1 * nl + o], &luf[2 * nl + o],
+                &luf[3 * nl + o], &luf[4 * nl + o]);
+      }
+      HIPCHK(hipMemcpy(b.lud, lud.data(), sizeof(double) * lud.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(b.luf, luf.data(), sizeof(double) * luf.size(), hipMemcpyHostToDevice));
+      c->blocks.push_back(b);
+      c->grids.emplace_back();
+      c->npts += b.N;
+    }
+  }
+  const int nb = (int)c->blocks.size();
+  int rc;
+  if ((rc = upload_vec(c, &c->d_blocks, c->blocks))) return rc;
+  std::vector<CaseCoef> coefs;
+  for (auto &h : c->cases) coefs.push_back(h.coef);
+  if ((rc = upload_vec(c, &c->d_coef, coefs))) return rc;
+  std::vector<double> sc(ncase * S_NSCALAR, 0.0);
+  for (int cs = 0; cs < ncase; cs++) sc[cs * S_NSCALAR + S_AOA] = c->cases[cs].dp[TP_D_AOA0];
+  if ((rc = upload_vec(c, &c->d_sc, sc))) return rc;
+  // ---- work tables ----
+  std::vector<Work> pts;
+  c->wr.assign(ncase + 1, 0);
+  for (int cs = 0; cs < ncase; cs++) {
+    c->wr[cs] = (int)pts.size();
+    const HostCase &h = c->cases[cs];
+    for (int lb = 0; lb < h.nb; lb++) {
+      const int g = h.block0 + lb;
+      const int CH = 1024;
+      for (int a0 = 0; a0 < c->blocks[g].N; a0 += CH) pts.push_back({g, a0, std::min(CH, c->blocks[g].N - a0), 0});
+    }
+  }
+  c->wr[ncase] = (int)pts.size();
+  if ((rc = upload_list(c, c->points, pts))) return rc;
+  std::vector<double> part(2 * std::max<size_t>(pts.size(), 1), 0.0);
+  if ((rc = upload_vec(c, &c->d_part, part))) return rc;
+  if ((rc = upload_vec(c, &c->d_wr, c->wr))) return rc;
+  // line tiles: NL lines of one block per 64-lane workgroup, LDS budget ~40 KB
+  for (int d = 0; d < 3; d++) {
+    int Lmax = 0;
+    for (auto &b : c->blocks) Lmax = std::max(Lmax, b.L[d]);
+    int nl = std::max(1, std::min(64, (int)(40960 / (8 * Lmax)) - 1));
+    if (nl >= 16) nl = (nl / 16) * 16;
+    c->nlp[d] = nl + 1;
+    c->lds[d] = (size_t)Lmax * (nl + 1) * sizeof(double);
+    if (c->lds[d] > 64 * 1024) return fail(TP_E_SIZE, "line too long for the LDS tile");
+    std::vector<Work> lw;
+    for (int g = 0; g < nb; g++) {
+      const Block &b = c->blocks[g];
+      const int nlines = b.N / b.L[d];
+      for (int q = 0; q < nlines; q += nl) lw.push_back({g, q, std::min(nl, nlines - q), d});
+    }
+    if ((rc = upload_list(c, c->lines[d], lw))) return rc;
+  }
+  // face tables
+  std::vector<Work> ex, gp, ga[3], gc, wl, ac, aa[3], nm;
+  for (int g = 0; g < nb; g++) {
+    const Block &b = c->blocks[g];
+    const bool gcic = c->cases[b.cs].ip[TP_I_NEXTGCIC] == 1;
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) {
+        const int np = b.nbc[d][s], F = b.F[d];
+        auto add = [&](std::vector<Work> &v) {
+          for (int a0 = 0; a0 < F; a0 += 256) v.push_back({g, a0, std::min(256, F - a0), d | (s << 8)});
+        };
+        const bool gc_face = np == 10 || np == 20 || np == 25 || np == 30;
+        const bool wall = np == 20 || np == 25;
+        if (b.ndf[d][s]) add(ex);
+        if (gc_face) { add(gp); add(ga[d]); }
+        if (np == 30 || (gcic && wall)) add(gc);
+        if (wall) add(wl);
+        if (np == 30 || np == 35 || np == 45 || (gcic && wall)) { add(ac); add(aa[d]); }
+        add(nm);
+      }
+  }
+  if ((rc = upload_list(c, c->exch, ex))) return rc;
+  if ((rc = upload_list(c, c->gprep, gp))) return rc;
+  if ((rc = upload_list(c, c->gcopy, gc))) return rc;
+  if ((rc = upload_list(c, c->wall, wl))) return rc;
+  if ((rc = upload_list(c, c->acap, ac))) return rc;
+  if ((rc = upload_list(c, c->normals, nm))) return rc;
+  for (int d = 0; d < 3; d++) {
+    if ((rc = upload_list(c, c->gapply[d], ga[d]))) return rc;
+    if ((rc = upload_list(c, c->aapply[d], aa[d]))) return rc;
+  }
+  // junctions
+  std::vector<int> jct, jvel;
+  for (auto &h : c->cases)
+    for (int j = 0; j < h.njct; j++) {
+      for (int q = 0; q < 4; q++) {
+        const int lb = h.jct[12 * j + 3 * q];
+        if (lb < 0 || lb >= h.nb) return fail(TP_E_LAYOUT, "junction member outside its case");
+        if (c->blocks[h.block0 + lb].nz != c->blocks[h.block0 + h.jct[12 * j]].nz)
+          return fail(TP_E_LAYOUT, "junction members differ in zeta size");
+        jct.push_back(h.block0 + lb);
+        jct.push_back(h.jct[12 * j + 3 * q + 1]);
+        jct.push_back(h.jct[12 * j + 3 * q + 2]);
+        c->maxk = std::max(c->maxk, c->blocks[h.block0 + lb].nz);
+      }
+      jvel.push_back(h.jvel[2 * j]);
+      jvel.push_back(h.jvel[2 * j + 1]);
+      c->njct++;
+    }
+  if ((rc = upload_vec(c, &c->d_jct, jct))) return rc;
+  if ((rc = upload_vec(c, &c->d_jvel, jvel))) return rc;
+  c->finalized = true;
+  return 0;
+}
+
+int32_t tp_set_grid(tp_context *c, int32_t cs, int32_t lb, const double *xyz) {
+  if (int rc = check_ready(c)) return rc;
+  const int g = block_index(c, cs, lb);
+  if (g < 0 || !xyz) return fail(TP_E_ARG, "tp_set_grid: bad case/block");
+  const Block &b = c->blocks[g];
+  c->grids[g].assign(xyz, xyz + 3 * (size_t)b.N);
+  HIPCHK(hipMemcpyAsync(b.ss, xyz, sizeof(double) * 3 * b.N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int32_t tp_setup_geometry(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  for (size_t g = 0; g < c->blocks.size(); g++)
+    if (c->grids[g].empty()) return fail(TP_E_STATE, "grid of block " + std::to_string(g) + " not set");
+  // coordinate derivatives, src/Main3D.f90:365-375 (periodic jump correction on)
+  const int dst[3] = {F_QO, F_QA, F_DE};
+  for (int m = 0; m < 3; m++) {
+    run_exchange(c, 0, 1, 1, F_SS, m);
+    run_line(c, 0, 2, F_SS, m, dst[m], 2, 0, 0);
+    run_line(c, 0, 1, F_SS, m, dst[m], 1, 0, 0);
+    run_line(c, 0, 0, F_SS, m, dst[m], 0, 0, 0);
+  }
+  run_points(c, PK_METRICS, base_args(c), K_MISC, 0.0);
+  run_faces(c, c->normals, FK_NORMALS, base_args(c), K_MISC);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // wall-slot flags (SingleTurbulence3D.f90:447-464) and sponge coefficients
+  // (SingleTurbulence3D.f90:209-249) on the host from the grid and Jacobian
+  std::vector<Work> sp;
+  for (size_t g = 0; g < c->blocks.size(); g++) {
+    Block &b = c->blocks[g];
+    const HostCase &h = c->cases[b.cs];
+    const double *x = c->grids[g].data(), *y = x + b.N, *z = y + b.N;
+    std::vector<double> yaco(b.N);
+    HIPCHK(hipMemcpy(yaco.data(), b.yaco, sizeof(double) * b.N, hipMemcpyDeviceToHost));
+    // wall flags in the reference's face order
+    {
+      int64_t cnt;
+      array_ptr(b, TP_RPEX, &cnt);
+      std::vector<double> rp(std::max<int64_t>(cnt, 1), 0.0);
+      double state = h.cbca_ra0;
+      const double wlea = h.dp[TP_D_WLEA], wlew = h.dp[TP_D_WLEW], span = h.dp[TP_D_SPAN];
+      int lp = 0;
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++) {
+          const int np = b.nbc[d][s];
+          if (!(np == 10 || np == 20 || np == 25 || np == 30)) continue;
+          const int fd = d == 0 ? b.ny : d == 1 ? b.nz : b.nx, fo = b.F[d] / fd;
+          for (int kk = 0; kk < fo; kk++)
+            for (int jj = 0; jj < fd; jj++) {
+              if (!(np == 20 || np == 25)) continue;
+              const int l = indx3(b, s * (b.L[d] - 1), jj, kk, d);
+              double flag = 0;
+              if (x[l] < state) {
+                const double slitw = 0.4 * wlea;
+                state = -0.5 + wlea + wlea;
+                const double r1 = -0.5 * span + 0.25 * wlew - 0.5 * slitw, r2 = r1 + slitw;
+                const int ie = (int)((span + 1.0e-6) / wlew) - 1;
+                for (int ii = 0; ii <= ie; ii++) {
+                  const double r3 = ii * wlew, res = (z[l] - (r1 + r3)) * (z[l] - (r2 + r3));
+                  if (res < 0 && h.ip[TP_I_NEXTRABC] == 1) flag = 1;
+                }
+              }
+              rp[lp++] = flag;
+            }
+        }
+      HIPCHK(hipMemcpy(b.rpex, rp.data(), sizeof(double) * rp.size(), hipMemcpyHostToDevice));
+    }
+    // sponge
+    {
+      const double domlen = h.dp[TP_D_DOMLEN], szth1 = h.dp[TP_D_SZTH1], szth2 = h.dp[TP_D_SZTH2];
+      const double szxt = h.dp[TP_D_SZXT], szco = h.dp[TP_D_SZCO];
+      const double p07 = (double)0.7f;   // single-precision literal 0.7e0 of the reference
+      const double szr[2][3] = {{1 / szth1, 1 / szth2, 0}, {1 / (szth1 + szxt), 1 / szth2, 0}};
+      const double szp[2][3] = {{-p07 * domlen + szth1, -p07 * domlen + szth2, 0},
+                                {domlen - szth1, domlen + szxt - szth2, 0}};
+      const int lb = (int)g - h.block0;
+      int nsz[3][2];
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++) nsz[d][s] = h.nsz[6 * lb + 2 * d + s];
+      const double ra1 = 0.25, ra2 = kPi / (2 * domlen);
+      std::vector<int> lst;
+      std::vector<double> as, bs;
+      for (int l = 0; l < b.N; l++) {
+        const double cx[3] = {x[l], y[l], z[l]};
+        double r[3];
+        for (int d = 0; d < 3; d++)
+          r[d] = nsz[d][0] * szr[0][d] * std::fmax(szp[0][d] - cx[d], 0.0) +
+                 nsz[d][1] * szr[1][d] * std::fmax(cx[d] - szp[1][d], 0.0);
+        const double sg = szco * (1 - 0.125 * (1 + std::cos(kPi * r[0])) * (1 + std::cos(kPi * r[1])) *
+                                          (1 + std::cos(kPi * r[2])));
+        const double lm = ra1 + 0.5 * (1 - ra1) * (1 + std::cos(ra2 * (std::fmin(x[l], domlen) + domlen)));
+        if (sg > 0) {
+          const double res = 1 / yaco[l];
+          lst.push_back(l);
+          as.push_back(res * sg * lm);
+          bs.push_back(res * (1 / (1.4 - 1)) * sg);
+        }
+      }
+      int rc;
+      if (!lst.empty()) {
+        if ((rc = upload_vec(c, &b.lcsz, lst))) return rc;
+        if ((rc = upload_vec(c, &b.asz, as))) return rc;
+        if ((rc = upload_vec(c, &b.bsz, bs))) return rc;
+      }
+      b.nsponge = (int)lst.size();
+      for (int a0 = 0; a0 < b.nsponge; a0 += 1024) sp.push_back({(int)g, a0, std::min(1024, b.nsponge - a0), 0});
+    }
+  }
+  HIPCHK(hipMemcpy(c->d_blocks, c->blocks.data(), sizeof(Block) * c->blocks.size(), hipMemcpyHostToDevice));
+  int rc;
+  if ((rc = upload_list(c, c->sponge, sp))) return rc;
+  return 0;
+}
+
+int32_t tp_initialo(tp_context *c, int32_t cs) {
+  if (int rc = check_ready(c)) return rc;
+  if (cs < 0 || cs >= (int)c->cases.size()) return fail(TP_E_ARG, "bad case");
+  StageArgs a = base_args(c);
+  a.work = c->points.d + c->wr[cs];
+  const int n = c->wr[cs + 1] - c->wr[cs];
+  if (n) launch_point(PK_INIT, n, c->stream, a);
+  double s[S_NSCALAR] = {0};
+  s[S_AOA] = c->cases[cs].dp[TP_D_AOA0];
+  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_NSCALAR, s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int64_t tp_array_count(tp_context *c, int32_t cs, int32_t lb, int32_t id) {
+  if (check_ready(c)) return -1;
+  const int g = block_index(c, cs, lb);
+  if (g < 0) return -1;
+  int64_t n = 0;
+  array_ptr(c->blocks[g], id, &n);
+  return n;
+}
+
+int32_t tp_upload(tp_context *c, int32_t cs, int32_t lb, int32_t id, const double *src, int64_t count) {
+  if (int rc = check_ready(c)) return rc;
+  const int g = block_index(c, cs, lb);
+  if (g < 0 || !src) return fail(TP_E_ARG, "tp_upload: bad case/block");
+  int64_t n = 0;
+  double *p = array_ptr(c->blocks[g], id, &n);
+  if (!p) return fail(TP_E_ARG, "tp_upload: unknown array id " + std::to_string(id));
+  if (count != n) return fail(TP_E_SIZE, "tp_upload: expected " + std::to_string(n) + " values");
+  HIPCHK(hipMemcpyAsync(p, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int32_t tp_download(tp_context *c, int32_t cs, int32_t lb, int32_t id, double *dst, int64_t count) {
+  if (int rc = check_ready(c)) return rc;
+  const int g = block_index(c, cs, lb);
+  if (g < 0 || !dst) return fail(TP_E_ARG, "tp_download: bad case/block");
+  int64_t n = 0;
+  double *p = array_ptr(c->blocks[g], id, &n);
+  if (!p) return fail(TP_E_ARG, "tp_download: unknown array id " + std::to_string(id));
+  if (count != n) return fail(TP_E_SIZE, "tp_download: expected " + std::to_string(n) + " values");
+  HIPCHK(hipMemcpyAsync(dst, p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int32_t tp_get_scalars(tp_context *c, int32_t cs, double *s16) {
+  if (int rc = check_ready(c)) return rc;
+  if (cs < 0 || cs >= (int)c->cases.size() || !s16) return fail(TP_E_ARG, "bad case");
+  HIPCHK(hipMemcpyAsync(s16, c->d_sc + cs * S_NSCALAR, sizeof(double) * S_NSCALAR, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int32_t tp_set_scalars(tp_context *c, int32_t cs, const double *s16) {
+  if (int rc = check_ready(c)) return rc;
+  if (cs < 0 || cs >= (int)c->cases.size() || !s16) return fail(TP_E_ARG, "bad case");
+  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_NSCALAR, s16, sizeof(double) * S_NSCALAR, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int32_t tp_mpigo(tp_context *c, int32_t nt, int32_t nrt, int32_t n45) {
+  if (int rc = check_ready(c)) return rc;
+  if (nt < 0 || nt > 1 || nrt < 0 || nrt > 1 || n45 < 0 || n45 > 1) return fail(TP_E_ARG, "tp_mpigo: bad selector");
+  run_exchange(c, nt, nrt, n45, F_RR, 0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_deriv(tp_context *c, int32_t nn, int32_t nz, int32_t m) {
+  if (int rc = check_ready(c)) return rc;
+  if (nn < 1 || nn > 3 || nz < 1 || nz > 3 || m < 0 || m > 5) return fail(TP_E_ARG, "tp_deriv: bad selector");
+  run_line(c, 0, nn - 1, F_RR, nz - 1, F_RR, nn - 1, m, 0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_filte(tp_context *c, int32_t nn, int32_t nz) {
+  if (int rc = check_ready(c)) return rc;
+  if (nn < 1 || nn > 3 || nz < 1 || nz > 3) return fail(TP_E_ARG, "tp_filte: bad selector");
+  run_line(c, 1, nn - 1, F_RR, nz - 1, F_RR, nz - 1, 0, 0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_filter_step(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  enqueue_filter_step(c);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_stage(tp_context *c, int32_t nk) {
+  if (int rc = check_ready(c)) return rc;
+  if (nk < 1 || nk > c->cases[0].ip[TP_I_NKRK]) return fail(TP_E_ARG, "tp_stage: bad stage");
+  enqueue_stage(c, nk);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_step(tp_context *c, int32_t nsteps) {
+  if (int rc = check_ready(c)) return rc;
+  for (int s = 0; s < nsteps; s++) enqueue_step(c);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_step_graph(tp_context *c, int32_t nsteps) {
+  if (int rc = check_ready(c)) return rc;
+  if (!c->gexec) {
+    const bool prof = c->prof;
+    c->prof = false;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    c->prof = prof;
+    if (e != hipSuccess) return fail(TP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  }
+  for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  return 0;
+}
+
+int32_t tp_sync(tp_context *c) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_profile(tp_context *c, int32_t on) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  c->prof = on != 0;
+  return 0;
+}
+
+int32_t tp_profile_read(tp_context *c, int32_t k, double *ms, int64_t *launches, double *bytes, int32_t reset) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  if (k < 0 || k >= K_NCLASS) return fail(TP_E_ARG, "bad kernel class");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < K_NCLASS; q++) {
+    for (auto &r : c->pend[q]) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+      c->ms[q] += t;
+      c->bytes[q] += r.bytes;
+      c->cnt[q] += 1;
+      c->pool.push_back(r.a);
+      c->pool.push_back(r.b);
+    }
+    c->pend[q].clear();
+  }
+  if (ms) *ms = c->ms[k];
+  if (launches) *launches = c->cnt[k];
+  if (bytes) *bytes = c->bytes[k];
+  if (reset)
+    for (int q = 0; q < K_NCLASS; q++) { c->ms[q] = 0; c->cnt[q] = 0; c->bytes[q] = 0; }
+  return 0;
+}
+
+int32_t tp_info(tp_context *c, int64_t *out) {
+  if (!c || !out) return fail(TP_E_ARG, "null argument");
+  out[0] = c->npts;
+  out[1] = (int64_t)c->blocks.size();
+  out[2] = (int64_t)c->cases.size();
+  out[3] = (int64_t)c->dev_bytes;
+  return 0;
+}
+
+}  // extern "C"

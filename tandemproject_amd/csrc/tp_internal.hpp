@@ -1,0 +1,150 @@
+// tp_internal.hpp -- device data model of the MI355X time stepper.
+//
+// One process owns every sub-block of every case.  Each sub-block keeps its
+// FP64 fields in HBM in the reference's layout (a(0:lmx, var), xi fastest,
+// l = (k*(let+1)+j)*(lxi+1)+i; face buffers face(0:F-1, c, 0:1)), so a field
+// column is one contiguous plane-major array and every face exchange is a
+// device-side copy between block buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace tp {
+
+constexpr int LMD = 11;   // deriv closure length - 1 (src/MainVar3D.f90:19)
+constexpr int LMF = 8;    // filter closure length - 1
+constexpr int LMP = 11;
+constexpr int MBCI = 3;   // GCBC spreading depth
+constexpr int MAX_CASE = 4096;
+
+// ---- per-case scalars on the device (src/MainVar3D.f90:139-142) ----
+enum Scalar {
+  S_N = 0, S_NDT, S_DT, S_DTS, S_DTE, S_TIMO, S_UMF1, S_UMF2, S_UMF3,
+  S_DUDT1, S_DUDT2, S_DUDT3, S_DTK, S_DTKO, S_AOA, S_NSCALAR = 16
+};
+
+// ---- per-case constants (scheme coefficients and physics) ----
+struct CaseCoef {
+  double abc[3][6];
+  double pbci[2][2][LMP + 1];
+  double pbco[2][2][LMP + 1];
+  double pbcot[2][2];
+  double fa, fb, fc, fam[3], fbm[3], fcm[3];
+  double sbci[MBCI];
+  double fex[3], ffx[3];
+  double srefoo, srefp1dre, cfl, dto, wtemp, smago1, smago2;
+  double timf, tp, tps, aoa0, aoa1, amachoo, uoo[3];
+  int nkrk, nviscous, nsmf, les, nextgcic, pad;
+};
+
+// ---- device view of one sub-block ----
+struct Block {
+  int nx, ny, nz, N;          // points per direction and total
+  int L[3];                   // line length per direction (nx, ny, nz)
+  int F[3];                   // face size per direction (nbsize)
+  int nbc[3][2], ncd[3][2], ndf[3][2];
+  int cs;                     // case index
+  int lu_off[3];              // offset of direction d in LU tables
+  int nlu;                    // LU table stride (lim + 1)
+  int wall_off[3][2];         // rpex offset of each wall face (-1 none)
+  int nsponge;
+  double *qa, *qo, *de, *rr, *ss, *xim, *etm, *zem, *p, *yaco, *tv;
+  double *drva[3], *drvb[3], *recv[3], *cm[3];
+  double *lud, *luf;          // [5][nlu]: l1, l2, u1, u2, u3 for deriv / filter
+  double *rpex;
+  int *lcsz;
+  double *asz, *bsz;
+};
+
+// array ids exposed through the C ABI (order shared with include/tandem_hip.h)
+enum ArrayId {
+  A_QA = 0, A_QO, A_DE, A_RR, A_SS, A_XIM, A_ETM, A_ZEM, A_P, A_YACO, A_TV,
+  A_DRVA1, A_DRVA2, A_DRVA3, A_DRVB1, A_DRVB2, A_DRVB3, A_SEND1, A_SEND2, A_SEND3,
+  A_RECV1, A_RECV2, A_RECV3, A_CM1, A_CM2, A_CM3, A_XU, A_XL, A_YU, A_YL, A_RPEX,
+  A_NARRAY
+};
+
+// field selector for generic line kernels: (array, column)
+enum FieldArr { F_RR = 0, F_DE, F_QA, F_SS, F_QO };
+
+__host__ __device__ inline double *field_ptr(const Block &b, int arr, int col) {
+  double *base = arr == F_RR ? b.rr : arr == F_DE ? b.de : arr == F_QA ? b.qa
+               : arr == F_SS ? b.ss : b.qo;
+  return base + (size_t)col * b.N;
+}
+
+// linear index of point (i along direction d, face indices j, k), Subroutineso.f90:364-375
+__host__ __device__ inline int indx3(const Block &b, int i, int j, int k, int d) {
+  return d == 0 ? (k * b.ny + j) * b.nx + i
+       : d == 1 ? (j * b.ny + i) * b.nx + k
+                : (i * b.ny + k) * b.nx + j;
+}
+
+// work item of a launch: one workgroup's share
+struct Work {
+  int blk;    // block index
+  int a;      // first element (point / line / face point)
+  int n;      // count
+  int d;      // direction (line and face launches), side in bit 8
+};
+
+// kernel classes timed by the profiler hooks
+enum KClass {
+  K_DERIV = 0, K_FILTER, K_EXCH, K_POINT, K_FLUX, K_VISC, K_GCBC, K_RK, K_MISC, K_NCLASS
+};
+
+
+// ---- kernel argument blocks (filled by the host, passed by value) ----
+struct LineArgs {
+  const Block *blocks;
+  const Work *work;
+  const CaseCoef *coef;
+  int in_arr, in_col;     // source field
+  int out_arr, out_col;   // destination (deriv)
+  int m;                  // drva capture column 1..5, 0 = none
+  int acc;                // deriv: 1 = add into destination
+  int nlp;                // LDS row stride (lines per tile + 1)
+};
+
+struct ExchArgs {
+  const Block *blocks;
+  const Work *work;        // a = first face point, n = count, d = dir | side << 8
+  const CaseCoef *coef;
+  int nt, nrt, n45;
+  int arr, col0;           // source field: column col0 + (1-nrt)*dir
+};
+
+struct StageArgs {
+  const Block *blocks;
+  const Work *work;
+  const CaseCoef *coef;
+  double *sc;        // [ncase][S_NSCALAR]
+  double *part;      // reduction partials [nwork][2]
+  int nk, m, d, opt; // stage number, field, direction, option
+  int arr, col;      // generic field selector
+  int o0, o1, o2;    // tv slots for gradient combination (-1..-3 = ss columns)
+};
+
+// launchers (tp_line.hip, tp_stage.hip)
+void launch_deriv(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a);
+void launch_filter(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a);
+void launch_exchange(int nwg, hipStream_t st, const ExchArgs &a);
+void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a);
+void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a);
+void launch_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st);
+void launch_sc_dt(const CaseCoef *coef, double *sc, const double *part, const int *wr, int ncase,
+                  int reduced, hipStream_t st);
+void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st);
+void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st);
+void launch_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk,
+                     hipStream_t st);
+
+enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_GRAD, PK_STRESS, PK_CONTRA, PK_FLUX, PK_SPONGE, PK_RK,
+                   PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS };
+enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS };
+
+}  // namespace tp

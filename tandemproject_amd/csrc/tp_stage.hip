@@ -1,0 +1,765 @@
+// tp_stage.hip -- pointwise, boundary, face-exchange and scalar kernels of one
+// Runge-Kutta stage.
+//
+// Reference: src/Main3D.f90:571-952 (stage body), src/Subroutines3D.f90:235-324
+// (characteristic transforms), src/SingleTurbulence3D.f90:332-443 (sponge,
+// junction), src/Subroutineso.f90:192-259 (moving frame).  Every statement keeps
+// the reference's evaluation order; the library is compiled without FP
+// contraction.
+//
+// All per-case scalars (n, dt, time, frame velocity) live on the device and are
+// advanced by single-thread kernels, so a whole step is enqueued without a host
+// synchronisation and can be captured in a HIP graph.
+#include "tp_internal.hpp"
+
+namespace tp {
+
+constexpr double GAM = 1.4;
+constexpr double PI_D = 3.14159265358979323846;
+
+struct Phys {
+  double gamm1, ham, hamm1, gamm1prndtli, tgamm1prndtli, two_three;
+};
+__device__ inline Phys phys() {
+  Phys p;
+  p.gamm1 = GAM - 1.0;
+  p.ham = 1.0 / GAM;
+  p.hamm1 = 1.0 / p.gamm1;
+  p.gamm1prndtli = 1.0 / (p.gamm1 * 0.71);
+  p.tgamm1prndtli = 1.0 / (p.gamm1 * 0.99);
+  p.two_three = 2.0 / 3.0;
+  return p;
+}
+
+
+
+#define WORK_POINTS                                                            \
+  const Work w = a.work[blockIdx.x];                                           \
+  const Block &b = a.blocks[w.blk];                                            \
+  const int N = b.N;                                                           \
+  (void)N;
+
+// ---- moving frame, Subroutineso.f90:192-259 ----
+__device__ void movef_dev(const CaseCoef &c, double *sc, double dtko, double dtk) {
+  double *umf = sc + S_UMF1, *dudt = sc + S_DUDT1;
+  const double timo = sc[S_TIMO];
+  if (c.nsmf == 0) {
+    double ra0 = PI_D / c.timf;
+    double ra1 = ra0 * fmin(timo, c.timf), ra2 = ra0 * fmin(timo + dtko, c.timf);
+    double fctr = 1.0 - cos(ra1), dfdt = ra0 * sin(ra2);
+    double progmf = 0.5 * (fctr + dtk * dfdt);
+    for (int d = 0; d < 3; d++) umf[d] = progmf * c.uoo[d];
+    fctr = sin(ra1);
+    dfdt = ra0 * cos(ra2);
+    progmf = 0.5 * ra0 * (fctr + dtk * dfdt);
+    for (int d = 0; d < 3; d++) dudt[d] = progmf * c.uoo[d];
+    if (c.tp > 0 && timo + dtk > c.tps) {
+      double daoa = (c.aoa1 - c.aoa0) * PI_D / 180, raoa0 = c.aoa0 * PI_D / 180;
+      double ra3 = timo - c.tps, dtsdt = PI_D / c.tp;
+      double ts = fmin(dtsdt * ra3, PI_D), tsdt = fmin(dtsdt * (ra3 + dtko), PI_D);
+      double aoat = raoa0 + 0.5 * daoa * (1 - cos(ts));
+      double aoatdt = raoa0 + 0.5 * daoa * (1 - cos(tsdt));
+      double f = 0.5 * daoa * dtsdt, am = c.amachoo;
+      double ut0 = am * cos(aoat), ut1 = am * sin(aoat);
+      double du0 = -(am * f * sin(aoatdt) * sin(tsdt)), du1 = am * f * cos(aoatdt) * sin(tsdt);
+      umf[0] = ut0 + dtk * du0; umf[1] = ut1 + dtk * du1; umf[2] = 0.0 + dtk * 0.0;
+      double dd0 = -(am * f * sin(aoat) * sin(ts)), dd1 = am * f * cos(aoat) * sin(ts);
+      double st = sin(tsdt);
+      double d20 = -(am * f * (cos(aoatdt) * f * (st * st) + dtsdt * sin(aoatdt) * cos(tsdt)));
+      double d21 = -(am * f * (sin(aoatdt) * f * (st * st) - dtsdt * cos(aoatdt) * cos(tsdt)));
+      dudt[0] = dd0 + dtk * d20; dudt[1] = dd1 + dtk * d21; dudt[2] = 0.0 + dtk * 0.0;
+      sc[S_AOA] = atan(umf[1] / umf[0]) * 180 / PI_D;
+    }
+  } else {
+    for (int d = 0; d < 3; d++) { umf[d] = c.uoo[d]; dudt[d] = 0.0; }
+  }
+  sc[S_DTK] = dtk;
+  sc[S_DTKO] = dtko;
+}
+
+// stage start: Main3D.f90:571-572
+__global__ void k_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk) {
+  int cs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cs >= ncase) return;
+  const CaseCoef &c = coef[cs];
+  double *s = sc + cs * S_NSCALAR;
+  int nkrk = c.nkrk;
+  double dt = s[S_DT];
+  int f1 = nk - 2 > 0 ? (nk - 2 < 1 ? nk - 2 : 1) : 0;
+  int f2 = nk - 1 < 1 ? nk - 1 : 1;
+  double dtko = dt * (double)f1 / (nkrk - nk + 3);
+  double dtk = dt * (double)f2 / (nkrk - nk + 2);
+  movef_dev(c, s, dtko, dtk);
+}
+
+// time step size: Main3D.f90:587-610 (maxima reduced from the partials of the
+// blocks of each case; wr[cs], wr[cs+1] bound its work items)
+__global__ void k_sc_dt(const CaseCoef *coef, double *sc, const double *part, const int *wr,
+                        int ncase, int reduced) {
+  int cs = blockIdx.x;
+  if (cs >= ncase) return;
+  __shared__ double mi[256], mv[256];
+  double fi = -INFINITY, fv = -INFINITY;
+  double *s = sc + cs * S_NSCALAR;
+  int n = (int)s[S_N];
+  bool need = reduced && (n % 10 == 1) && coef[cs].dto < 0.0;
+  if (need) {
+    for (int q = wr[cs] + threadIdx.x; q < wr[cs + 1]; q += blockDim.x) {
+      fi = fmax(fi, part[2 * q]);
+      fv = fmax(fv, part[2 * q + 1]);
+    }
+  }
+  mi[threadIdx.x] = fi; mv[threadIdx.x] = fv;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      mi[threadIdx.x] = fmax(mi[threadIdx.x], mi[threadIdx.x + h]);
+      mv[threadIdx.x] = fmax(mv[threadIdx.x], mv[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const CaseCoef &c = coef[cs];
+    if (n % 10 == 1) {
+      s[S_NDT] = n;
+      s[S_DTS] = s[S_DTE];
+      if (c.dto < 0.0) {
+        double ra0 = c.cfl / mi[0], ra1 = ra0;
+        if (c.nviscous == 1) ra1 = 0.5 / mv[0];
+        s[S_DTE] = fmin(ra0, ra1);
+      } else {
+        s[S_DTE] = c.dto;
+      }
+    }
+    double sn = sin(0.05 * PI_D * (double)(n - (int)s[S_NDT]));
+    s[S_DT] = s[S_DTS] + (s[S_DTE] - s[S_DTS]) * (sn * sn);
+  }
+}
+
+// before the RK update: Main3D.f90:873-874
+__global__ void k_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk) {
+  int cs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cs >= ncase) return;
+  const CaseCoef &c = coef[cs];
+  double *s = sc + cs * S_NSCALAR;
+  double dt = s[S_DT];
+  int f2 = nk - 1 < 1 ? nk - 1 : 1;
+  double dtko = dt * (double)f2 / (c.nkrk - nk + 2);
+  double dtk = dt / (c.nkrk - nk + 1);
+  movef_dev(c, s, dtko, dtk);
+}
+
+// end of step: Main3D.f90:964-965
+__global__ void k_sc_end(double *sc, int ncase, const int *active) {
+  int cs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cs >= ncase || (active && !active[cs])) return;
+  double *s = sc + cs * S_NSCALAR;
+  s[S_N] = s[S_N] + 1;
+  s[S_TIMO] = s[S_TIMO] + s[S_DT];
+}
+
+// ---- primitives, Main3D.f90:576-583 ----
+__global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
+  WORK_POINTS
+  const CaseCoef &c = a.coef[b.cs];
+  const Phys ph = phys();
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    const double *qa = b.qa;
+    double d1 = 1.0 / qa[l];
+    double q2 = qa[N + l], q3 = qa[2 * N + l], q4 = qa[3 * N + l];
+    double d2 = q2 * d1, d3 = q3 * d1, d4 = q4 * d1;
+    double p = ph.gamm1 * (qa[4 * N + l] - 0.5 * (q2 * d2 + q3 * d3 + q4 * d4));
+    double d5 = GAM * p * d1;
+    b.de[l] = d1; b.de[N + l] = d2; b.de[2 * N + l] = d3; b.de[3 * N + l] = d4; b.de[4 * N + l] = d5;
+    b.p[l] = p;
+    b.ss[l] = c.srefp1dre * pow(d5, 1.5) / (d5 + c.srefoo);
+  }
+}
+
+// ---- time-step maxima, Main3D.f90:590-603 (partials per workgroup) ----
+__global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
+  WORK_POINTS
+  const double *sc = a.sc + b.cs * S_NSCALAR;
+  __shared__ double mi[256], mv[256];
+  double fi = -INFINITY, fv = -INFINITY;
+  int n = (int)sc[S_N];
+  if (n % 10 == 1) {
+    const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
+    const double *xi = b.xim, *et = b.etm, *ze = b.zem, *de = b.de;
+    for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+      double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
+      double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
+      double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
+      double r1 = x1 * x1 + x2 * x2 + x3 * x3 + e1 * e1 + e2 * e2 + e3 * e3 + z1 * z1 + z2 * z2 + z3 * z3;
+      double u = de[N + l] + u0, v = de[2 * N + l] + u1, ww = de[3 * N + l] + u2;
+      double r2 = fabs(x1 * u + x2 * v + x3 * ww) + fabs(e1 * u + e2 * v + e3 * ww) + fabs(z1 * u + z2 * v + z3 * ww);
+      double s2 = fabs(b.yaco[l]);
+      fi = fmax(fi, (sqrt(de[4 * N + l] * r1) + r2) * s2);
+      fv = fmax(fv, de[l] * b.ss[l] * r1 * s2 * s2);
+    }
+  }
+  mi[threadIdx.x] = fi; mv[threadIdx.x] = fv;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      mi[threadIdx.x] = fmax(mi[threadIdx.x], mi[threadIdx.x + h]);
+      mv[threadIdx.x] = fmax(mv[threadIdx.x], mv[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { a.part[2 * blockIdx.x] = mi[0]; a.part[2 * blockIdx.x + 1] = mv[0]; }
+}
+
+// ---- gradient combination with the metrics, Main3D.f90:631-651 ----
+// rr(:,1:3) hold d/dxi, d/deta, d/dzeta; outputs go to tv slots o0..o2
+// (negative slot -1..-3 = ss column 0..2)
+__device__ inline double *tv_slot(const Block &b, int o) {
+  return o >= 0 ? b.tv + (size_t)o * b.N : b.ss + (size_t)(-o - 1) * b.N;
+}
+__global__ void __launch_bounds__(256) k_grad_combine(StageArgs a) {
+  WORK_POINTS
+  double *o0 = tv_slot(b, a.o0), *o1 = tv_slot(b, a.o1), *o2 = tv_slot(b, a.o2);
+  const double *r1 = b.rr, *r2 = b.rr + N, *r3 = b.rr + 2 * N;
+  const double *xi = b.xim, *et = b.etm, *ze = b.zem;
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    double g1 = r1[l], g2 = r2[l], g3 = r3[l];
+    o0[l] = xi[l] * g1 + et[l] * g2 + ze[l] * g3;
+    o1[l] = xi[N + l] * g1 + et[N + l] * g2 + ze[N + l] * g3;
+    o2[l] = xi[2 * N + l] * g1 + et[2 * N + l] * g2 + ze[2 * N + l] * g3;
+  }
+}
+
+// ---- stresses and heat fluxes, Main3D.f90:653-692 ----
+__global__ void __launch_bounds__(256) k_stress(StageArgs a) {
+  WORK_POINTS
+  const CaseCoef &c = a.coef[b.cs];
+  const Phys ph = phys();
+  double *tv = b.tv, *de = b.de;
+  const double *ss = b.ss, *ya = b.yaco, *qa = b.qa;
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    double txx = tv[l], tyy = tv[N + l], tzz = tv[2 * N + l], txy = tv[3 * N + l], tyz = tv[4 * N + l],
+           tzx = tv[5 * N + l], hxx = tv[6 * N + l], hyy = tv[7 * N + l], hzz = tv[8 * N + l];
+    double r1 = de[l] * ya[l];
+    double r2 = ph.gamm1prndtli * r1, d5;
+    if (c.les == 1) {
+      double d1 = txx * txx + tyy * tyy + tzz * tzz + (hzz + txy) * (hzz + txy) + (hyy + tzx) * (hyy + tzx) +
+                  (hxx + tyz) * (hxx + tyz);
+      double varr = (double)(float)pow(-1 / ya[l], 1.5);   // single-precision work array
+      double r3 = qa[l] * (c.smago1 * c.smago1) * varr * sqrt(2 * d1);
+      r1 = r1 + r3 * ya[l];
+      r2 = r2 + ph.tgamm1prndtli * r3 * ya[l];
+      r3 = ph.two_three * (qa[l] * c.smago2 * varr * d1);
+      d5 = ph.two_three * (txx + tyy + tzz);
+      double ntxx = r1 * (2 * txx - d5) - r3, ntyy = r1 * (2 * tyy - d5) - r3, ntzz = r1 * (2 * tzz - d5) - r3;
+      double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);
+      txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;
+      de[l] = d1;
+    } else {
+      d5 = ph.two_three * (txx + tyy + tzz);
+      double ntxx = r1 * (2 * txx - d5), ntyy = r1 * (2 * tyy - d5), ntzz = r1 * (2 * tzz - d5);
+      double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);
+      txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;
+    }
+    de[4 * N + l] = d5;
+    double u = de[N + l], v = de[2 * N + l], ww = de[3 * N + l];
+    hxx = r2 * ss[l] + u * txx + v * txy + ww * tzx;
+    hyy = r2 * ss[N + l] + u * txy + v * tyy + ww * tyz;
+    hzz = r2 * ss[2 * N + l] + u * tzx + v * tyz + ww * tzz;
+    tv[l] = txx; tv[N + l] = tyy; tv[2 * N + l] = tzz; tv[3 * N + l] = txy; tv[4 * N + l] = tyz;
+    tv[5 * N + l] = tzx; tv[6 * N + l] = hxx; tv[7 * N + l] = hyy; tv[8 * N + l] = hzz;
+  }
+}
+
+// ---- contravariant velocities, Main3D.f90:710-715 ----
+__global__ void __launch_bounds__(256) k_contra(StageArgs a) {
+  WORK_POINTS
+  const double *sc = a.sc + b.cs * S_NSCALAR;
+  const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
+  const double *xi = b.xim, *et = b.etm, *ze = b.zem, *de = b.de;
+  double *ss = b.ss;
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    double r1 = de[N + l] + u0, r2 = de[2 * N + l] + u1, r3 = de[3 * N + l] + u2;
+    ss[l] = xi[l] * r1 + xi[N + l] * r2 + xi[2 * N + l] * r3;
+    ss[N + l] = et[l] * r1 + et[N + l] * r2 + et[2 * N + l] * r3;
+    ss[2 * N + l] = ze[l] * r1 + ze[N + l] * r2 + ze[2 * N + l] * r3;
+  }
+}
+
+// ---- metric-weighted Euler + viscous flux of field m, Main3D.f90:718-766 ----
+__global__ void __launch_bounds__(256) k_flux(StageArgs a) {
+  WORK_POINTS
+  const CaseCoef &c = a.coef[b.cs];
+  const double *sc = a.sc + b.cs * S_NSCALAR;
+  const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
+  const int m = a.m, visc = c.nviscous == 1;
+  const double *qa = b.qa, *ss = b.ss, *xi = b.xim, *et = b.etm, *ze = b.zem, *p = b.p, *tv = b.tv;
+  double *rr = b.rr;
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    double r1, r2, r3;
+    double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
+    double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
+    double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
+    double s1 = ss[l], s2 = ss[N + l], s3 = ss[2 * N + l], pl = p[l];
+    if (m == 1) {
+      double q = qa[l];
+      r1 = q * s1; r2 = q * s2; r3 = q * s3;
+    } else if (m <= 4) {
+      double q = qa[(m - 1) * N + l];
+      double mx = m == 2 ? x1 : m == 3 ? x2 : x3;
+      double me = m == 2 ? e1 : m == 3 ? e2 : e3;
+      double mz = m == 2 ? z1 : m == 3 ? z2 : z3;
+      r1 = q * s1 + mx * pl; r2 = q * s2 + me * pl; r3 = q * s3 + mz * pl;
+      if (visc) {
+        double ta, tb, tc;
+        if (m == 2) { ta = tv[l]; tb = tv[3 * N + l]; tc = tv[5 * N + l]; }
+        else if (m == 3) { ta = tv[3 * N + l]; tb = tv[N + l]; tc = tv[4 * N + l]; }
+        else { ta = tv[5 * N + l]; tb = tv[4 * N + l]; tc = tv[2 * N + l]; }
+        r1 = r1 - x1 * ta - x2 * tb - x3 * tc;
+        r2 = r2 - e1 * ta - e2 * tb - e3 * tc;
+        r3 = r3 - z1 * ta - z2 * tb - z3 * tc;
+      }
+    } else {
+      double d5 = qa[4 * N + l] + pl;
+      b.de[4 * N + l] = d5;
+      r1 = d5 * s1 - pl * (u0 * x1 + u1 * x2 + u2 * x3);
+      r2 = d5 * s2 - pl * (u0 * e1 + u1 * e2 + u2 * e3);
+      r3 = d5 * s3 - pl * (u0 * z1 + u1 * z2 + u2 * z3);
+      if (visc) {
+        double ha = tv[6 * N + l], hb = tv[7 * N + l], hc = tv[8 * N + l];
+        r1 = r1 - x1 * ha - x2 * hb - x3 * hc;
+        r2 = r2 - e1 * ha - e2 * hb - e3 * hc;
+        r3 = r3 - z1 * ha - z2 * hb - z3 * hc;
+      }
+    }
+    rr[l] = r1; rr[N + l] = r2; rr[2 * N + l] = r3;
+  }
+}
+
+// ---- sponge, SingleTurbulence3D.f90:360-372 (eddy inflow switched off in the
+// reference: its target is the uniform state rho = 1, rho u = 0) ----
+__global__ void __launch_bounds__(256) k_sponge(StageArgs a) {
+  WORK_POINTS
+  const Phys ph = phys();
+  const double target_p = ph.ham * pow(1.0, GAM);
+  for (int q = w.a + threadIdx.x; q < w.a + w.n; q += blockDim.x) {
+    int l = b.lcsz[q];
+    double as = b.asz[q];
+    b.de[l] = b.de[l] + as * (b.qa[l] - 1.0);
+    b.de[N + l] = b.de[N + l] + as * (b.qa[N + l] - 0.0);
+    b.de[2 * N + l] = b.de[2 * N + l] + as * (b.qa[2 * N + l] - 0.0);
+    b.de[3 * N + l] = b.de[3 * N + l] + as * (b.qa[3 * N + l] - 0.0);
+    b.de[4 * N + l] = b.de[4 * N + l] + b.bsz[q] * (b.p[l] - target_p);
+  }
+}
+
+// ---- characteristic algebra, Subroutines3D.f90:235-324 ----
+struct Elem { double rhoi, ao, aoi, ve[3], hv2, vn, vs; };
+
+__device__ inline void eleme(const double *q, double p, const double *cm, const double *umf, Elem &e) {
+  e.rhoi = 1.0 / q[0];
+  e.ao = sqrt(GAM * e.rhoi * p);
+  e.aoi = 1.0 / e.ao;
+  e.ve[0] = e.rhoi * q[1]; e.ve[1] = e.rhoi * q[2]; e.ve[2] = e.rhoi * q[3];
+  e.hv2 = 0.5 * (e.ve[0] * e.ve[0] + e.ve[1] * e.ve[1] + e.ve[2] * e.ve[2]);
+  e.vn = cm[0] * e.ve[0] + cm[1] * e.ve[1] + cm[2] * e.ve[2];
+  e.vs = cm[0] * umf[0] + cm[1] * umf[1] + cm[2] * umf[2];
+}
+
+// X_L . v (Q -> R) without forming the matrix
+__device__ inline void xtq2r_apply(const Elem &e, const double *cm, const double *v, double *o) {
+  const double gamm1 = GAM - 1.0;
+  double ho = gamm1 * e.aoi * e.aoi, bo = 1 - ho * e.hv2, co = e.aoi * e.vn;
+  double dm0 = e.aoi * cm[0], dm1 = e.aoi * cm[1], dm2 = e.aoi * cm[2];
+  double rv0 = ho * e.ve[0], rv1 = ho * e.ve[1], rv2 = ho * e.ve[2];
+  const double *ve = e.ve;
+  double xt[5][5];
+  xt[0][0] = bo * cm[0] + dm1 * ve[2] - dm2 * ve[1];
+  xt[0][1] = cm[0] * rv0; xt[0][2] = cm[0] * rv1 + dm2; xt[0][3] = cm[0] * rv2 - dm1; xt[0][4] = -ho * cm[0];
+  xt[1][0] = bo * cm[1] + dm2 * ve[0] - dm0 * ve[2];
+  xt[1][1] = cm[1] * rv0 - dm2; xt[1][2] = cm[1] * rv1; xt[1][3] = cm[1] * rv2 + dm0; xt[1][4] = -ho * cm[1];
+  xt[2][0] = bo * cm[2] + dm0 * ve[1] - dm1 * ve[0];
+  xt[2][1] = cm[2] * rv0 + dm1; xt[2][2] = cm[2] * rv1 - dm0; xt[2][3] = cm[2] * rv2; xt[2][4] = -ho * cm[2];
+  xt[3][0] = 1.0 - bo - co; xt[3][1] = dm0 - rv0; xt[3][2] = dm1 - rv1; xt[3][3] = dm2 - rv2; xt[3][4] = ho;
+  xt[4][0] = 1.0 - bo + co; xt[4][1] = -dm0 - rv0; xt[4][2] = -dm1 - rv1; xt[4][3] = -dm2 - rv2; xt[4][4] = ho;
+  for (int r = 0; r < 5; r++) {
+    double s = 0;
+    for (int k = 0; k < 5; k++) s = s + xt[r][k] * v[k];
+    o[r] = s;
+  }
+}
+
+__device__ inline void xtr2q_apply(const Elem &e, const double *cm, const double *v, double *o) {
+  const double hamm1 = 1.0 / (GAM - 1.0);
+  double bo = e.hv2 + hamm1 * e.ao * e.ao, co = e.ao * e.vn;
+  double dm0 = e.ao * cm[0], dm1 = e.ao * cm[1], dm2 = e.ao * cm[2];
+  const double *ve = e.ve;
+  double xt[5][5];
+  xt[0][0] = cm[0]; xt[0][1] = cm[1]; xt[0][2] = cm[2]; xt[0][3] = 0.5; xt[0][4] = 0.5;
+  xt[1][0] = cm[0] * ve[0]; xt[1][1] = cm[1] * ve[0] - dm2; xt[1][2] = cm[2] * ve[0] + dm1;
+  xt[1][3] = 0.5 * (ve[0] + dm0); xt[1][4] = xt[1][3] - dm0;
+  xt[2][0] = cm[0] * ve[1] + dm2; xt[2][1] = cm[1] * ve[1]; xt[2][2] = cm[2] * ve[1] - dm0;
+  xt[2][3] = 0.5 * (ve[1] + dm1); xt[2][4] = xt[2][3] - dm1;
+  xt[3][0] = cm[0] * ve[2] - dm1; xt[3][1] = cm[1] * ve[2] + dm0; xt[3][2] = cm[2] * ve[2];
+  xt[3][3] = 0.5 * (ve[2] + dm2); xt[3][4] = xt[3][3] - dm2;
+  xt[4][0] = e.hv2 * cm[0] + dm2 * ve[1] - dm1 * ve[2];
+  xt[4][1] = e.hv2 * cm[1] + dm0 * ve[2] - dm2 * ve[0];
+  xt[4][2] = e.hv2 * cm[2] + dm1 * ve[0] - dm0 * ve[1];
+  xt[4][3] = 0.5 * (bo + co); xt[4][4] = xt[4][3] - co;
+  for (int r = 0; r < 5; r++) {
+    double s = 0;
+    for (int k = 0; k < 5; k++) s = s + xt[r][k] * v[k];
+    o[r] = s;
+  }
+}
+
+__device__ inline bool gcbc_code(int np) { return np == 10 || np == 20 || np == 25 || np == 30; }
+
+__device__ inline void face_jk(const Block &b, int d, int jk, int &j, int &k) {
+  int fd = d == 0 ? b.ny : d == 1 ? b.nz : b.nx;
+  j = jk % fd; k = jk / fd;
+}
+
+// ---- GCBC/GCIC preparation, Main3D.f90:780-795 ----
+__global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  if ((int)threadIdx.x >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int np = b.nbc[d][ip];
+  const double *sc = a.sc + b.cs * S_NSCALAR;
+  int j, k;
+  face_jk(b, d, jk, j, k);
+  int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
+  double ra0 = (double)((20 - np) * (25 - np) * (30 - np) / 3000), ra1 = 1.0 - ra0;
+  double q[5], cm[3], cha[5], v[5], o[5];
+  for (int c = 0; c < 5; c++) q[c] = b.qa[(size_t)c * N + l];
+  for (int c = 0; c < 3; c++) cm[c] = b.cm[d][(ip * 3 + c) * F + jk];
+  Elem e;
+  eleme(q, b.p[l], cm, sc + S_UMF1, e);
+  double *drva = b.drva[d];
+  for (int c = 0; c < 5; c++) cha[c] = ra0 * drva[(ip * 5 + c) * F + jk] + ra1 * b.de[(size_t)c * N + l];
+  double ya = b.yaco[l];
+  for (int c = 0; c < 5; c++) v[c] = ya * cha[c];
+  xtq2r_apply(e, cm, v, o);
+  for (int c = 0; c < 5; c++) drva[(ip * 5 + c) * F + jk] = o[c];
+}
+
+// ---- face copy from the neighbour: drvb(:,:,ip) <- drva(:,:,1-ip) of ncd
+// (GCIC exchange Main3D.f90:799-813 and interface averaging :915-925) ----
+__global__ void __launch_bounds__(256) k_face_copy(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  if ((int)threadIdx.x >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const int d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const Block &s = a.blocks[b.ncd[d][ip]];
+  for (int c = 0; c < 5; c++) b.drvb[d][(ip * 5 + c) * F + jk] = s.drva[d][((1 - ip) * 5 + c) * F + jk];
+}
+
+// number of characteristic faces whose band (face point + MBCI inward) holds
+// index (i,j,k): the counter the reference accumulates in rr(:,1), Main3D.f90:460-473
+__device__ inline double gcbc_count(const Block &b, int x, int y, int z) {
+  int idx[3] = {x, y, z};
+  double cnt = 0.0;
+  for (int d = 0; d < 3; d++) {
+    if (gcbc_code(b.nbc[d][0]) && idx[d] <= MBCI) cnt = cnt + 1.0;
+    if (gcbc_code(b.nbc[d][1]) && idx[d] >= b.L[d] - 1 - MBCI) cnt = cnt + 1.0;
+  }
+  return cnt;
+}
+
+__device__ inline void point_ijk(const Block &b, int l, int &x, int &y, int &z) {
+  x = l % b.nx; y = (l / b.nx) % b.ny; z = l / (b.nx * b.ny);
+}
+
+// ---- GCBC/GCIC application along direction d (both ends), Main3D.f90:817-869 ----
+__global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  if ((int)threadIdx.x >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const CaseCoef &c = a.coef[b.cs];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int np = b.nbc[d][ip];
+  const double *sc = a.sc + b.cs * S_NSCALAR;
+  int j, k;
+  face_jk(b, d, jk, j, k);
+  const int i = ip * (b.L[d] - 1), iq = 1 - 2 * ip;
+  const double ra0 = iq;
+  int l = indx3(b, i, j, k, d);
+  double q[5], cm[3], cha[5], dha[5], dv[5], dq[5];
+  for (int v = 0; v < 5; v++) q[v] = b.qa[(size_t)v * N + l];
+  for (int v = 0; v < 3; v++) cm[v] = b.cm[d][(ip * 3 + v) * F + jk];
+  Elem e;
+  eleme(q, b.p[l], cm, sc + S_UMF1, e);
+  const double *drva = b.drva[d], *drvb = b.drvb[d];
+  for (int v = 0; v < 5; v++) { cha[v] = drva[(ip * 5 + v) * F + jk]; dha[v] = drvb[(ip * 5 + v) * F + jk]; }
+  double vv = e.vn + e.vs;
+  if (np == 10) {
+    if (ra0 * (vv + e.ao) > 0.0) cha[3] = 0.0;
+    if (ra0 * (vv - e.ao) > 0.0) cha[4] = 0.0;
+  } else if (np == 20 || np == 25) {
+    double rp = b.rpex[b.wall_off[d][ip] + jk];
+    if (rp == 0.0) {
+      const double *dd = sc + S_DUDT1;
+      double s = cm[0] * dd[0] + cm[1] * dd[1] + cm[2] * dd[2];
+      cha[3 + ip] = cha[4 - ip] + 2.0 * ra0 * e.aoi * q[0] * (s + 50.0 * vv);
+    } else {
+      if (ra0 * vv > 0.0) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }
+      if (ra0 * (vv + e.ao) > 0.0) cha[3] = dha[3];
+      if (ra0 * (vv - e.ao) > 0.0) cha[4] = dha[4];
+    }
+  } else {
+    if (ra0 * vv > 0.0) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }
+    if (ra0 * (vv + e.ao) > 0.0) cha[3] = dha[3];
+    if (ra0 * (vv - e.ao) > 0.0) cha[4] = dha[4];
+  }
+  for (int v = 0; v < 5; v++) dv[v] = cha[v] - drva[(ip * 5 + v) * F + jk];
+  xtr2q_apply(e, cm, dv, dq);
+  const double res = 1.0 / b.yaco[l];
+  for (int ii = 0; ii <= MBCI; ii++) {
+    int l2 = indx3(b, i + iq * ii, j, k, d);
+    int x, y, z;
+    point_ijk(b, l2, x, y, z);
+    double wgt = ii == 0 ? res : res * c.sbci[ii - 1];
+    double sbcc = wgt / gcbc_count(b, x, y, z);
+    for (int v = 0; v < 5; v++) b.de[(size_t)v * N + l2] = b.de[(size_t)v * N + l2] + sbcc * dq[v];
+  }
+}
+
+// ---- RK update, Main3D.f90:876-881 ----
+__global__ void __launch_bounds__(256) k_rk(StageArgs a) {
+  WORK_POINTS
+  const double dtk = a.sc[b.cs * S_NSCALAR + S_DTK];
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    double r1 = dtk * b.yaco[l];
+    for (int v = 0; v < 5; v++) b.qa[(size_t)v * N + l] = b.qo[(size_t)v * N + l] - r1 * b.de[(size_t)v * N + l];
+  }
+}
+
+// ---- wall temperature and velocity, Main3D.f90:886-895 ----
+__global__ void __launch_bounds__(256) k_wall(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  if ((int)threadIdx.x >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const CaseCoef &c = a.coef[b.cs];
+  const Phys ph = phys();
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x;
+  const int np = b.nbc[d][ip];
+  const double *umf = a.sc + b.cs * S_NSCALAR + S_UMF1;
+  int j, k;
+  face_jk(b, d, jk, j, k);
+  int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
+  double ra0 = ph.ham * ph.hamm1 * c.wtemp;
+  double ra1 = (double)((np - 20) / 5);
+  double rp = b.rpex[b.wall_off[d][ip] + jk];
+  double ra2 = 1.0 - rp, ra3 = 1.0 - ra1 * ra2;
+  double *qa = b.qa;
+  double fctr = (1.0 - ra3) * qa[l];
+  for (int v = 0; v < 3; v++) qa[(1 + v) * N + l] = ra3 * qa[(1 + v) * N + l] - fctr * umf[v];
+  double q1 = qa[l], q2 = qa[N + l], q3 = qa[2 * N + l], q4 = qa[3 * N + l];
+  qa[4 * N + l] = rp * qa[4 * N + l] + ra2 * (ra0 * q1 + 0.5 * (q2 * q2 + q3 * q3 + q4 * q4) / q1);
+}
+
+// ---- junction averaging, SingleTurbulence3D.f90:378-443 ----
+// jct[j*12 + mbr*3 + {block, ip, jp}], jvel[j*2 + {0,1}]; one thread per (junction, k)
+__global__ void k_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int jn = t / maxk, k = t % maxk;
+  if (jn >= njct) return;
+  const int *J = jct + jn * 12;
+  if (k > blocks[J[0]].nz - 1) return;
+  double v[4][5];
+  int lq[4];
+  for (int q = 0; q < 4; q++) {
+    const Block &b = blocks[J[q * 3]];
+    int i = J[q * 3 + 1] * (b.nx - 1), j = J[q * 3 + 2] * (b.ny - 1);
+    lq[q] = indx3(b, i, j, k, 0);
+    for (int m = 0; m < 5; m++) v[q][m] = b.qa[(size_t)m * b.N + lq[q]];
+  }
+  int va = jvel[jn * 2], vb = jvel[jn * 2 + 1];
+  double r[5];
+  r[0] = 0.25 * (v[0][0] + v[1][0] + v[2][0] + v[3][0]);
+  for (int m = 1; m <= 3; m++) r[m] = 0.5 * (v[va][m] + v[vb][m]);
+  r[4] = 0.25 * (v[0][4] + v[1][4] + v[2][4] + v[3][4]);
+  for (int q = 0; q < 4; q++) {
+    const Block &b = blocks[J[q * 3]];
+    for (int m = 0; m < 5; m++) b.qa[(size_t)m * b.N + lq[q]] = r[m];
+  }
+}
+
+// ---- interface averaging, Main3D.f90:901-952 ----
+__device__ inline bool avg_code(int np) { return np == 30 || np == 35 || np == 45; }
+
+// capture face values into drva (first loop of the reference)
+__global__ void __launch_bounds__(256) k_avg_capture(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  if ((int)threadIdx.x >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  int j, k;
+  face_jk(b, d, jk, j, k);
+  int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
+  for (int v = 0; v < 5; v++) b.drva[d][(ip * 5 + v) * F + jk] = b.qa[(size_t)v * N + l];
+}
+
+// running-average weight of face (d, ip) at (x,y,z): the reference's rr(l,1)
+// counter after this face's increment (faces processed in the order
+// xi start, xi end, eta start, ..., each adding one (or rpex for GCIC walls))
+__device__ inline double avg_count(const Block &b, const CaseCoef &c, int x, int y, int z, int d, int ip,
+                                   double self_inc) {
+  int idx[3] = {x, y, z};
+  double cnt = 1.0;
+  for (int dd = 0; dd < d; dd++)
+    for (int s = 0; s < 2; s++) {
+      int np = b.nbc[dd][s];
+      bool on = s == 0 ? idx[dd] == 0 : idx[dd] == b.L[dd] - 1;
+      if (!on) continue;
+      if (avg_code(np)) cnt = cnt + 1.0;
+      else if (c.nextgcic == 1 && (np == 20 || np == 25)) {
+        int fjk;
+        int fj = dd == 0 ? y : dd == 1 ? z : x, fk = dd == 0 ? z : dd == 1 ? x : y;
+        int fd = dd == 0 ? b.ny : dd == 1 ? b.nz : b.nx;
+        fjk = fk * fd + fj;
+        cnt = cnt + b.rpex[b.wall_off[dd][s] + fjk];
+      }
+    }
+  return cnt + self_inc;
+}
+
+__global__ void __launch_bounds__(256) k_avg_apply(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  if ((int)threadIdx.x >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const CaseCoef &c = a.coef[b.cs];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int np = b.nbc[d][ip];
+  int j, k;
+  face_jk(b, d, jk, j, k);
+  int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
+  int x, y, z;
+  point_ijk(b, l, x, y, z);
+  // a face of the same direction at the other end cannot hold this point
+  // (lines have >= 7 points), so only earlier directions contribute
+  double *qa = b.qa;
+  const double *drvb = b.drvb[d];
+  if (avg_code(np)) {
+    double r1 = avg_count(b, c, x, y, z, d, ip, 1.0);
+    double r2 = 1.0 / r1;
+    for (int v = 0; v < 5; v++)
+      qa[(size_t)v * N + l] = r2 * ((r1 - 1.0) * qa[(size_t)v * N + l] + drvb[(ip * 5 + v) * F + jk]);
+  } else {
+    double rp = b.rpex[b.wall_off[d][ip] + jk];
+    double ra1 = 1.0 - rp;
+    double r1 = avg_count(b, c, x, y, z, d, ip, rp);
+    double r2 = 1.0 / r1;
+    for (int v = 0; v < 5; v++)
+      qa[(size_t)v * N + l] = rp * r2 * ((r1 - 1.0) * qa[(size_t)v * N + l] + drvb[(ip * 5 + v) * F + jk]) +
+                              ra1 * qa[(size_t)v * N + l];
+  }
+}
+
+// ---- copies ----
+__global__ void __launch_bounds__(256) k_copy5(StageArgs a) {   // qo = qa
+  WORK_POINTS
+  for (int v = 0; v < 5; v++)
+    for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) b.qo[(size_t)v * N + l] = b.qa[(size_t)v * N + l];
+}
+
+__global__ void __launch_bounds__(256) k_copy_field(StageArgs a) {  // rr(:,1) = field(arr, col)
+  WORK_POINTS
+  const double *s = field_ptr(b, a.arr, a.col);
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) b.rr[l] = s[l];
+}
+
+// ---- initial state, SingleTurbulence3D.f90:203 ----
+__global__ void __launch_bounds__(256) k_initialo(StageArgs a) {
+  WORK_POINTS
+  const Phys ph = phys();
+  double e = ph.hamm1 * (1.0 / GAM);
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    b.qa[l] = 1.0; b.qa[N + l] = 0.0; b.qa[2 * N + l] = 0.0; b.qa[3 * N + l] = 0.0; b.qa[4 * N + l] = e;
+  }
+}
+
+// ---- metrics from coordinate derivatives, Main3D.f90:378-386, 411-413 ----
+// qo/qa/de columns 0..2 hold d(x,y,z)/d(xi,eta,zeta)
+__global__ void __launch_bounds__(256) k_metrics(StageArgs a) {
+  WORK_POINTS
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
+    double qo1 = b.qo[l], qo2 = b.qo[N + l], qo3 = b.qo[2 * N + l];
+    double qa1 = b.qa[l], qa2 = b.qa[N + l], qa3 = b.qa[2 * N + l];
+    double de1 = b.de[l], de2 = b.de[N + l], de3 = b.de[2 * N + l];
+    double x1 = qa2 * de3 - de2 * qa3, x2 = de2 * qo3 - qo2 * de3, x3 = qo2 * qa3 - qa2 * qo3;
+    double e1 = qa3 * de1 - de3 * qa1, e2 = de3 * qo1 - qo3 * de1, e3 = qo3 * qa1 - qa3 * qo1;
+    double z1 = qa1 * de2 - de1 * qa2, z2 = de1 * qo2 - qo1 * de2, z3 = qo1 * qa2 - qa1 * qo2;
+    b.xim[l] = x1; b.xim[N + l] = x2; b.xim[2 * N + l] = x3;
+    b.etm[l] = e1; b.etm[N + l] = e2; b.etm[2 * N + l] = e3;
+    b.zem[l] = z1; b.zem[N + l] = z2; b.zem[2 * N + l] = z3;
+    b.yaco[l] = 3.0 / (qo1 * x1 + qo2 * e1 + qo3 * z1 + qa1 * x2 + qa2 * e2 + qa3 * z2 + de1 * x3 + de2 * e3 + de3 * z3);
+  }
+}
+
+// ---- face unit normals, Main3D.f90:429-439 ----
+__global__ void __launch_bounds__(256) k_normals(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  if ((int)threadIdx.x >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  int j, k;
+  face_jk(b, d, jk, j, k);
+  int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
+  const double *mt = d == 0 ? b.xim : d == 1 ? b.etm : b.zem;
+  double rv0 = b.yaco[l] * mt[l], rv1 = b.yaco[l] * mt[N + l], rv2 = b.yaco[l] * mt[2 * N + l];
+  double f = 1.0 / sqrt(rv0 * rv0 + rv1 * rv1 + rv2 * rv2);
+  b.cm[d][(ip * 3 + 0) * F + jk] = f * rv0;
+  b.cm[d][(ip * 3 + 1) * F + jk] = f * rv1;
+  b.cm[d][(ip * 3 + 2) * F + jk] = f * rv2;
+}
+
+// ---- host launchers ----
+#define LAUNCH(K, NWG, BS, ST, A) hipLaunchKernelGGL(K, dim3(NWG), dim3(BS), 0, ST, A)
+void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
+  switch (which) {
+    case 0: LAUNCH(k_primitives, nwg, 256, st, a); break;
+    case 1: LAUNCH(k_dt_reduce, nwg, 256, st, a); break;
+    case 2: LAUNCH(k_grad_combine, nwg, 256, st, a); break;
+    case 3: LAUNCH(k_stress, nwg, 256, st, a); break;
+    case 4: LAUNCH(k_contra, nwg, 256, st, a); break;
+    case 5: LAUNCH(k_flux, nwg, 256, st, a); break;
+    case 6: LAUNCH(k_sponge, nwg, 256, st, a); break;
+    case 7: LAUNCH(k_rk, nwg, 256, st, a); break;
+    case 8: LAUNCH(k_copy5, nwg, 256, st, a); break;
+    case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;
+    case 10: LAUNCH(k_initialo, nwg, 256, st, a); break;
+    case 11: LAUNCH(k_metrics, nwg, 256, st, a); break;
+  }
+}
+void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a) {
+  switch (which) {
+    case 0: LAUNCH(k_gcbc_prep, nwg, 256, st, a); break;
+    case 1: LAUNCH(k_face_copy, nwg, 256, st, a); break;
+    case 2: LAUNCH(k_gcbc_apply, nwg, 256, st, a); break;
+    case 3: LAUNCH(k_wall, nwg, 256, st, a); break;
+    case 4: LAUNCH(k_avg_capture, nwg, 256, st, a); break;
+    case 5: LAUNCH(k_avg_apply, nwg, 256, st, a); break;
+    case 6: LAUNCH(k_normals, nwg, 256, st, a); break;
+  }
+}
+void launch_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st) {
+  hipLaunchKernelGGL(k_sc_stage, dim3((ncase + 63) / 64), dim3(64), 0, st, coef, sc, ncase, nk);
+}
+void launch_sc_dt(const CaseCoef *coef, double *sc, const double *part, const int *wr, int ncase, int reduced,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_sc_dt, dim3(ncase), dim3(256), 0, st, coef, sc, part, wr, ncase, reduced);
+}
+void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st) {
+  hipLaunchKernelGGL(k_sc_rk, dim3((ncase + 63) / 64), dim3(64), 0, st, coef, sc, ncase, nk);
+}
+void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st) {
+  hipLaunchKernelGGL(k_sc_end, dim3((ncase + 63) / 64), dim3(64), 0, st, sc, ncase, active);
+}
+void launch_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk, hipStream_t st) {
+  int tot = njct * maxk;
+  hipLaunchKernelGGL(k_junction, dim3((tot + 63) / 64), dim3(64), 0, st, blocks, jct, jvel, njct, maxk);
+}
+
+}  // namespace tp
