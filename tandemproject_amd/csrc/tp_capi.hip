@@ -470,6 +470,7 @@ void enqueue_stage(tp_context *c, int nk) {
     // src/Main3D.f90:627-651: gradients of u, v, w, T combined with the metrics
     // tv slots: 0 txx 1 tyy 2 tzz 3 txy 4 tyz 5 tzx 6 hxx 7 hyy 8 hzz; -1..-3 ss
     const int slots[4][3] = {{0, 8, 5}, {3, 1, 6}, {7, 4, 2}, {-1, -2, -3}};
+    run_points(c, PK_MU, a, K_VISC, 16.0);
     for (int f = 0; f < 4; f++) {
       run_exchange(c, 0, 1, 0, F_DE, f + 1);
       run_line(c, 0, 2, F_DE, f + 1, F_RR, 2, 0, 0);

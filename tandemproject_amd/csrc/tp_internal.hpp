@@ -144,7 +144,7 @@ void launch_junction(const Block *blocks, const int *jct, const int *jvel, int n
                      hipStream_t st);
 
 enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_GRAD, PK_STRESS, PK_CONTRA, PK_FLUX, PK_SPONGE, PK_RK,
-                   PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS };
+                   PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS, PK_MU };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS };
 
 }  // namespace tp

@@ -664,6 +664,11 @@ __global__ void __launch_bounds__(256) k_copy5(StageArgs a) {   // qo = qa
     for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) b.qo[(size_t)v * N + l] = b.qa[(size_t)v * N + l];
 }
 
+__global__ void __launch_bounds__(256) k_mu_to_de(StageArgs a) {  // de(:,1) = ss(:,1), Main3D.f90:627
+  WORK_POINTS
+  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) b.de[l] = b.ss[l];
+}
+
 __global__ void __launch_bounds__(256) k_copy_field(StageArgs a) {  // rr(:,1) = field(arr, col)
   WORK_POINTS
   const double *s = field_ptr(b, a.arr, a.col);
@@ -731,6 +736,7 @@ void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
     case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;
     case 10: LAUNCH(k_initialo, nwg, 256, st, a); break;
     case 11: LAUNCH(k_metrics, nwg, 256, st, a); break;
+    case 12: LAUNCH(k_mu_to_de, nwg, 256, st, a); break;
   }
 }
 void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a) {

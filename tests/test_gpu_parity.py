@@ -150,11 +150,34 @@ def compare_state(o, s, lay, tol, case=0):
     return worst
 
 
-def test_one_step_tight():
-    say("one full step (filter + 4 stages), viscous 3x2 aerofoil case")
+def test_first_steps_tight():
+    say("four full steps (dt > 0 from the third), viscous 3x2 aerofoil case")
     lay, ss, p = aerofoil_case()
-    o, s = run_steps(lay, ss, p, 2)
+    o, s = run_steps(lay, ss, p, 4)
+    assert o.scalars()["dt"] > 0
     compare_state(o, s, lay, 1e-12)
+
+
+@pytest.mark.parametrize("nviscous", [1, 0])
+def test_stage_arrays(nviscous):
+    """Every array a stage writes, after one filter sweep and one stage with dt > 0."""
+    say("stage arrays, nviscous =", nviscous)
+    lay, ss, p = aerofoil_case(nviscous=nviscous)
+    o, s = run_steps(lay, ss, p, 2)
+    o.filter_step(); s.filter_step()
+    o.stage(1); s.stage(1)
+    s.sync()
+    names = ["qa", "qo", "p", "de"] + (["tv"] if nviscous else [])
+    for b in range(lay.nb):
+        for name in names:
+            g = s.download(0, b, name); r = o.array(b, name)
+            k = 9 if name == "tv" else (5 if name in ("qa", "qo", "de") else 1)
+            g = g.reshape(k, -1); r = r.reshape(k, -1)
+            for v in range(k):
+                d = rel(g[v], r[v])
+                assert d <= 1e-12, (b, name, v, d)
+    for key in ("dt", "umf1", "dudt1", "dtk"):
+        assert abs(o.scalars()[key] - s.scalars(0)[key]) <= 1e-14 * max(1.0, abs(o.scalars()[key]))
 
 
 @pytest.mark.parametrize("variant", ["viscous", "euler", "les-gcic"])
