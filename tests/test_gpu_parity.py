@@ -135,13 +135,21 @@ def run_steps(lay, ss, p, nsteps, graph=False):
 
 
 def compare_state(o, s, lay, tol, case=0):
-    worst = 0.0
-    for b in range(lay.nb):
-        q = s.download(case, b, "qa").reshape(5, -1)
-        r = o.array(b, "qa").reshape(5, -1)
+    """Max difference of the conserved variables relative to their magnitude:
+    rho and E by their own maxima, the momentum components by the maximum of
+    the momentum vector (a spanwise-uniform flow has rho*w at round-off level,
+    so normalising it by itself would measure noise against noise)."""
+    qs = [s.download(case, b, "qa").reshape(5, -1) for b in range(lay.nb)]
+    rs = [o.array(b, "qa").reshape(5, -1) for b in range(lay.nb)]
+    for q in qs:
         assert np.isfinite(q).all()
+    scale = [max(np.abs(r[v]).max() for r in rs) for v in range(5)]
+    mom = max(scale[1:4])
+    scale = [scale[0], mom, mom, mom, scale[4]]
+    worst = 0.0
+    for q, r in zip(qs, rs):
         for v in range(5):
-            worst = max(worst, rel(q[v], r[v]))
+            worst = max(worst, float(np.abs(q[v] - r[v]).max() / scale[v]))
     so, sg = o.scalars(), s.scalars(case)
     for k in ("n", "dt", "timo", "umf1"):
         assert abs(so[k] - sg[k]) <= tol * max(abs(so[k]), 1e-300), (k, so[k], sg[k])
@@ -168,6 +176,7 @@ def test_stage_arrays(nviscous):
     o.stage(1); s.stage(1)
     s.sync()
     names = ["qa", "qo", "p", "de"] + (["tv"] if nviscous else [])
+    worst = 0.0
     for b in range(lay.nb):
         for name in names:
             g = s.download(0, b, name); r = o.array(b, name)
@@ -175,7 +184,9 @@ def test_stage_arrays(nviscous):
             g = g.reshape(k, -1); r = r.reshape(k, -1)
             for v in range(k):
                 d = rel(g[v], r[v])
+                worst = max(worst, d)
                 assert d <= 1e-12, (b, name, v, d)
+    say("  max rel diff over stage arrays", worst)
     for key in ("dt", "umf1", "dudt1", "dtk"):
         assert abs(o.scalars()[key] - s.scalars(0)[key]) <= 1e-14 * max(1.0, abs(o.scalars()[key]))
 
@@ -197,7 +208,13 @@ def test_hundred_steps(variant):
     o, s = run_steps(lay, ss, p, 100)
     if variant == "les-gcic":
         assert max(o.array(b, "rpex").max() if o.array(b, "rpex").size else 0 for b in range(lay.nb)) == 1
-    compare_state(o, s, lay, 1e-10)
+        # the reference keeps the Smagorinsky volume term (-1/yaco)**1.5 in a
+        # single-precision array (varr, src/MainVar3D.f90:105; Main3D.f90:664):
+        # a one-ulp difference of the device pow() can flip its float32
+        # rounding (6e-8 relative at that point), so this path is held to 2e-9
+        compare_state(o, s, lay, 2e-9)
+    else:
+        compare_state(o, s, lay, 1e-10)
 
 
 def test_graph_replay_matches_eager():
