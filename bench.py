@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel events)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default line-solver launch shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -91,6 +92,12 @@ def main():
     for b in range(lay.nb):
         s.set_grid(0, b, ss[b])
     s.setup_geometry()
+    tuned = None
+    if not args.no_tune:
+        tuned = s.autotune(10)
+        print("[bench] line solves tuned (chunks per line, us per launch): "
+              + ", ".join(f"{'xez'[d]}-{k} {c} {t:.0f}" for (d, k), (c, t) in tuned.items()),
+              file=sys.stderr, flush=True)
     s.initialo(0)
     print("[bench] geometry ready; warmup", file=sys.stderr, flush=True)
     s.step(args.warmup)
@@ -122,7 +129,7 @@ def main():
         achieved = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_deriv (compact derivative line solve, 3 directions)",
+                "kernel": "compact derivative line solves (k_col / k_deriv, 3 directions)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl}
     breakdown = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1]}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
@@ -135,6 +142,7 @@ def main():
                                f"sub-blocks, lxibk={dims['lxibk']}, letbk={dims['letbk']}, nz={dims['lzebk'][0] + 1}, "
                                f"{npts / 1e6:.2f} M points, M=0.3, Re_c=4e5, viscous, 4-stage RK",
                    "points": npts, "sub_blocks": lay.nb, "graph": bool(args.graph),
+                   "line_chunks": {f"{'xez'[d]}-{k}": c for (d, k), c in s.col_config().items()},
                    "ms_per_step_by_class": breakdown},
         "roofline": roof, "cpu_baseline": cpu,
     }

@@ -105,7 +105,8 @@ int32_t tp_set_scalars(tp_context *ctx, int32_t case_id, const double *s16);
 /* ---- per-call drop-ins, all sub-blocks of all cases ---- */
 /* mpigo(nt,nrt,n45,itag): src/Subroutines3D.f90:16-80 (closure sums to the neighbours' recv) */
 int32_t tp_mpigo(tp_context *ctx, int32_t nt, int32_t nrt, int32_t n45);
-/* deriv(nn,nz,m): src/Subroutines3D.f90:84-141 (rr(:,nn) = d rr(:,nz)/d index_nn, drva capture) */
+/* deriv(nn,nz,m): src/Subroutines3D.f90:84-141 (rr(:,nn) = d rr(:,nz)/d index_nn; m = 1..5
+ * keeps the end values in drva(:,m,0:1) as the reference does, m = 0 skips that) */
 int32_t tp_deriv(tp_context *ctx, int32_t nn, int32_t nz, int32_t m);
 /* filte(nn,nz): src/Subroutines3D.f90:145-206 (rr(:,nz) += filter increment) */
 int32_t tp_filte(tp_context *ctx, int32_t nn, int32_t nz);
@@ -120,6 +121,32 @@ int32_t tp_step(tp_context *ctx, int32_t nsteps);
 /* the same, replaying one step captured in a HIP graph */
 int32_t tp_step_graph(tp_context *ctx, int32_t nsteps);
 int32_t tp_sync(tp_context *ctx);
+
+/* ---- options ----
+ * TP_OPT_EXACT_LINES = 1: line solves run the reference's sequential LU
+ *   sweeps one lane per line (bit-identical to a sequential CPU sweep);
+ *   0 (default): chunked sweeps -- each line cut into P chunks swept by P
+ *   waves, the chunks joined by composing their 2x2 carry maps -- equal to
+ *   the sequential sweep up to rounding of the chunk carries (~1e-16
+ *   relative).  May be changed between steps.
+ * TP_OPT_CHUNKS = P (1, 2, 4, 8, 16) fixes the chunks per line of the
+ *   chunked sweeps for every direction where chunks of ceil(L/P) <= 32
+ *   points fit; 0 (default) lets the library choose per direction.  With a
+ *   fixed P a case computes bit-identical results alone or batched with
+ *   others whose lines need the same P. */
+enum { TP_OPT_EXACT_LINES = 0, TP_OPT_CHUNKS = 1 };
+int32_t tp_set_option(tp_context *ctx, int32_t option, int64_t value);
+
+/* Time every usable line-solver configuration (sequential kernel and P =
+ * 1..16 chunks) on the context's own blocks, reps launches each, and keep
+ * the fastest per direction and kind.  Overwrites the scratch array rr, so
+ * call it between steps.  chosen[d*2+kind] (kind 0 deriv, 1 filter) receives
+ * the chunk count (0: sequential kernel), us[d*2+kind] the time per launch;
+ * either may be NULL.  No counterpart in the reference: it replaces the
+ * fixed 1-D loop structure of deriv/filte with a measured launch shape. */
+int32_t tp_autotune(tp_context *ctx, int32_t reps, int32_t *chosen, double *us);
+/* chunks per line in use, [d*2+kind] (0: sequential kernel) */
+int32_t tp_col_config(tp_context *ctx, int32_t *chunks6);
 
 /* ---- measurement ---- */
 /* record HIP events around every launch (on the context's stream) */

@@ -922,13 +922,19 @@ static void deriv_block(orc *c, oblk *B, int nn, int nz, int m) {
       sb[ie - 1] = xu1[ie - 1] * sa[ie - 1] - xu2[ie - 1] * sb[ie];
       for (int i = ie - 2; i >= is; i--) sb[i] = xu1[i] * sa[i] - xu2[i] * sb[i + 1] - xu3[i] * sb[i + 2];
       for (int i = is; i <= ie; i++) out[li[i]] = sb[i];
-      drv[(0 * 5 + m - 1) * F + jk] = sb[is];
-      drv[(1 * 5 + m - 1) * F + jk] = sb[ie];
+      if (m > 0) {   /* m = 0: no boundary capture (internal use only) */
+        drv[(0 * 5 + m - 1) * F + jk] = sb[is];
+        drv[(1 * 5 + m - 1) * F + jk] = sb[ie];
+      }
     }
   }
 }
 
 void orc_deriv(orc *c, int nn, int nz, int m) {
+  if (nn < 1 || nn > 3 || nz < 1 || nz > 3 || m < 0 || m > 5) {
+    fprintf(stderr, "oracle: deriv(%d, %d, %d) out of range\n", nn, nz, m);
+    abort();
+  }
   for (int b = 0; b < c->nb; b++) deriv_block(c, &c->b[b], nn - 1, nz, m);
 }
 

@@ -28,7 +28,8 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_set_grid", "tp_setup_geometry", "tp_initialo", "tp_upload", "tp_download",
            "tp_array_count", "tp_get_scalars", "tp_set_scalars", "tp_mpigo", "tp_deriv",
            "tp_filte", "tp_filter_step", "tp_stage", "tp_step", "tp_step_graph", "tp_sync",
-           "tp_profile", "tp_profile_read", "tp_info"]
+           "tp_profile", "tp_profile_read", "tp_info", "tp_set_option", "tp_autotune",
+           "tp_col_config"]
 
 _lib = None
 
@@ -63,6 +64,9 @@ def load(path=LIB_PATH):
         "tp_profile": [P, i32],
         "tp_profile_read": [P, i32, pd, ctypes.POINTER(i64), pd, i32],
         "tp_info": [P, ctypes.POINTER(i64)],
+        "tp_set_option": [P, i32, i64],
+        "tp_autotune": [P, i32, pi32, pd],
+        "tp_col_config": [P, pi32],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -90,8 +94,10 @@ def _check(rc):
 class Solver:
     """Device-resident state of one or more cases (each a list of sub-blocks)."""
 
-    def __init__(self, cases):
-        """cases: list of (layout, params) pairs."""
+    def __init__(self, cases, chunks=0, exact_lines=False):
+        """cases: list of (layout, params) pairs.  chunks: fixed number of
+        chunks per line of the chunked line sweeps (1, 2, 4, 8, 16; 0: chosen
+        per direction); exact_lines: sequential sweeps, one lane per line."""
         L = load()
         self.h = ctypes.c_void_p()
         _check(L.tp_create(ctypes.byref(self.h)))
@@ -112,7 +118,10 @@ class Solver:
             _check(L.tp_add_case(self.h, layout.nb, _ip(d), _ip(b), _ip(n), _ip(s), layout.njct,
                                  _ip(j), _ip(v), _ip(ipar), _dp(dpar), ctypes.byref(cid)))
             self.layouts.append(layout)
+        _check(L.tp_set_option(self.h, 1, int(chunks)))
         _check(L.tp_finalize(self.h))
+        if exact_lines:
+            self.set_exact_lines(True)
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -124,6 +133,30 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+    def set_exact_lines(self, exact=True):
+        """Sequential one-lane-per-line sweeps (bit-identical to the reference's
+        order) instead of the chunked sweeps."""
+        _check(load().tp_set_option(self.h, 0, 1 if exact else 0))
+
+    def set_chunks(self, chunks):
+        """Chunks per line of the chunked sweeps (0: the library's choice)."""
+        _check(load().tp_set_option(self.h, 1, int(chunks)))
+
+    def autotune(self, reps=10):
+        """Time every line-solver configuration on these blocks and keep the
+        fastest per direction and kind.  Returns {(direction, kind): (chunks,
+        microseconds per launch)}; chunks 0 is the sequential kernel.
+        Overwrites the scratch array rr."""
+        ch = np.zeros(6, np.int32); us = np.zeros(6)
+        _check(load().tp_autotune(self.h, int(reps), _ip(ch), _dp(us)))
+        return {(i // 2, ("deriv", "filter")[i % 2]): (int(ch[i]), float(us[i])) for i in range(6)}
+
+    def col_config(self):
+        """Chunks per line in use: {(direction, kind): chunks} (0: sequential)."""
+        ch = np.zeros(6, np.int32)
+        _check(load().tp_col_config(self.h, _ip(ch)))
+        return {(i // 2, ("deriv", "filter")[i % 2]): int(ch[i]) for i in range(6)}
 
     # ---- data ----
     def set_grid(self, case, block, xyz):

@@ -11,7 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["tp_line.hip", "tp_stage.hip", "tp_capi.hip"]
+SOURCES = ["tp_line.hip", "tp_col.hip", "tp_stage.hip", "tp_capi.hip"]
 LIB = os.path.join(HERE, "libtandem_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TP_OFFLOAD_ARCH", "gfx950")
@@ -21,7 +21,7 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=
          "-munsafe-fp-atomics", "-Wall", "-Wno-unused-variable", "-Wno-unused-function"]
 
 
-def build(verbose=False, jobs=3):
+def build(verbose=False):
     objs = []
     procs = []
     for s in SOURCES:

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -267,6 +268,77 @@ void closure_rows(double fk, CaseCoef &c) {
     }
 }
 
+// Tables of the chunked line solver for one line family cut into P chunks of
+// Cl = ceil(L/P) <= 32 points, laid out for one 64-lane load per register:
+//   [0, 64)    forward carry maps: lane 4q+t holds entry t of chunk q's 2x2
+//              response of its last pair (y[e-1], y[e]) to the incoming
+//              pair (y[s-2], y[s-1]);
+//   [64, 128)  backward carry maps: the response of (x[s], x[s+1]) to
+//              (x[e+1], x[e+2]);
+//   then per chunk q three registers: {l1 | l2}, {u1 | u2}, {u3 | 0}, row k
+//   of the chunk in lane k (first half) or 32+k (second half).
+// Factor entries the sweeps never use (l1, l2 of rows 0, 1; u2, u3 of rows
+// L-1, L-2) and rows past the chunk are zero.
+std::vector<double> col_tables(int L, int P, const double *l1, const double *l2, const double *u1,
+                               const double *u2, const double *u3) {
+  const int Cl = (L + P - 1) / P, Pe = (L + Cl - 1) / Cl;
+  std::vector<double> T(128 + 192 * (size_t)Pe, 0.0);
+  auto L1 = [&](int i) { return i < 2 ? 0.0 : l1[i]; };
+  auto L2 = [&](int i) { return i < 1 ? 0.0 : l2[i]; };
+  auto U2 = [&](int i) { return i >= L - 1 ? 0.0 : u2[i]; };
+  auto U3 = [&](int i) { return i >= L - 2 ? 0.0 : u3[i]; };
+  for (int q = 0; q < Pe; q++) {
+    const int s0 = q * Cl, e = std::min(L, s0 + Cl);
+    double *F = &T[128 + 192 * (size_t)q];
+    for (int i = s0; i < e; i++) {
+      const int k = i - s0;
+      F[k] = L1(i); F[32 + k] = L2(i);
+      F[64 + k] = u1[i]; F[96 + k] = U2(i);
+      F[128 + k] = U3(i);
+    }
+    for (int which = 0; which < 2; which++) {
+      double ym2 = which == 0 ? 1 : 0, ym1 = which == 0 ? 0 : 1;
+      for (int i = s0; i < e; i++) {
+        const double y = -L1(i) * ym2 - L2(i) * ym1;
+        ym2 = ym1; ym1 = y;
+      }
+      T[4 * q + which] = ym2;
+      T[4 * q + 2 + which] = ym1;
+      double xp1 = which == 0 ? 1 : 0, xp2 = which == 0 ? 0 : 1;
+      for (int i = e - 1; i >= s0; i--) {
+        const double x = -U2(i) * xp1 - U3(i) * xp2;
+        xp2 = xp1; xp1 = x;
+      }
+      T[64 + 4 * q + which] = xp1;
+      T[64 + 4 * q + 2 + which] = xp2;
+    }
+  }
+  return T;
+}
+
+// chunk counts the chunked solver can run with (P waves of 64 lanes)
+constexpr int kColP[] = {1, 2, 4, 8, 16};
+constexpr int kNColP = 5;
+
+// capacity needed by P chunks over the lines of direction d (0: P unusable)
+int col_need(const std::vector<Block> &blocks, int d, int P) {
+  int cmax = 0;
+  for (const Block &b : blocks) cmax = std::max(cmax, (b.L[d] + P - 1) / P);
+  return col_capacity(cmax);
+}
+
+// default chunk count: the fewest chunks whose length stays within 24 points
+// (registers for 4+ waves per SIMD), else within 32
+int col_default(const std::vector<Block> &blocks, int d) {
+  for (int lim : {24, 32})
+    for (int P : kColP) {
+      int cmax = 0;
+      for (const Block &b : blocks) cmax = std::max(cmax, (b.L[d] + P - 1) / P);
+      if (cmax <= lim) return P;
+    }
+  return 0;
+}
+
 // GCBC spreading weights over the MBCI points inside a characteristic face
 double gcbc_weights(double sbci[MBCI]) {
   std::vector<double> m = {kBnd[0][3], kBnd[0][4], 0, kBnd[1][2], kBnd[1][3], kBnd[1][4],
@@ -332,6 +404,12 @@ struct tp_context {
   DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
   int nlp[3] = {0, 0, 0};
   size_t lds[3] = {0, 0, 0};
+  bool exact_lines = false;        // sequential one-lane-per-line sweeps
+  int col_fixed = 0;               // requested chunks per line (0: chosen per direction)
+  int colP[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // chunks per line, direction x kind (0: sequential kernel)
+  DevList colwork[3];              // groups of 64 lines
+  // carry maps per block, direction, kind and chunk count index (kColP)
+  std::vector<std::array<const double *, 3 * 2 * 5>> coltab;
   long long npts = 0;
   bool any_visc = false;
   // measurement
@@ -443,8 +521,45 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   a.blocks = c->d_blocks; a.work = c->lines[d].d; a.coef = c->d_coef;
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
   a.m = m; a.acc = acc; a.nlp = c->nlp[d];
-  if (kind == 0) launch_deriv(d, c->lines[d].n, c->lds[d], c->stream, a);
-  else launch_filter(d, c->lines[d].n, c->lds[d], c->stream, a);
+  const int P = c->colP[d][kind];
+  if (!c->exact_lines && P) {
+    a.work = c->colwork[d].d;
+    launch_col(d, kind, col_need(c->blocks, d, P), P, c->colwork[d].n, c->stream, a);
+  } else if (kind == 0) {
+    launch_deriv(d, c->lines[d].n, c->lds[d], c->stream, a);
+  } else {
+    launch_filter(d, c->lines[d].n, c->lds[d], c->stream, a);
+  }
+}
+
+// drop a captured step: it holds the launch configuration it was built with
+void drop_graph(tp_context *c) {
+  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+}
+
+// use P chunks per line for the line solves of direction d, kind (0: the
+// sequential one-lane-per-line kernel); host copies of the blocks only
+void select_col(tp_context *c, int d, int kind, int P) {
+  int pi = -1;
+  for (int i = 0; i < kNColP; i++)
+    if (kColP[i] == P) pi = i;
+  if (pi < 0 || !col_need(c->blocks, d, P)) {
+    c->colP[d][kind] = 0;
+    return;
+  }
+  c->colP[d][kind] = P;
+  for (size_t g = 0; g < c->blocks.size(); g++)
+    c->blocks[g].col[d][kind] = c->coltab[g][(d * 2 + kind) * kNColP + pi];
+}
+
+// copy the host block views to the device after a selection change
+int push_blocks(tp_context *c) {
+  drop_graph(c);
+  if (!c->d_blocks) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_blocks, c->blocks.data(), sizeof(Block) * c->blocks.size(), hipMemcpyHostToDevice));
+  return 0;
 }
 
 void enqueue_filter_step(tp_context *c) {
@@ -708,6 +823,7 @@ int32_t tp_finalize(tp_context *c) {
   if (c->cases.empty()) return fail(TP_E_STATE, "no case added");
   HIPCHK(hipSetDevice(c->device));
   const int ncase = (int)c->cases.size();
+  std::vector<std::vector<double>> lu_d, lu_f;   // host LU factors per block
   // ---- blocks and their device memory ----
   for (int cs = 0; cs < ncase; cs++) {
     HostCase &h = c->cases[cs];
@@ -772,6 +888,8 @@ int32_t tp_finalize(tp_context *c) {
       }
       HIPCHK(hipMemcpy(b.lud, lud.data(), sizeof(double) * lud.size(), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(b.luf, luf.data(), sizeof(double) * luf.size(), hipMemcpyHostToDevice));
+      lu_d.push_back(lud);
+      lu_f.push_back(luf);
       c->blocks.push_back(b);
       c->grids.emplace_back();
       c->npts += b.N;
@@ -779,6 +897,32 @@ int32_t tp_finalize(tp_context *c) {
   }
   const int nb = (int)c->blocks.size();
   int rc;
+  // chunked solver: carry maps for every usable chunk count, 64-line groups
+  c->coltab.assign(nb, {});
+  for (int d = 0; d < 3; d++) {
+    std::vector<Work> cw;
+    for (int g = 0; g < nb; g++) {
+      Block &b = c->blocks[g];
+      const int L = b.L[d], nl = b.nlu, o = b.lu_off[d];
+      for (int pi = 0; pi < kNColP; pi++) {
+        if (!col_need(c->blocks, d, kColP[pi])) continue;
+        for (int kind = 0; kind < 2; kind++) {
+          const std::vector<double> &lu = kind == 0 ? lu_d[g] : lu_f[g];
+          std::vector<double> T = col_tables(L, kColP[pi], &lu[0 * nl + o], &lu[1 * nl + o],
+                                             &lu[2 * nl + o], &lu[3 * nl + o], &lu[4 * nl + o]);
+          double *dptr = nullptr;
+          if ((rc = upload_vec(c, &dptr, T))) return rc;
+          c->coltab[g][(d * 2 + kind) * kNColP + pi] = dptr;
+        }
+      }
+      const int nlines = b.N / L;
+      for (int q = 0; q < nlines; q += 64) cw.push_back({g, q, std::min(64, nlines - q), d});
+    }
+    if ((rc = upload_list(c, c->colwork[d], cw))) return rc;
+    int P = col_default(c->blocks, d);
+    if (c->col_fixed && col_need(c->blocks, d, c->col_fixed)) P = c->col_fixed;
+    for (int kind = 0; kind < 2; kind++) select_col(c, d, kind, P);
+  }
   if ((rc = upload_vec(c, &c->d_blocks, c->blocks))) return rc;
   std::vector<CaseCoef> coefs;
   for (auto &h : c->cases) coefs.push_back(h.coef);
@@ -1121,6 +1265,82 @@ int32_t tp_sync(tp_context *c) {
   if (!c) return fail(TP_E_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int32_t tp_set_option(tp_context *c, int32_t option, int64_t value) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  if (option == TP_OPT_EXACT_LINES) {
+    c->exact_lines = value != 0;
+    drop_graph(c);   // a captured step keeps its kernels: capture again
+    return 0;
+  }
+  if (option == TP_OPT_CHUNKS) {
+    const int P = (int)value;
+    if (P != 0 && P != 1 && P != 2 && P != 4 && P != 8 && P != 16)
+      return fail(TP_E_ARG, "chunks per line must be 0, 1, 2, 4, 8 or 16, not " + std::to_string(value));
+    c->col_fixed = P;
+    if (!c->finalized) return 0;
+    for (int d = 0; d < 3; d++) {
+      int q = P && col_need(c->blocks, d, P) ? P : col_default(c->blocks, d);
+      for (int kind = 0; kind < 2; kind++) select_col(c, d, kind, q);
+    }
+    return push_blocks(c);
+  }
+  return fail(TP_E_ARG, "unknown option " + std::to_string(option));
+}
+
+int32_t tp_autotune(tp_context *c, int32_t reps, int32_t *chosen, double *us) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  if (!c->finalized) return fail(TP_E_STATE, "tp_autotune needs tp_finalize");
+  if (reps < 1) reps = 1;
+  const bool prof = c->prof, exact = c->exact_lines;
+  c->prof = false;
+  c->exact_lines = false;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = 0;
+  for (int d = 0; d < 3 && !rc; d++) {
+    for (int kind = 0; kind < 2 && !rc; kind++) {
+      int best = c->colP[d][kind];
+      float bestms = 1e30f;
+      for (int pi = -1; pi < kNColP && !rc; pi++) {
+        const int P = pi < 0 ? 0 : kColP[pi];   // 0: the sequential kernel
+        if (P && !col_need(c->blocks, d, P)) continue;
+        select_col(c, d, kind, P);
+        if ((rc = push_blocks(c))) break;
+        // rr is scratch between stages: deriv rr(:,1) -> rr(:,2), filter rr(:,3) in place
+        auto once = [&]() {
+          if (kind == 0) run_line(c, 0, d, F_RR, 0, F_RR, 1, 0, 0);
+          else run_line(c, 1, d, F_RR, 2, F_RR, 2, 0, 0);
+        };
+        once(); once();
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(TP_E_HIP, "event record"); break; }
+        for (int r = 0; r < reps; r++) once();
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipGetLastError() != hipSuccess) { rc = fail(TP_E_HIP, "autotune launch failed"); break; }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < bestms) { bestms = ms; best = P; }
+      }
+      if (rc) break;
+      select_col(c, d, kind, best);
+      if (chosen) chosen[d * 2 + kind] = c->colP[d][kind];
+      if (us) us[d * 2 + kind] = bestms < 1e29f ? 1e3 * bestms / reps : 0.0;
+    }
+  }
+  if (!rc) rc = push_blocks(c);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->prof = prof;
+  c->exact_lines = exact;
+  return rc;
+}
+
+int32_t tp_col_config(tp_context *c, int32_t *chunks6) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  for (int i = 0; i < 6; i++) chunks6[i] = c->exact_lines ? 0 : c->colP[i / 2][i % 2];
   return 0;
 }
 

@@ -58,6 +58,7 @@ struct Block {
   double *rpex;
   int *lcsz;
   double *asz, *bsz;
+  const double *col[3][2];    // chunked-solver carry maps per direction, kind (deriv, filter)
 };
 
 // array ids exposed through the C ABI (order shared with include/tandem_hip.h)
@@ -129,7 +130,12 @@ struct StageArgs {
   int o0, o1, o2;    // tv slots for gradient combination (-1..-3 = ss columns)
 };
 
-// launchers (tp_line.hip, tp_stage.hip)
+
+// launchers (tp_line.hip, tp_col.hip, tp_stage.hip)
+// chunked line solves: P waves per 64 lines, chunk capacity C points per lane
+int col_capacity(int chunk);
+size_t col_lds_bytes(int d, int P);
+void launch_col(int d, int kind, int C, int P, int nwg, hipStream_t st, const LineArgs &a);
 void launch_deriv(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a);
 void launch_filter(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a);
 void launch_exchange(int nwg, hipStream_t st, const ExchArgs &a);

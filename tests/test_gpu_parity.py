@@ -28,10 +28,19 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def pair(layout, params):
+def pair(layout, params, exact=False, chunks=0):
     o = po.Oracle(layout, params)
-    s = api.Solver([(layout, params)])
+    s = api.Solver([(layout, params)], chunks=chunks)
+    if exact:
+        s.set_exact_lines(True)
     return o, s
+
+
+# line-solver configurations of the per-call tests: the sequential sweep, and
+# chunked sweeps with fixed chunk counts so that every line is split over
+# several waves (the automatic choice may keep short lines in one chunk)
+CALL_MODES = {"sequential": dict(exact=True), "chunks2": dict(chunks=2),
+              "chunks4": dict(chunks=4), "chunks16": dict(chunks=16), "auto": dict()}
 
 
 LINE_CASES = {
@@ -41,6 +50,7 @@ LINE_CASES = {
     "closure 35 self": L.single_block(13, 12, 13, [[35, 35], [10, 10], [45, 45]]),
     "3x2 split zeta": L.h_layout([14, 18, 14], [12, 14], [31], npcz=[2]),
     "3x2 split xi": L.h_layout([14, 27, 14], [12, 14], [12], npcx=[1, 2, 1]),
+    "long lines": L.single_block(239, 63, 12, [[10, 30], [25, 10], [45, 45]]),
 }
 
 
@@ -54,11 +64,14 @@ def fill_rr(o, s, layout, seed):
     return
 
 
+@pytest.mark.parametrize("mode", list(CALL_MODES))
 @pytest.mark.parametrize("name", list(LINE_CASES))
-def test_deriv_per_call(name):
-    say("deriv per call:", name)
+def test_deriv_per_call(name, mode):
+    """chunked sweeps: <= 1e-13; sequential sweeps: bit-identical"""
+    say("deriv per call:", name, mode)
     lay = LINE_CASES[name]
-    o, s = pair(lay, BASE_PARAMS)
+    exact = mode == "sequential"
+    o, s = pair(lay, BASE_PARAMS, **CALL_MODES[mode])
     for nrt in (1, 0):
         fill_rr(o, s, lay, 5 + nrt)
         o.mpigo(0, nrt, 0); s.mpigo(0, nrt, 0)
@@ -73,14 +86,16 @@ def test_deriv_per_call(name):
                 if np.abs(o.array(b, f"recv{d}")).max() > 0:
                     worst = max(worst, rel(s.download(0, b, f"recv{d}"), o.array(b, f"recv{d}")))
         say("  nrt", nrt, "max rel diff", worst)
-        assert worst <= 1e-13
+        assert worst == 0.0 if exact else worst <= 1e-13
 
 
+@pytest.mark.parametrize("mode", list(CALL_MODES))
 @pytest.mark.parametrize("name", list(LINE_CASES))
-def test_filte_per_call(name):
-    say("filte per call:", name)
+def test_filte_per_call(name, mode):
+    say("filte per call:", name, mode)
     lay = LINE_CASES[name]
-    o, s = pair(lay, BASE_PARAMS)
+    exact = mode == "sequential"
+    o, s = pair(lay, BASE_PARAMS, **CALL_MODES[mode])
     fill_rr(o, s, lay, 11)
     o.mpigo(1, 1, 0); s.mpigo(1, 1, 0)
     for nn in (1, 2, 3):
@@ -88,7 +103,7 @@ def test_filte_per_call(name):
     s.sync()
     worst = max(rel(s.download(0, b, "rr"), o.array(b, "rr")) for b in range(lay.nb))
     say("  max rel diff", worst)
-    assert worst <= 1e-13
+    assert worst == 0.0 if exact else worst <= 1e-13
 
 
 def test_mpigo_periodic_jump():
@@ -107,8 +122,8 @@ def test_mpigo_periodic_jump():
                 assert rel(s.download(0, b, f"recv{d}"), ref) <= 1e-13
 
 
-def geometry_pair(lay, ss, p):
-    o, s = pair(lay, p)
+def geometry_pair(lay, ss, p, exact=False):
+    o, s = pair(lay, p, exact)
     for b in range(lay.nb):
         o.set_grid(b, ss[b]); s.set_grid(0, b, ss[b])
     o.metrics(); s.setup_geometry()
@@ -125,8 +140,8 @@ def test_metrics_and_normals():
             assert r <= 1e-13, (b, name, r)
 
 
-def run_steps(lay, ss, p, nsteps, graph=False):
-    o, s = geometry_pair(lay, ss, p)
+def run_steps(lay, ss, p, nsteps, graph=False, exact=True):
+    o, s = geometry_pair(lay, ss, p, exact)
     o.initialo(); s.initialo(0)
     o.step(nsteps)
     s.step(nsteps, graph=graph)
@@ -134,17 +149,23 @@ def run_steps(lay, ss, p, nsteps, graph=False):
     return o, s
 
 
-def compare_state(o, s, lay, tol, case=0):
+def compare_state(o, s, lay, tol, case=0, acoustic=False):
     """Max difference of the conserved variables relative to their magnitude:
     rho and E by their own maxima, the momentum components by the maximum of
     the momentum vector (a spanwise-uniform flow has rho*w at round-off level,
-    so normalising it by itself would measure noise against noise)."""
+    so normalising it by itself would measure noise against noise).
+    acoustic: the momentum scale is at least rho_max * a_inf (a_inf = 1 in the
+    reference's scaling) -- during the moving-frame ramp the momentum is
+    still ~1e-6 of that, and line solves that round differently from the
+    sequential sweep differ from it by ~1e-16 of rho*a_inf."""
     qs = [s.download(case, b, "qa").reshape(5, -1) for b in range(lay.nb)]
     rs = [o.array(b, "qa").reshape(5, -1) for b in range(lay.nb)]
     for q in qs:
         assert np.isfinite(q).all()
     scale = [max(np.abs(r[v]).max() for r in rs) for v in range(5)]
     mom = max(scale[1:4])
+    if acoustic:
+        mom = max(mom, scale[0])
     scale = [scale[0], mom, mom, mom, scale[4]]
     worst = 0.0
     for q, r in zip(qs, rs):
@@ -158,20 +179,31 @@ def compare_state(o, s, lay, tol, case=0):
     return worst
 
 
-def test_first_steps_tight():
-    say("four full steps (dt > 0 from the third), viscous 3x2 aerofoil case")
+LINE_MODES = pytest.mark.parametrize("exact", [True, False], ids=["sequential", "chunked"])
+
+
+@LINE_MODES
+def test_first_steps_tight(exact):
+    say("four full steps (dt > 0 from the third), viscous 3x2 aerofoil case,",
+        "sequential" if exact else "chunked")
     lay, ss, p = aerofoil_case()
-    o, s = run_steps(lay, ss, p, 4)
+    o, s = run_steps(lay, ss, p, 4, exact=exact)
     assert o.scalars()["dt"] > 0
-    compare_state(o, s, lay, 1e-12)
+    compare_state(o, s, lay, 1e-12, acoustic=not exact)
 
 
+@LINE_MODES
 @pytest.mark.parametrize("nviscous", [1, 0])
-def test_stage_arrays(nviscous):
-    """Every array a stage writes, after one filter sweep and one stage with dt > 0."""
-    say("stage arrays, nviscous =", nviscous)
+def test_stage_arrays(nviscous, exact):
+    """Every array a stage writes, after one filter sweep and one stage with
+    dt > 0.  Sequential line solves: each component against its own maximum,
+    to 1e-12.  Chunked line solves round the chunk carries differently, so
+    an array that is still round-off (the right-hand side de of the
+    near-uniform start holds ~1e-15) is compared against the magnitude of
+    the conserved state instead (rho*a_inf scale)."""
+    say("stage arrays, nviscous =", nviscous, "sequential" if exact else "chunked")
     lay, ss, p = aerofoil_case(nviscous=nviscous)
-    o, s = run_steps(lay, ss, p, 2)
+    o, s = run_steps(lay, ss, p, 2, exact=exact)
     o.filter_step(); s.filter_step()
     o.stage(1); s.stage(1)
     s.sync()
@@ -182,8 +214,9 @@ def test_stage_arrays(nviscous):
             g = s.download(0, b, name); r = o.array(b, name)
             k = 9 if name == "tv" else (5 if name in ("qa", "qo", "de") else 1)
             g = g.reshape(k, -1); r = r.reshape(k, -1)
+            whole = max(np.abs(r).max(), np.abs(o.array(b, "qa")).max())
             for v in range(k):
-                d = rel(g[v], r[v])
+                d = rel(g[v], r[v]) if exact else float(np.abs(g[v] - r[v]).max() / whole)
                 worst = max(worst, d)
                 assert d <= 1e-12, (b, name, v, d)
     say("  max rel diff over stage arrays", worst)
@@ -191,9 +224,10 @@ def test_stage_arrays(nviscous):
         assert abs(o.scalars()[key] - s.scalars(0)[key]) <= 1e-14 * max(1.0, abs(o.scalars()[key]))
 
 
+@LINE_MODES
 @pytest.mark.parametrize("variant", ["viscous", "euler", "les-gcic"])
-def test_hundred_steps(variant):
-    say("100 steps:", variant)
+def test_hundred_steps(variant, exact):
+    say("100 steps:", variant, "sequential" if exact else "chunked")
     if variant == "viscous":
         lay, ss, p = aerofoil_case()
     elif variant == "euler":
@@ -205,16 +239,16 @@ def test_hundred_steps(variant):
         lay, ss, p = aerofoil_case(lzebk=(31,), npcz=[2])
         ss = [np.concatenate([x[:2 * (x.size // 3)], -x[2 * (x.size // 3):]]) for x in ss]
         p = dict(p); p.update(les=1, nextgcic=1, nextrabc=1, wlea=0.4, wlew=0.1)
-    o, s = run_steps(lay, ss, p, 100)
+    o, s = run_steps(lay, ss, p, 100, exact=exact)
     if variant == "les-gcic":
         assert max(o.array(b, "rpex").max() if o.array(b, "rpex").size else 0 for b in range(lay.nb)) == 1
         # the reference keeps the Smagorinsky volume term (-1/yaco)**1.5 in a
         # single-precision array (varr, src/MainVar3D.f90:105; Main3D.f90:664):
         # a one-ulp difference of the device pow() can flip its float32
         # rounding (6e-8 relative at that point), so this path is held to 2e-9
-        compare_state(o, s, lay, 2e-9)
+        compare_state(o, s, lay, 2e-9, acoustic=not exact)
     else:
-        compare_state(o, s, lay, 1e-10)
+        compare_state(o, s, lay, 1e-10, acoustic=not exact)
 
 
 def test_graph_replay_matches_eager():
@@ -234,12 +268,16 @@ def test_graph_replay_matches_eager():
         np.testing.assert_array_equal(s1.download(0, b, "qa"), s2.download(0, b, "qa"))
 
 
-def test_ensemble_equals_alone():
-    say("two cases in one context == each alone")
+@pytest.mark.parametrize("mode", ["sequential", "chunks4"])
+def test_ensemble_equals_alone(mode):
+    """Two cases batched in one context compute exactly what each computes
+    alone (same line-solver configuration)."""
+    say("two cases in one context == each alone:", mode)
+    kw = dict(exact_lines=True) if mode == "sequential" else dict(chunks=4)
     lay1, ss1, p1 = aerofoil_case(lxibk=(16, 20, 16), letbk=(14, 14))
     lay2, ss2, p2 = aerofoil_case(lxibk=(20, 24, 18), letbk=(16, 12), amach1=0.3, reoo=4e4)
-    both = api.Solver([(lay1, p1), (lay2, p2)])
-    a1 = api.Solver([(lay1, p1)]); a2 = api.Solver([(lay2, p2)])
+    both = api.Solver([(lay1, p1), (lay2, p2)], **kw)
+    a1 = api.Solver([(lay1, p1)], **kw); a2 = api.Solver([(lay2, p2)], **kw)
     for s, cs, lay, ss in ((both, 0, lay1, ss1), (both, 1, lay2, ss2), (a1, 0, lay1, ss1), (a2, 0, lay2, ss2)):
         for b in range(lay.nb):
             s.set_grid(cs, b, ss[b])
