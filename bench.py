@@ -28,7 +28,6 @@ from tandemproject_amd import layout as L  # noqa: E402
 
 METRIC = "FP64 Mpoint-RK-steps/s + %HBM peak; ensemble cases/h and p99 latency"
 HBM_PEAK_GBS = 8000.0
-DERIV_BYTES_PER_POINT = 16.0   # one field in + one field out per line-solve launch
 
 
 def workload(scale=1.0):
@@ -70,6 +69,38 @@ def cpu_baseline(seconds_target=15.0):
                       f"{steps} steps, {dt:.1f} s"}
 
 
+def measured_traffic():
+    """HBM bytes per derivative launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes summarised under profiles/ (counter values scaled by a copy of
+    known size); None when no such summary is present."""
+    path = os.path.join(ROOT, "profiles", "line_solve_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def copy_probe(mib):
+    """One device-to-device copy of mib MiB through the HIP runtime (a blit
+    kernel): a reference of known traffic for scaling hardware byte counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a); hip.hipFree(b)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed")
+    print(f"[bench] copy probe: {n} bytes read, {n} bytes written", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +111,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel events)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default line-solver launch shapes")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="after the run, one plain device-to-device copy of MB MiB (a reference kernel "
+                         "of known traffic for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -125,13 +159,18 @@ def main():
     ms, nl, by = prof["deriv"]
     if nl > 0 and ms > 0:
         avg_s = ms / 1e3 / nl
-        per_launch = DERIV_BYTES_PER_POINT * npts
-        achieved = per_launch / avg_s / 1e9
+        # 16 B/pt per launch (field in, derivative out), 24 B/pt for the
+        # launches that accumulate into their output (the eta and zeta flux
+        # derivatives): the library counts these per launch
+        achieved = by / (ms / 1e3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                 "kernel": "compact derivative line solves (k_col / k_deriv, 3 directions)",
-                "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl}
+                "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl,
+                "bytes_per_launch": round(by / nl)}
     breakdown = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1]}
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpoint-steps/s", "n_gpus": 1,

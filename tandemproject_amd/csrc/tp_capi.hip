@@ -516,7 +516,8 @@ void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0) {
 void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_arr, int out_col, int m,
               int acc) {
   if (!c->lines[d].n) return;
-  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, 16.0 * (double)c->npts);
+  // algorithmic bytes: one field in, one out; an accumulated output is also read
+  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, (acc ? 24.0 : 16.0) * (double)c->npts);
   LineArgs a;
   a.blocks = c->d_blocks; a.work = c->lines[d].d; a.coef = c->d_coef;
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
