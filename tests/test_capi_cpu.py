@@ -60,3 +60,30 @@ def test_layout_matches_reference_decomposition():
     le = L.jct[0]
     assert [m[0] for m in le] == [blk(0, 0), blk(1, 0), blk(3, 0), blk(4, 0)]
     assert list(L.jvel[0]) == [1, 3] and list(L.jvel[1]) == [0, 2]
+
+
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="no Fortran compiler in this image")
+def test_fortran_binding_compiles_links_and_runs(tmp_path):
+    """The iso_c_binding module a Fortran host uses (tandemproject_amd/fortran)
+    declares the same entry points as the C header, compiles, links against
+    the library and makes calls that need no GPU."""
+    import subprocess
+    import torch
+    mod = os.path.join(ROOT, "tandemproject_amd", "fortran", "tandem_hip.f90")
+    src = open(mod).read().lower()
+    bound = set(re.findall(r'bind\(c, name="(tp_[a-z0-9_]+)"\)', src))
+    assert bound == set(declared_symbols())
+    prog = os.path.join(ROOT, "tests", "fortran", "check_binding.f90")
+    libdir = os.path.join(ROOT, "tandemproject_amd")
+    subprocess.check_call([FLANG, "-c", "-module-dir", str(tmp_path), mod, "-o", str(tmp_path / "m.o")])
+    exe = str(tmp_path / "check_binding")
+    subprocess.check_call([FLANG, "-I", str(tmp_path), "-module-dir", str(tmp_path), prog, str(tmp_path / "m.o"),
+                           "-L" + libdir, "-ltandem_hip", "-Wl,-rpath," + libdir, "-o", exe])
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "binding ok" in out.stdout
+    if not torch.cuda.is_available():
+        assert "tp_create failed as expected" in out.stdout and "no HIP device" in out.stdout
