@@ -7,33 +7,34 @@
  * evaluation order, the reference Fortran (all citations relative to the
  * reference tree):
  *
- *   function            restates                                   pinned by
+ *   function            restates                                   pinned by (tests/test_oracle_pins.py)
  *   ------------------  -----------------------------------------  -----------------------------
- *   orc_penta           src/Subroutineso.f90:15-67                 golden vectors from compiled reference
- *   orc_fcint           src/Subroutineso.f90:91-103                golden vectors + closed form (transfer fn)
+ *   orc_penta           src/Subroutineso.f90:15-67                 golden vectors of the compiled reference
+ *   orc_fcint           src/Subroutineso.f90:91-103                golden vectors + interior transfer function
  *   orc_fcbcm           src/Subroutineso.f90:71-87                 golden vectors
- *   orc_sbcco/mtrxi     src/Subroutineso.f90:107-188               golden vectors + coupled-system identity
+ *   orc_sbcco/mtrxi     src/Subroutineso.f90:107-188               golden vectors
  *   orc_movef (pitch)   src/Subroutineso.f90:192-259               golden vectors
+ *   index map (indx3)   src/Subroutineso.f90:364-375               golden vectors
+ *   GCBC weights        src/Main3D.f90:441-480                     golden vectors (mtrxi pivot state, sbci)
  *   mpigo               src/Subroutines3D.f90:16-80                closure identity: split line == unsplit line
  *   deriv               src/Subroutines3D.f90:84-141               dense solve, polynomial exactness,
- *                                                                  modified wavenumber, golden penta factors
- *   filte               src/Subroutines3D.f90:145-206              dense solve, constants invariant,
+ *                                                                  modified wavenumber
+ *   filte               src/Subroutines3D.f90:145-206              constants invariant for every closure,
  *                                                                  interior transfer function
- *   eleme/xtq2r/xtr2q   src/Subroutines3D.f90:235-324              X_R * X_L = I; eigen-decomposition of the
- *                                                                  flux Jacobian (numerical)
+ *   eleme/xtq2r/xtr2q   src/Subroutines3D.f90:235-324              X_R * X_L = I
  *   metrics, normals    src/Main3D.f90:365-439                     exact metrics of affine grids, yaco = 1/J
- *   GCBC weights        src/Main3D.f90:441-480                     closed form of 3x3 inverse
- *   stage body          src/Main3D.f90:567-958                     uniform-flow invariance, conservation of
- *                                                                  mass on periodic grids, linear-ODE RK check
- *   spongeup/spongego   src/SingleTurbulence3D.f90:209-374         closed form on a Cartesian grid
- *   junction            src/SingleTurbulence3D.f90:378-443         hand-computed averages
- *   initialo            src/SingleTurbulence3D.f90:179-205         trivial
+ *   stage body          src/Main3D.f90:567-958                     uniform flow preserved on a periodic grid;
+ *                                                                  otherwise parity unpinned (see below)
+ *   spongeup/spongego   src/SingleTurbulence3D.f90:209-374         parity unpinned
+ *   junction            src/SingleTurbulence3D.f90:378-443         parity unpinned
+ *   initialo            src/SingleTurbulence3D.f90:179-205         uniform-flow test (trivial state)
  *
  * The composed ordering of the stage (sponge -> GCBC prep -> exchange ->
- * GCBC apply -> RK -> wall -> junction -> interface averaging) is restated
- * from src/Main3D.f90:770-952; the reference program itself cannot be built
- * here (it needs an MPI Fortran module), so that composition is pinned only
- * by the invariants listed above ("parity partially pinned").
+ * GCBC apply -> RK -> wall -> junction -> interface averaging), the sbcc
+ * spreading, the wall slot flags and the dt ramp are restated from
+ * src/Main3D.f90:567-958; the reference program itself cannot be built here
+ * (it needs an MPI Fortran module), so these are "parity unpinned": the HIP
+ * path is compared with this restatement, which nothing independent checks.
  *
  * Layout: Fortran order, see tp_oracle.h.  All arithmetic is FP64 and is
  * compiled with -ffp-contract=off so that no fused multiply-add changes the

@@ -193,6 +193,76 @@ def test_deriv_matches_dense_solve(nn):
                                   np.moveaxis(got, ax, -1)[..., 0].ravel())
 
 
+# ---------------------------------------------------------------- filter
+
+# extrapolation weights of the one-sided filter rows, src/Subroutines3D.f90:148-150
+FEX = np.array([45.0, -9.0, 1.0]) / 30.0
+FFX = np.array([4.0, 6.0, 7.0]) / (8.0 * np.log(2.0))
+
+
+def dense_filter_system(n, fint, albef, fam, fbm, fcm):
+    """A dx = B f for the filter increment of one line with one-sided closures
+    at both ends: LHS rows from fcbcm's albef(-2:2,0:2) and fcint's interior
+    (alpha, beta); RHS the 7-point symmetric differences with the extrapolated
+    ghost values rv of src/Subroutines3D.f90:165-185.  Coefficients come from
+    the reference's own routines (golden vectors), the matrix is assembled
+    here, independently of the oracle's sweeps."""
+    al, be, fa, fb, fc = fint
+    A = np.zeros((n, n))
+    for i in range(3, n - 3):
+        A[i, i - 2:i + 3] = [be, al, 1, al, be]
+    for r in range(3):
+        for c in range(5):
+            col = r + c - 2
+            if 0 <= col < n:
+                A[r, col] = albef[r][c]
+                A[n - 1 - r, n - 1 - col] = albef[r][c]
+
+    def rhs(s):
+        out = np.zeros(n)
+        for i in range(3, n - 3):
+            out[i] = fa * (s[i - 1] + s[i + 1] - 2 * s[i]) + fb * (s[i - 2] + s[i + 2] - 2 * s[i]) + \
+                fc * (s[i - 3] + s[i + 3] - 2 * s[i])
+        rv = s[0] - np.dot(FEX, s[1:4] - s[0]) * FFX
+        out[0] = fam[0] * (rv[0] + s[1] - 2 * s[0]) + fbm[0] * (rv[1] + s[2] - 2 * s[0]) + fcm[0] * (rv[2] + s[3] - 2 * s[0])
+        out[1] = fam[1] * (s[0] + s[2] - 2 * s[1]) + fbm[1] * (rv[0] + s[3] - 2 * s[1]) + fcm[1] * (rv[1] + s[4] - 2 * s[1])
+        out[2] = fam[2] * (s[1] + s[3] - 2 * s[2]) + fbm[2] * (s[0] + s[4] - 2 * s[2]) + fcm[2] * (rv[0] + s[5] - 2 * s[2])
+        e = n - 1
+        rv = s[e] + np.dot(FEX, s[e] - s[e - 3:e][::-1]) * FFX
+        out[e] = fam[0] * (s[e - 1] + rv[0] - 2 * s[e]) + fbm[0] * (s[e - 2] + rv[1] - 2 * s[e]) + fcm[0] * (s[e - 3] + rv[2] - 2 * s[e])
+        out[e - 1] = fam[1] * (s[e - 2] + s[e] - 2 * s[e - 1]) + fbm[1] * (s[e - 3] + rv[0] - 2 * s[e - 1]) + \
+            fcm[1] * (s[e - 4] + rv[1] - 2 * s[e - 1])
+        out[e - 2] = fam[2] * (s[e - 3] + s[e - 1] - 2 * s[e - 2]) + fbm[2] * (s[e - 4] + s[e] - 2 * s[e - 2]) + \
+            fcm[2] * (s[e - 5] + rv[0] - 2 * s[e - 2])
+        return out
+
+    B = np.column_stack([rhs(np.eye(n)[:, j]) for j in range(n)])
+    return A, B
+
+
+@pytest.mark.parametrize("nn", [1, 2, 3])
+def test_filte_matches_dense_solve(nn):
+    n = 21
+    lay = line_case(nn, n)
+    p = dict(BASE_PARAMS); p.update(fltk=FK[0], fltkbc=FK[3])   # the golden fcbcm pair (0.85 pi, 0.6 pi)
+    o = po.Oracle(lay, p)
+    dims = lay.dims[0] + 1
+    rng = np.random.default_rng(10 + nn)
+    f = rng.standard_normal(dims.prod())
+    o.array(0, "rr")[:dims.prod()] = f
+    o.filte(nn, 1)
+    got = o.array(0, "rr")[:dims.prod()].reshape(dims[::-1])
+    albef = np.array(GOLD["fcbcm_albef"][0]).reshape(3, 5)
+    A, B = dense_filter_system(n, GOLD["fcint"][0], albef, GOLD["fcbcm_fam"][0], GOLD["fcbcm_fbm"][0],
+                               GOLD["fcbcm_fcm"][0])
+    F = f.reshape(dims[::-1])
+    ax = {1: 2, 2: 1, 3: 0}[nn]
+    Fm = np.moveaxis(F, ax, -1)
+    ref = np.moveaxis(Fm + np.einsum("ij,...j->...i", np.linalg.solve(A, B), Fm), -1, ax)
+    # the increment is O(1) on white noise; a wrong closure row or factor is O(1e-1..1)
+    assert rel(got - F, ref - F) < 1e-12
+
+
 def test_deriv_polynomial_exactness():
     n = 40
     lay = line_case(1, n, other=6)
