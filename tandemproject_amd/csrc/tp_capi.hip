@@ -583,31 +583,26 @@ void enqueue_stage(tp_context *c, int nk) {
     launch_sc_dt(c->d_coef, c->d_sc, c->d_part, c->d_wr, ncase, 1, c->stream);
   }
   if (c->any_visc) {
-    // src/Main3D.f90:627-651: gradients of u, v, w, T combined with the metrics
-    // tv slots: 0 txx 1 tyy 2 tzz 3 txy 4 tyz 5 tzx 6 hxx 7 hyy 8 hzz; -1..-3 ss
-    const int slots[4][3] = {{0, 8, 5}, {3, 1, 6}, {7, 4, 2}, {-1, -2, -3}};
-    run_points(c, PK_MU, a, K_VISC, 16.0);
+    // src/Main3D.f90:627-692: gradients of u, v, w, T (de(:,2:5)) along the
+    // three index directions into fx(:,3f+d), then one pass forming the
+    // stresses and heat fluxes
     for (int f = 0; f < 4; f++) {
       run_exchange(c, 0, 1, 0, F_DE, f + 1);
-      run_line(c, 0, 2, F_DE, f + 1, F_RR, 2, 0, 0);
-      run_line(c, 0, 1, F_DE, f + 1, F_RR, 1, 0, 0);
-      run_line(c, 0, 0, F_DE, f + 1, F_RR, 0, 0, 0);
-      StageArgs g = a;
-      g.o0 = slots[f][0]; g.o1 = slots[f][1]; g.o2 = slots[f][2];
-      run_points(c, PK_GRAD, g, K_VISC, 120.0);
+      run_line(c, 0, 2, F_DE, f + 1, F_FX, 3 * f + 2, 0, 0);
+      run_line(c, 0, 1, F_DE, f + 1, F_FX, 3 * f + 1, 0, 0);
+      run_line(c, 0, 0, F_DE, f + 1, F_FX, 3 * f + 0, 0, 0);
     }
-    run_points(c, PK_STRESS, a, K_VISC, 200.0);
+    run_points(c, PK_GRADSTRESS, a, K_VISC, 8.0 * (26 + 10));
   }
-  run_points(c, PK_CONTRA, a, K_FLUX, 120.0);
+  // src/Main3D.f90:710-768: all 15 flux components in one pass, then
+  // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
+  run_points(c, PK_FLUXALL, a, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 18 + 15));
   for (int m = 1; m <= 5; m++) {
-    StageArgs f = a;
-    f.m = m;
-    run_points(c, PK_FLUX, f, K_FLUX, 150.0);
-    run_exchange(c, 0, 0, 0, F_RR, 0);
-    // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
-    run_line(c, 0, 0, F_RR, 0, F_DE, m - 1, m, 0);
-    run_line(c, 0, 1, F_RR, 1, F_DE, m - 1, m, 1);
-    run_line(c, 0, 2, F_RR, 2, F_DE, m - 1, m, 1);
+    const int f0 = 3 * (m - 1);
+    run_exchange(c, 0, 0, 0, F_FX, f0);
+    run_line(c, 0, 0, F_FX, f0 + 0, F_DE, m - 1, m, 0);
+    run_line(c, 0, 1, F_FX, f0 + 1, F_DE, m - 1, m, 1);
+    run_line(c, 0, 2, F_FX, f0 + 2, F_DE, m - 1, m, 1);
   }
   if (c->sponge.n) {
     Timed t(c, K_MISC, 0.0);
@@ -859,7 +854,7 @@ int32_t tp_finalize(tp_context *c) {
       const size_t N = b.N;
       size_t o_qa = take(5 * N), o_qo = take(5 * N), o_de = take(5 * N), o_rr = take(3 * N), o_ss = take(3 * N);
       size_t o_xi = take(3 * N), o_et = take(3 * N), o_ze = take(3 * N), o_p = take(N), o_ya = take(N);
-      size_t o_tv = take(9 * N);
+      size_t o_tv = take(9 * N), o_fx = take(15 * N);
       size_t o_f[3][4];
       for (int d = 0; d < 3; d++) {
         o_f[d][0] = take(10 * (size_t)b.F[d]); o_f[d][1] = take(10 * (size_t)b.F[d]);
@@ -873,6 +868,7 @@ int32_t tp_finalize(tp_context *c) {
       auto P = [&](size_t o) { return (double *)(base + o); };
       b.qa = P(o_qa); b.qo = P(o_qo); b.de = P(o_de); b.rr = P(o_rr); b.ss = P(o_ss);
       b.xim = P(o_xi); b.etm = P(o_et); b.zem = P(o_ze); b.p = P(o_p); b.yaco = P(o_ya); b.tv = P(o_tv);
+      b.fx = P(o_fx);
       for (int d = 0; d < 3; d++) {
         b.drva[d] = P(o_f[d][0]); b.drvb[d] = P(o_f[d][1]); b.recv[d] = P(o_f[d][2]); b.cm[d] = P(o_f[d][3]);
       }

@@ -53,6 +53,7 @@ struct Block {
   int wall_off[3][2];         // rpex offset of each wall face (-1 none)
   int nsponge;
   double *qa, *qo, *de, *rr, *ss, *xim, *etm, *zem, *p, *yaco, *tv;
+  double *fx;                 // [15][N] scratch: gradients of u,v,w,T, then the 15 flux components
   double *drva[3], *drvb[3], *recv[3], *cm[3];
   double *lud, *luf;          // [5][nlu]: l1, l2, u1, u2, u3 for deriv / filter
   double *rpex;
@@ -70,11 +71,11 @@ enum ArrayId {
 };
 
 // field selector for generic line kernels: (array, column)
-enum FieldArr { F_RR = 0, F_DE, F_QA, F_SS, F_QO };
+enum FieldArr { F_RR = 0, F_DE, F_QA, F_SS, F_QO, F_FX };
 
 __host__ __device__ inline double *field_ptr(const Block &b, int arr, int col) {
   double *base = arr == F_RR ? b.rr : arr == F_DE ? b.de : arr == F_QA ? b.qa
-               : arr == F_SS ? b.ss : b.qo;
+               : arr == F_SS ? b.ss : arr == F_QO ? b.qo : b.fx;
   return base + (size_t)col * b.N;
 }
 
@@ -149,8 +150,8 @@ void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st);
 void launch_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk,
                      hipStream_t st);
 
-enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_GRAD, PK_STRESS, PK_CONTRA, PK_FLUX, PK_SPONGE, PK_RK,
-                   PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS, PK_MU };
+enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS,
+                   PK_GRADSTRESS = 13, PK_FLUXALL };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS };
 
 }  // namespace tp

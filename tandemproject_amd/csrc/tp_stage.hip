@@ -210,36 +210,40 @@ __global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
   if (threadIdx.x == 0) { a.part[2 * blockIdx.x] = mi[0]; a.part[2 * blockIdx.x + 1] = mv[0]; }
 }
 
-// ---- gradient combination with the metrics, Main3D.f90:631-651 ----
-// rr(:,1:3) hold d/dxi, d/deta, d/dzeta; outputs go to tv slots o0..o2
-// (negative slot -1..-3 = ss column 0..2)
-__device__ inline double *tv_slot(const Block &b, int o) {
-  return o >= 0 ? b.tv + (size_t)o * b.N : b.ss + (size_t)(-o - 1) * b.N;
-}
-__global__ void __launch_bounds__(256) k_grad_combine(StageArgs a) {
-  WORK_POINTS
-  double *o0 = tv_slot(b, a.o0), *o1 = tv_slot(b, a.o1), *o2 = tv_slot(b, a.o2);
-  const double *r1 = b.rr, *r2 = b.rr + N, *r3 = b.rr + 2 * N;
-  const double *xi = b.xim, *et = b.etm, *ze = b.zem;
-  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    double g1 = r1[l], g2 = r2[l], g3 = r3[l];
-    o0[l] = xi[l] * g1 + et[l] * g2 + ze[l] * g3;
-    o1[l] = xi[N + l] * g1 + et[N + l] * g2 + ze[N + l] * g3;
-    o2[l] = xi[2 * N + l] * g1 + et[2 * N + l] * g2 + ze[2 * N + l] * g3;
-  }
-}
-
-// ---- stresses and heat fluxes, Main3D.f90:653-692 ----
-__global__ void __launch_bounds__(256) k_stress(StageArgs a) {
+// ---- viscous stresses and heat fluxes, Main3D.f90:627-692, one pass ----
+// fx(:,3f+d) holds d(field f)/d(index d) for f = u, v, w, T (de(:,2:5)) and
+// d = xi, eta, zeta.  Each gradient is combined with the metric cofactors
+// exactly as the reference does into its work arrays (Main3D.f90:631-651:
+// u -> txx, hzz, tzx; v -> txy, tyy, hxx; w -> hyy, tyz, tzz; T -> ss(:,1:3))
+// and the stresses and heat fluxes follow in the reference's operation order
+// (Main3D.f90:653-692); the viscosity is ss(:,1) (the reference copies it
+// to de(:,1) first, :627, because its T gradient overwrites ss).  Reads the
+// 12 gradients and the metrics once instead of once per field.
+__global__ void __launch_bounds__(256) k_grad_stress(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
   const Phys ph = phys();
+  const double *g = b.fx, *xi = b.xim, *et = b.etm, *ze = b.zem;
   double *tv = b.tv, *de = b.de;
   const double *ss = b.ss, *ya = b.yaco, *qa = b.qa;
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    double txx = tv[l], tyy = tv[N + l], tzz = tv[2 * N + l], txy = tv[3 * N + l], tyz = tv[4 * N + l],
-           tzx = tv[5 * N + l], hxx = tv[6 * N + l], hyy = tv[7 * N + l], hzz = tv[8 * N + l];
-    double r1 = de[l] * ya[l];
+    const double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
+    const double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
+    const double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
+    double cb[4][3];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+      const double g1 = g[(size_t)(3 * f) * N + l], g2 = g[(size_t)(3 * f + 1) * N + l],
+                   g3 = g[(size_t)(3 * f + 2) * N + l];
+      cb[f][0] = x1 * g1 + e1 * g2 + z1 * g3;
+      cb[f][1] = x2 * g1 + e2 * g2 + z2 * g3;
+      cb[f][2] = x3 * g1 + e3 * g2 + z3 * g3;
+    }
+    double txx = cb[0][0], hzz = cb[0][1], tzx = cb[0][2];
+    double txy = cb[1][0], tyy = cb[1][1], hxx = cb[1][2];
+    double hyy = cb[2][0], tyz = cb[2][1], tzz = cb[2][2];
+    const double sx = cb[3][0], sy = cb[3][1], sz = cb[3][2];
+    double r1 = ss[l] * ya[l];
     double r2 = ph.gamm1prndtli * r1, d5;
     if (c.les == 1) {
       double d1 = txx * txx + tyy * tyy + tzz * tzz + (hzz + txy) * (hzz + txy) + (hyy + tzx) * (hyy + tzx) +
@@ -261,77 +265,83 @@ __global__ void __launch_bounds__(256) k_stress(StageArgs a) {
       txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;
     }
     de[4 * N + l] = d5;
-    double u = de[N + l], v = de[2 * N + l], ww = de[3 * N + l];
-    hxx = r2 * ss[l] + u * txx + v * txy + ww * tzx;
-    hyy = r2 * ss[N + l] + u * txy + v * tyy + ww * tyz;
-    hzz = r2 * ss[2 * N + l] + u * tzx + v * tyz + ww * tzz;
+    const double u = de[N + l], v = de[2 * N + l], ww = de[3 * N + l];
+    hxx = r2 * sx + u * txx + v * txy + ww * tzx;
+    hyy = r2 * sy + u * txy + v * tyy + ww * tyz;
+    hzz = r2 * sz + u * tzx + v * tyz + ww * tzz;
     tv[l] = txx; tv[N + l] = tyy; tv[2 * N + l] = tzz; tv[3 * N + l] = txy; tv[4 * N + l] = tyz;
     tv[5 * N + l] = tzx; tv[6 * N + l] = hxx; tv[7 * N + l] = hyy; tv[8 * N + l] = hzz;
   }
 }
 
-// ---- contravariant velocities, Main3D.f90:710-715 ----
-__global__ void __launch_bounds__(256) k_contra(StageArgs a) {
-  WORK_POINTS
-  const double *sc = a.sc + b.cs * S_NSCALAR;
-  const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
-  const double *xi = b.xim, *et = b.etm, *ze = b.zem, *de = b.de;
-  double *ss = b.ss;
-  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    double r1 = de[N + l] + u0, r2 = de[2 * N + l] + u1, r3 = de[3 * N + l] + u2;
-    ss[l] = xi[l] * r1 + xi[N + l] * r2 + xi[2 * N + l] * r3;
-    ss[N + l] = et[l] * r1 + et[N + l] * r2 + et[2 * N + l] * r3;
-    ss[2 * N + l] = ze[l] * r1 + ze[N + l] * r2 + ze[2 * N + l] * r3;
-  }
-}
-
-// ---- metric-weighted Euler + viscous flux of field m, Main3D.f90:718-766 ----
-__global__ void __launch_bounds__(256) k_flux(StageArgs a) {
+// ---- contravariant velocities and the metric-weighted Euler + viscous
+// fluxes of all five fields, Main3D.f90:710-766, one pass ----
+// fx(:,3(m-1)+d) = flux of field m along direction d, computed with the
+// reference's operations: contravariant U_d = m_d . (u + umf) (:710-715),
+// then per field (:718-766).  The reference evaluates one field at a time
+// into rr(:,1:3) (and puts E + p into de(:,5), which its m = 5 derivative
+// then overwrites); here the shared inputs are read once for all 15 fluxes.
+__global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
   const double *sc = a.sc + b.cs * S_NSCALAR;
   const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
-  const int m = a.m, visc = c.nviscous == 1;
-  const double *qa = b.qa, *ss = b.ss, *xi = b.xim, *et = b.etm, *ze = b.zem, *p = b.p, *tv = b.tv;
-  double *rr = b.rr;
+  const int visc = c.nviscous == 1;
+  const double *qa = b.qa, *xi = b.xim, *et = b.etm, *ze = b.zem, *p = b.p, *tv = b.tv, *de = b.de;
+  double *fx = b.fx;
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    double r1, r2, r3;
-    double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
-    double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
-    double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
-    double s1 = ss[l], s2 = ss[N + l], s3 = ss[2 * N + l], pl = p[l];
-    if (m == 1) {
-      double q = qa[l];
-      r1 = q * s1; r2 = q * s2; r3 = q * s3;
-    } else if (m <= 4) {
-      double q = qa[(m - 1) * N + l];
-      double mx = m == 2 ? x1 : m == 3 ? x2 : x3;
-      double me = m == 2 ? e1 : m == 3 ? e2 : e3;
-      double mz = m == 2 ? z1 : m == 3 ? z2 : z3;
-      r1 = q * s1 + mx * pl; r2 = q * s2 + me * pl; r3 = q * s3 + mz * pl;
+    const double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
+    const double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
+    const double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
+    const double q1 = de[N + l] + u0, q2 = de[2 * N + l] + u1, q3 = de[3 * N + l] + u2;
+    const double s1 = x1 * q1 + x2 * q2 + x3 * q3;
+    const double s2 = e1 * q1 + e2 * q2 + e3 * q3;
+    const double s3 = z1 * q1 + z2 * q2 + z3 * q3;
+    const double pl = p[l];
+    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (visc) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) t[k] = tv[(size_t)k * N + l];
+    }
+    auto put = [&](int m, double r1, double r2, double r3) {
+      fx[(size_t)(3 * (m - 1)) * N + l] = r1;
+      fx[(size_t)(3 * (m - 1) + 1) * N + l] = r2;
+      fx[(size_t)(3 * (m - 1) + 2) * N + l] = r3;
+    };
+    {
+      const double q = qa[l];
+      put(1, q * s1, q * s2, q * s3);
+    }
+#pragma unroll
+    for (int m = 2; m <= 4; m++) {
+      const double q = qa[(size_t)(m - 1) * N + l];
+      const double mx = m == 2 ? x1 : m == 3 ? x2 : x3;
+      const double me = m == 2 ? e1 : m == 3 ? e2 : e3;
+      const double mz = m == 2 ? z1 : m == 3 ? z2 : z3;
+      double r1 = q * s1 + mx * pl, r2 = q * s2 + me * pl, r3 = q * s3 + mz * pl;
       if (visc) {
-        double ta, tb, tc;
-        if (m == 2) { ta = tv[l]; tb = tv[3 * N + l]; tc = tv[5 * N + l]; }
-        else if (m == 3) { ta = tv[3 * N + l]; tb = tv[N + l]; tc = tv[4 * N + l]; }
-        else { ta = tv[5 * N + l]; tb = tv[4 * N + l]; tc = tv[2 * N + l]; }
+        // (txx, txy, tzx), (txy, tyy, tyz), (tzx, tyz, tzz): tv slots 0..5 = txx tyy tzz txy tyz tzx
+        const double ta = m == 2 ? t[0] : m == 3 ? t[3] : t[5];
+        const double tb = m == 2 ? t[3] : m == 3 ? t[1] : t[4];
+        const double tc = m == 2 ? t[5] : m == 3 ? t[4] : t[2];
         r1 = r1 - x1 * ta - x2 * tb - x3 * tc;
         r2 = r2 - e1 * ta - e2 * tb - e3 * tc;
         r3 = r3 - z1 * ta - z2 * tb - z3 * tc;
       }
-    } else {
-      double d5 = qa[4 * N + l] + pl;
-      b.de[4 * N + l] = d5;
-      r1 = d5 * s1 - pl * (u0 * x1 + u1 * x2 + u2 * x3);
-      r2 = d5 * s2 - pl * (u0 * e1 + u1 * e2 + u2 * e3);
-      r3 = d5 * s3 - pl * (u0 * z1 + u1 * z2 + u2 * z3);
-      if (visc) {
-        double ha = tv[6 * N + l], hb = tv[7 * N + l], hc = tv[8 * N + l];
-        r1 = r1 - x1 * ha - x2 * hb - x3 * hc;
-        r2 = r2 - e1 * ha - e2 * hb - e3 * hc;
-        r3 = r3 - z1 * ha - z2 * hb - z3 * hc;
-      }
+      put(m, r1, r2, r3);
     }
-    rr[l] = r1; rr[N + l] = r2; rr[2 * N + l] = r3;
+    {
+      const double d5 = qa[4 * N + l] + pl;
+      double r1 = d5 * s1 - pl * (u0 * x1 + u1 * x2 + u2 * x3);
+      double r2 = d5 * s2 - pl * (u0 * e1 + u1 * e2 + u2 * e3);
+      double r3 = d5 * s3 - pl * (u0 * z1 + u1 * z2 + u2 * z3);
+      if (visc) {
+        r1 = r1 - x1 * t[6] - x2 * t[7] - x3 * t[8];
+        r2 = r2 - e1 * t[6] - e2 * t[7] - e3 * t[8];
+        r3 = r3 - z1 * t[6] - z2 * t[7] - z3 * t[8];
+      }
+      put(5, r1, r2, r3);
+    }
   }
 }
 
@@ -664,10 +674,6 @@ __global__ void __launch_bounds__(256) k_copy5(StageArgs a) {   // qo = qa
     for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) b.qo[(size_t)v * N + l] = b.qa[(size_t)v * N + l];
 }
 
-__global__ void __launch_bounds__(256) k_mu_to_de(StageArgs a) {  // de(:,1) = ss(:,1), Main3D.f90:627
-  WORK_POINTS
-  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) b.de[l] = b.ss[l];
-}
 
 __global__ void __launch_bounds__(256) k_copy_field(StageArgs a) {  // rr(:,1) = field(arr, col)
   WORK_POINTS
@@ -726,17 +732,14 @@ void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
   switch (which) {
     case 0: LAUNCH(k_primitives, nwg, 256, st, a); break;
     case 1: LAUNCH(k_dt_reduce, nwg, 256, st, a); break;
-    case 2: LAUNCH(k_grad_combine, nwg, 256, st, a); break;
-    case 3: LAUNCH(k_stress, nwg, 256, st, a); break;
-    case 4: LAUNCH(k_contra, nwg, 256, st, a); break;
-    case 5: LAUNCH(k_flux, nwg, 256, st, a); break;
     case 6: LAUNCH(k_sponge, nwg, 256, st, a); break;
     case 7: LAUNCH(k_rk, nwg, 256, st, a); break;
     case 8: LAUNCH(k_copy5, nwg, 256, st, a); break;
     case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;
     case 10: LAUNCH(k_initialo, nwg, 256, st, a); break;
     case 11: LAUNCH(k_metrics, nwg, 256, st, a); break;
-    case 12: LAUNCH(k_mu_to_de, nwg, 256, st, a); break;
+    case 13: LAUNCH(k_grad_stress, nwg, 256, st, a); break;
+    case 14: LAUNCH(k_flux_all, nwg, 256, st, a); break;
   }
 }
 void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a) {
