@@ -188,11 +188,11 @@ __device__ inline int opaque_zero() {
   return z;
 }
 
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// Orders the LDS accesses of one wave around a transpose step.  The LDS
+// instructions of a wave execute in issue order, so no memory fence (which
+// would also wait for the wave's outstanding global loads) is needed; this
+// only keeps the compiler from moving memory operations across.
+__device__ inline void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
 // value of a double held in lane l (wave-uniform result)
 __device__ inline double lane_val(double v, int l) {
@@ -307,17 +307,23 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   // ---- window rows s0-3 .. s0+C+2, clamped to the line ----
   double v[C + 6];
   if (D == 0) {
+    // all global loads of the window first (8 lines x 8 consecutive points per
+    // instruction), then the transposes through LDS piece by piece
+    constexpr int NPC = (C + 6 + 7) / 8;
+    double tmp[NPC][8];
 #pragma unroll
-    for (int t = 0; t < (C + 6 + 7) / 8; t++) {
+    for (int t = 0; t < NPC; t++) {
       const int row = min(max(s0 - 3 + 8 * t + (lane & 7), 0), L - 1);
-      double tmp[8];
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int rl = min((lane >> 3) + 8 * j, nl - 1);
-        tmp[j] = at(src, 8u * (unsigned)((w.a + rl) * b.nx + row));
+        tmp[t][j] = at(src, 8u * (unsigned)((w.a + rl) * b.nx + row));
       }
+    }
 #pragma unroll
-      for (int j = 0; j < 8; j++) tr[(lane & 7) * COL_TR + (lane >> 3) + 8 * j] = tmp[j];
+    for (int t = 0; t < NPC; t++) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) tr[(lane & 7) * COL_TR + (lane >> 3) + 8 * j] = tmp[t][j];
       wave_sync();
 #pragma unroll
       for (int rr = 0; rr < 8; rr++)
@@ -334,7 +340,7 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   }
 
   // ---- right-hand side ----
-  // rows 0..2 and L-3..L-1 use the closure formulas: evaluated once, outside
+  // rows 0..2 and L-3..L-1 use the closure formulas, evaluated once, outside
   // the unrolled loop, by the (at most two) chunks that hold them
   double rb0 = 0.0, rb1 = 0.0, rb2 = 0.0, re2 = 0.0, re1 = 0.0, re0 = 0.0;
   {
@@ -363,6 +369,7 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     }
   }
   double r[C];
+  const bool edge = n > 0 && (s0 <= 2 || s0 + n - 1 >= L - 3);   // chunk holds closure rows
 #pragma unroll
   for (int k = 0; k < C; k++) {
     const int i = s0 + k;
@@ -375,8 +382,10 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
       rv = c.fa * (v[k + 2] + v[k + 4] - res) + c.fb * (v[k + 1] + v[k + 5] - res) +
            c.fc * (v[k] + v[k + 6] - res);
     }
-    rv = i == 0 ? rb0 : i == 1 ? rb1 : i == 2 ? rb2 : rv;
-    rv = i == L - 1 ? re0 : i == L - 2 ? re1 : i == L - 3 ? re2 : rv;
+    if (edge) {
+      rv = i == 0 ? rb0 : i == 1 ? rb1 : i == 2 ? rb2 : rv;
+      rv = i == L - 1 ? re0 : i == L - 2 ? re1 : i == L - 3 ? re2 : rv;
+    }
     r[k] = rv;
   }
 
