@@ -36,14 +36,26 @@ enum {
   TP_OK = 0, TP_E_ARG = 1, TP_E_STATE = 2, TP_E_HIP = 3, TP_E_SIZE = 4, TP_E_LAYOUT = 5
 };
 
-/* integer case parameters, iparams[] (inputo.dat lines 5-10, 20; src/Main3D.f90:39-56) */
-enum { TP_I_NKRK = 0, TP_I_NVISCOUS, TP_I_NSMF, TP_I_LES, TP_I_NEXTRABC, TP_I_NEXTGCIC, TP_I_NPARAM = 16 };
-/* double case parameters, dparams[] (inputo.dat, inputp.dat; fltk/fltkbc in units of pi) */
+/* integer case parameters, iparams[] (inputo.dat lines 5-10, 20; src/Main3D.f90:39-56).
+ * TP_I_INFLOW = 1 switches on the turbulent inflow through the sponge target
+ * (the reference's spongego gust path, src/SingleTurbulence3D.f90:332-372, which
+ * it ships disabled): a velocity table over the inflow-sponge points, filled
+ * from TP_I_NMODES seeded random Fourier modes (tp_inflow_modes), sampled at
+ * TP_I_NTZ+1 times over one period. */
+enum { TP_I_NKRK = 0, TP_I_NVISCOUS, TP_I_NSMF, TP_I_LES, TP_I_NEXTRABC, TP_I_NEXTGCIC,
+       TP_I_INFLOW, TP_I_NMODES, TP_I_SEED, TP_I_NTZ, TP_I_NPARAM = 16 };
+/* double case parameters, dparams[] (inputo.dat, inputp.dat; fltk/fltkbc in units of pi).
+ * Inflow: TP_D_TGUSTO start time (tgusto), TP_D_SLIT length of the periodic
+ * turbulence box along x (the table's period is slit/u_inf, as slit/uoo(1) in
+ * the reference), TP_D_TU turbulence intensity (rms of each velocity
+ * component over |u_inf|), TP_D_LTURB integral length scale, TP_D_GUSTW half
+ * width in y of the band of inflow-sponge points that receive the turbulence. */
 enum {
   TP_D_REOO = 0, TP_D_TEMPOO, TP_D_AMACH1, TP_D_AMACH2, TP_D_AMACH3, TP_D_WTEMP, TP_D_CFL,
   TP_D_TMAX, TP_D_TIMF, TP_D_FLTK, TP_D_FLTKBC, TP_D_DTO, TP_D_AOA0, TP_D_AOA1, TP_D_TPS,
   TP_D_TP, TP_D_SMAGO1, TP_D_SMAGO2, TP_D_DOMLEN, TP_D_SZTH1, TP_D_SZTH2, TP_D_SZXT,
-  TP_D_SZCO, TP_D_SPAN, TP_D_WLEW, TP_D_WLEA, TP_D_NPARAM = 32
+  TP_D_SZCO, TP_D_SPAN, TP_D_WLEW, TP_D_WLEA, TP_D_TGUSTO, TP_D_SLIT, TP_D_TU, TP_D_LTURB,
+  TP_D_GUSTW, TP_D_NPARAM = 32
 };
 /* arrays reachable by tp_upload / tp_download (names of src/MainVar3D.f90:78-102) */
 enum {
@@ -147,6 +159,20 @@ int32_t tp_set_option(tp_context *ctx, int32_t option, int64_t value);
 int32_t tp_autotune(tp_context *ctx, int32_t reps, int32_t *chosen, double *us);
 /* chunks per line in use, [d*2+kind] (0: sequential kernel) */
 int32_t tp_col_config(tp_context *ctx, int32_t *chunks6);
+
+/* ---- turbulent inflow ----
+ * Seeded random Fourier modes of homogeneous isotropic turbulence with a von
+ * Karman spectrum: u'(x) = sum_n a_n cos(k_n . x + phi_n), a_n perpendicular
+ * to k_n (divergence free), <u'.u'> = sum_n |a_n|^2 / 2 = 3 (tu |uoo|)^2
+ * (rms tu|uoo| in each component), k_x a multiple of 2 pi/slit
+ * and k_z of 2 pi/span (periodic in x and in the span; span <= 0: no z
+ * constraint).  Magnitudes log-uniform in [k_e/5, 40 k_e] with k_e =
+ * sqrt(pi) Gamma(5/6) / (Gamma(1/3) lturb), directions uniform on the sphere,
+ * phases uniform; splitmix64 stream from seed.  out8[n*8 + {0..7}] = k_n (3),
+ * a_n (3), phi_n, 0.  Needs no device.  No counterpart in the reference
+ * (its eddy generator is switched off there); used for the inflow table. */
+int32_t tp_inflow_modes(int32_t nmodes, uint64_t seed, double tu, double lturb, const double *uoo3,
+                        double slit, double span, double *out8);
 
 /* ---- measurement ---- */
 /* record HIP events around every launch (on the context's stream) */

@@ -12,16 +12,17 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-I_NKRK, I_NVISCOUS, I_NSMF, I_LES, I_NEXTRABC, I_NEXTGCIC = range(6)
+I_NKRK, I_NVISCOUS, I_NSMF, I_LES, I_NEXTRABC, I_NEXTGCIC, I_INFLOW, I_NMODES, I_SEED, I_NTZ = range(10)
 N_IPARAM = 16
 D_NAMES = ["reoo", "tempoo", "amach1", "amach2", "amach3", "wtemp", "cfl", "tmax",
            "timf", "fltk", "fltkbc", "dto", "aoa0", "aoa1", "tps", "tp", "smago1",
-           "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea"]
+           "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea",
+           "tgusto", "slit", "tu", "lturb", "gustw"]
 N_DPARAM = 32
 A_NAMES = ["qa", "qo", "de", "rr", "ss", "xim", "etm", "zem", "p", "yaco", "tv",
            "drva1", "drva2", "drva3", "drvb1", "drvb2", "drvb3", "send1", "send2", "send3",
            "recv1", "recv2", "recv3", "cm1", "cm2", "cm3", "xu", "xl", "yu", "yl",
-           "rpex", "sbcc", "asz", "bsz"]
+           "rpex", "sbcc", "asz", "bsz", "vito", "atz"]
 S_NAMES = ["n", "ndt", "dt", "dts", "dte", "timo", "umf1", "umf2", "umf3", "dudt1",
            "dudt2", "dudt3", "dtk", "dtko", "aoa"]
 
@@ -64,6 +65,9 @@ def lib():
         L.orc_xtr2q.argtypes = [dp, ctypes.c_double, dp, dp, dp]
         L.orc_movef.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4 + [dp] + \
             [ctypes.c_double] * 5 + [dp, dp, dp]
+        L.orc_set_inflow_modes.argtypes = [P, ctypes.c_int, dp]
+        L.orc_inflow_weights.restype = ctypes.c_int
+        L.orc_inflow_weights.argtypes = [P, ctypes.c_double, ctypes.c_double, ip, dp, dp]
         _lib = L
     return _lib
 
@@ -93,6 +97,10 @@ class Oracle:
         ipar[I_LES] = params.get("les", 0)
         ipar[I_NEXTRABC] = params.get("nextrabc", 0)
         ipar[I_NEXTGCIC] = params.get("nextgcic", 0)
+        ipar[I_INFLOW] = params.get("inflow", 0)
+        ipar[I_NMODES] = params.get("nmodes", 0)
+        ipar[I_SEED] = params.get("seed", 0)
+        ipar[I_NTZ] = params.get("ntz", 0)
         dpar = np.zeros(N_DPARAM)
         for i, k in enumerate(D_NAMES):
             dpar[i] = params.get(k, 0.0)
@@ -102,6 +110,18 @@ class Oracle:
         self.h = L.orc_create(layout.nb, _ip(d), _ip(b), _ip(n), _ip(s), layout.njct,
                               _ip(j), _ip(v), _ip(ipar), _dp(dpar))
         self.nb = layout.nb
+
+    def set_inflow_modes(self, modes):
+        """Fourier modes (n, 8) the inflow table is built from (before metrics)."""
+        m = np.ascontiguousarray(modes, np.float64)
+        self._modes = m
+        lib().orc_set_inflow_modes(self.h, m.shape[0], _dp(m))
+
+    def inflow_weights(self, timo, dtk):
+        """Table rows, node times and weights of the time interpolation, or None."""
+        il = np.zeros(5, np.int32); tl = np.zeros(5); w = np.zeros(5)
+        on = lib().orc_inflow_weights(self.h, float(timo), float(dtk), _ip(il), _dp(tl), _dp(w))
+        return (il, tl, w) if on else None
 
     def __del__(self):
         if getattr(self, "h", None):

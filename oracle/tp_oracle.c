@@ -105,6 +105,9 @@ typedef struct {
   int lsz;          /* sponge points - 1 */
   int *lcsz;
   double *asz, *bsz;
+  int ltz;          /* inflow points - 1 */
+  int *lctz;        /* their sponge-list positions */
+  double *atz, *vito;   /* ramp; table vito(0:ltz, 0:ntz, 3) */
   double xbc_ra0;   /* state of the global ra0 seen by extrabcc */
 } oblk;
 
@@ -114,6 +117,8 @@ struct orc {
   int njct;
   int *jct, *jvel;
   int nkrk, nviscous, nsmf, les, nextrabc, nextgcic;
+  int inflow, nmodes, ntz;
+  double tgusto, slit, gustw, *modes;
   double reoo, tempoo, amach1, amach2, amach3, wtemp, cfl, tmax, timf, fltk,
       fltkbc, dto, aoa0, aoa1, tps, tp, smago1, smago2, domlen, szth1, szth2,
       szxt, szco, span, wlew, wlea;
@@ -608,6 +613,8 @@ orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
   c->nkrk = ip[ORC_I_NKRK]; c->nviscous = ip[ORC_I_NVISCOUS];
   c->nsmf = ip[ORC_I_NSMF]; c->les = ip[ORC_I_LES];
   c->nextrabc = ip[ORC_I_NEXTRABC]; c->nextgcic = ip[ORC_I_NEXTGCIC];
+  c->inflow = ip[ORC_I_INFLOW]; c->ntz = ip[ORC_I_NTZ];
+  c->tgusto = dp[ORC_D_TGUSTO]; c->slit = dp[ORC_D_SLIT]; c->gustw = dp[ORC_D_GUSTW];
   c->reoo = dp[ORC_D_REOO]; c->tempoo = dp[ORC_D_TEMPOO];
   c->amach1 = dp[ORC_D_AMACH1]; c->amach2 = dp[ORC_D_AMACH2]; c->amach3 = dp[ORC_D_AMACH3];
   c->wtemp = dp[ORC_D_WTEMP]; c->cfl = dp[ORC_D_CFL]; c->tmax = dp[ORC_D_TMAX];
@@ -710,8 +717,10 @@ void orc_destroy(orc *c) {
     }
     free(B->xu); free(B->xl); free(B->yu); free(B->yl);
     free(B->rpex); free(B->sbcc); free(B->lcsz); free(B->asz); free(B->bsz);
+    free(B->lctz); free(B->atz); free(B->vito);
   }
   free(c->b); free(c->jct); free(c->jvel); free(c);
+  free(c->modes);
 }
 
 int orc_npoints(const orc *c, int b) { return c->b[b].N; }
@@ -735,6 +744,8 @@ int orc_count(const orc *c, int b, int w) {
     case ORC_A_RPEX: return B->nwall;
     case ORC_A_SBCC: return B->nsbcc;
     case ORC_A_ASZ: case ORC_A_BSZ: return B->lsz + 1;
+    case ORC_A_VITO: return B->vito ? 3 * (B->ltz + 1) * (c->ntz + 1) : 0;
+    case ORC_A_ATZ: return B->atz ? B->ltz + 1 : 0;
   }
   return 0;
 }
@@ -759,11 +770,55 @@ double *orc_array(orc *c, int b, int w) {
     case ORC_A_YU: return B->yu; case ORC_A_YL: return B->yl;
     case ORC_A_RPEX: return B->rpex; case ORC_A_SBCC: return B->sbcc;
     case ORC_A_ASZ: return B->asz; case ORC_A_BSZ: return B->bsz;
+    case ORC_A_VITO: return B->vito; case ORC_A_ATZ: return B->atz;
   }
   return NULL;
 }
 
 double *orc_scalars(orc *c) { return c->sc; }
+
+void orc_set_inflow_modes(orc *c, int nmodes, const double *modes8) {
+  free(c->modes);
+  c->nmodes = nmodes;
+  c->modes = dalloc((size_t)8 * nmodes);
+  memcpy(c->modes, modes8, sizeof(double) * 8 * nmodes);
+}
+
+/* src/SingleTurbulence3D.f90:336-356: five-point Lagrange interpolation in the
+ * periodic table time tt(ii) = ii*fctr, ii = 0..ntz (fctr = slit/(ntz*uoo(1)),
+ * period ra1 = slit/uoo(1)), at ra3 = (timo - tgusto + dtk) mod ra1 */
+int orc_inflow_weights(const orc *c, double timo, double dtk, int *il5, double *tl5, double *w5) {
+  if (!(timo > c->tgusto)) return 0;
+  double fctr = c->slit / (c->ntz * c->uoo[0]);
+  double ra0 = timo - c->tgusto + dtk, ra1 = c->slit / c->uoo[0], ra2 = ra0 / ra1;
+  double ra3 = ra0 - ra1 * (double)(int)ra2;
+  int is = 0, ie = c->ntz;
+  int ii = 0; /* minloc(abs(tt(:)-ra3)): first minimum */
+  double best = fabs(0 * fctr - ra3);
+  for (int i = 1; i <= ie; i++) {
+    double d = fabs(i * fctr - ra3);
+    if (d < best) { best = d; ii = i; }
+  }
+  int *il = il5 + 2;       /* il[-2..2] */
+  double *tl = tl5 + 2;
+  for (int jj = -2; jj <= 2; jj++) {
+    int v = ii + jj; v = v < is ? is : v; v = v > ie ? ie : v;
+    il[jj] = v; tl[jj] = v * fctr;
+  }
+  if (ii - is == 0) { il[-2] = ie - 2; il[-1] = ie - 1; tl[-2] = il[-2] * fctr - ra1; tl[-1] = il[-1] * fctr - ra1; }
+  if (ii - is == 1) { il[-2] = ie - 1; tl[-2] = il[-2] * fctr - ra1; }
+  if (ie - ii == 1) { il[2] = is + 1; tl[2] = il[2] * fctr + ra1; }
+  if (ie - ii == 0) { il[1] = is + 1; il[2] = is + 2; tl[1] = il[1] * fctr + ra1; tl[2] = il[2] * fctr + ra1; }
+  double al[5];
+  for (int q = -2; q <= 2; q++) al[q + 2] = ra3 - tl[q];
+  for (int jj = -2; jj <= 2; jj++) {
+    double ao = one, bo = one;
+    for (int q = -2; q <= 2; q++)
+      if (q != jj) { ao = ao * al[q + 2]; bo = bo * (tl[jj] - tl[q]); }
+    w5[jj + 2] = ao / bo;
+  }
+  return 1;
+}
 
 void orc_coefficients(orc *c, double *pbci, double *pbco, double *pbcot,
                       double *albef, double *fabc, double *sbci) {
@@ -1185,6 +1240,47 @@ void orc_metrics(orc *c) {
       B->bsz[q] = res * hamm1 * sg[l];
     }
     free(lst); free(sg); free(lm);
+    /* inflow points and their velocity table, src/SingleTurbulence3D.f90:253-292
+     * (the table is filled from the given Fourier modes instead of eddies) */
+    free(B->lctz); free(B->atz); free(B->vito);
+    B->lctz = NULL; B->atz = NULL; B->vito = NULL; B->ltz = -1;
+    if (c->inflow && c->modes) {
+      double ra0 = PI / c->szth1;
+      int *gl = (int *)malloc(sizeof(int) * (B->lsz + 2));
+      int lt = -1;
+      for (int q = 0; q <= B->lsz; q++) {
+        int l = B->lcsz[q];
+        if (B->ss[l] < szp[0][0] && fabs(B->ss[N + l]) < c->gustw) gl[++lt] = q;
+      }
+      B->ltz = lt;
+      B->lctz = gl;
+      B->atz = dalloc(lt + 1);
+      int nt1 = c->ntz + 1;
+      B->vito = dalloc((size_t)3 * (lt + 1) * nt1);
+      double fctr = c->slit / (c->ntz * c->uoo[0]);
+      for (int g = 0; g <= lt; g++) {
+        int l = B->lcsz[gl[g]];
+        double sn = sin(fmin(ra0 * (B->ss[l] + c->domlen), PI / 2));
+        B->atz[g] = sn * sn;
+      }
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+      for (int e = 0; e < (lt + 1) * nt1; e++) {
+        int g = e % (lt + 1), ii = e / (lt + 1);
+        int l = B->lcsz[gl[g]];
+        double tt = ii * fctr;
+        double x0 = B->ss[l] - tt * c->uoo[0], x1 = B->ss[N + l] - tt * c->uoo[1],
+               x2 = B->ss[2 * (size_t)N + l] - tt * c->uoo[2];
+        double s0 = zero, s1 = zero, s2 = zero;
+        for (int n = 0; n < c->nmodes; n++) {
+          const double *m = c->modes + 8 * (size_t)n;
+          double cs = cos(m[0] * x0 + m[1] * x1 + m[2] * x2 + m[6]);
+          s0 = s0 + m[3] * cs; s1 = s1 + m[4] * cs; s2 = s2 + m[5] * cs;
+        }
+        B->vito[((size_t)0 * nt1 + ii) * (lt + 1) + g] = s0;
+        B->vito[((size_t)1 * nt1 + ii) * (lt + 1) + g] = s1;
+        B->vito[((size_t)2 * nt1 + ii) * (lt + 1) + g] = s2;
+      }
+    }
   }
 }
 
@@ -1473,7 +1569,10 @@ void orc_stage(orc *c, int nk) {
       for (int l = 0; l < N; l++) de[l] = rr[l] + rr[N + l] + rr[2 * N + l];
     }
   }
-  /* sponge, src/SingleTurbulence3D.f90:360-372 (inflow eddies disabled: ltz=-1) */
+  /* sponge, src/SingleTurbulence3D.f90:332-372 */
+  int lag_on = 0, il5[5];
+  double tl5[5], w5[5];
+  if (c->inflow) lag_on = orc_inflow_weights(c, c->sc[ORC_S_TIMO], c->sc[ORC_S_DTK], il5, tl5, w5);
   for (int b = 0; b < c->nb; b++) {
     oblk *B = &c->b[b];
     int N = B->N;
@@ -1481,6 +1580,20 @@ void orc_stage(orc *c, int nk) {
       int l = B->lcsz[q];
       B->rr[l] = one;
       B->ss[l] = zero; B->ss[N + l] = zero; B->ss[2 * N + l] = zero;
+    }
+    if (B->vito) {
+      int nt1 = c->ntz + 1, ng = B->ltz + 1;
+      double fctr = half * gamm1;
+      for (int g = 0; g < ng; g++) {
+        double vit[3] = {zero, zero, zero};
+        if (lag_on)
+          for (int jj = 0; jj < 5; jj++)
+            for (int d = 0; d < 3; d++)
+              vit[d] = vit[d] + w5[jj] * B->vito[((size_t)d * nt1 + il5[jj]) * ng + g];
+        int l = B->lcsz[B->lctz[g]];
+        B->rr[l] = pow(one - fctr * (vit[0] * vit[0] + vit[1] * vit[1] + vit[2] * vit[2]), hamm1);
+        for (int d = 0; d < 3; d++) B->ss[(size_t)d * N + l] = B->atz[g] * B->rr[l] * vit[d];
+      }
     }
     for (int q = 0; q <= B->lsz; q++) {
       int l = B->lcsz[q];

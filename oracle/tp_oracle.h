@@ -25,7 +25,7 @@ extern "C" {
 /* integer parameters (index into iparams[]) */
 enum {
   ORC_I_NKRK = 0, ORC_I_NVISCOUS, ORC_I_NSMF, ORC_I_LES, ORC_I_NEXTRABC,
-  ORC_I_NEXTGCIC, ORC_I_NPARAM = 16
+  ORC_I_NEXTGCIC, ORC_I_INFLOW, ORC_I_NMODES, ORC_I_SEED, ORC_I_NTZ, ORC_I_NPARAM = 16
 };
 /* double parameters (index into dparams[]) */
 enum {
@@ -33,7 +33,8 @@ enum {
   ORC_D_WTEMP, ORC_D_CFL, ORC_D_TMAX, ORC_D_TIMF, ORC_D_FLTK, ORC_D_FLTKBC,
   ORC_D_DTO, ORC_D_AOA0, ORC_D_AOA1, ORC_D_TPS, ORC_D_TP, ORC_D_SMAGO1,
   ORC_D_SMAGO2, ORC_D_DOMLEN, ORC_D_SZTH1, ORC_D_SZTH2, ORC_D_SZXT,
-  ORC_D_SZCO, ORC_D_SPAN, ORC_D_WLEW, ORC_D_WLEA, ORC_D_NPARAM = 32
+  ORC_D_SZCO, ORC_D_SPAN, ORC_D_WLEW, ORC_D_WLEA, ORC_D_TGUSTO, ORC_D_SLIT,
+  ORC_D_TU, ORC_D_LTURB, ORC_D_GUSTW, ORC_D_NPARAM = 32
 };
 
 /* per-block arrays reachable through orc_array() */
@@ -43,7 +44,7 @@ enum {
   ORC_A_DRVA1, ORC_A_DRVA2, ORC_A_DRVA3, ORC_A_DRVB1, ORC_A_DRVB2, ORC_A_DRVB3,
   ORC_A_SEND1, ORC_A_SEND2, ORC_A_SEND3, ORC_A_RECV1, ORC_A_RECV2, ORC_A_RECV3,
   ORC_A_CM1, ORC_A_CM2, ORC_A_CM3, ORC_A_XU, ORC_A_XL, ORC_A_YU, ORC_A_YL,
-  ORC_A_RPEX, ORC_A_SBCC, ORC_A_ASZ, ORC_A_BSZ, ORC_A_NARRAY
+  ORC_A_RPEX, ORC_A_SBCC, ORC_A_ASZ, ORC_A_BSZ, ORC_A_VITO, ORC_A_ATZ, ORC_A_NARRAY
 };
 
 /* scalar state reachable through orc_scalars() */
@@ -67,6 +68,12 @@ orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
                 const int32_t *jct, const int32_t *jvel,
                 const int32_t *iparams, const double *dparams);
 void orc_destroy(orc *c);
+/* inflow: the Fourier modes (8 doubles each: k, a, phase, 0) the velocity
+ * table is built from; call before orc_metrics */
+void orc_set_inflow_modes(orc *c, int nmodes, const double *modes8);
+/* time interpolation of the inflow table at (timo, dtk): table rows il[5],
+ * node times tl[5] and weights w[5]; returns 0 before tgusto */
+int orc_inflow_weights(const orc *c, double timo, double dtk, int *il5, double *tl5, double *w5);
 int orc_npoints(const orc *c, int b);
 int orc_count(const orc *c, int b, int which); /* element count of an array */
 double *orc_array(orc *c, int b, int which);

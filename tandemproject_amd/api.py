@@ -14,10 +14,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtandem_hip.so")
 
-I_NAMES = ["nkrk", "nviscous", "nsmf", "les", "nextrabc", "nextgcic"]
+I_NAMES = ["nkrk", "nviscous", "nsmf", "les", "nextrabc", "nextgcic", "inflow", "nmodes", "seed", "ntz"]
 D_NAMES = ["reoo", "tempoo", "amach1", "amach2", "amach3", "wtemp", "cfl", "tmax",
            "timf", "fltk", "fltkbc", "dto", "aoa0", "aoa1", "tps", "tp", "smago1",
-           "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea"]
+           "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea", "tgusto",
+           "slit", "tu", "lturb", "gustw"]
 ARRAYS = ["qa", "qo", "de", "rr", "ss", "xim", "etm", "zem", "p", "yaco", "tv",
           "drva1", "drva2", "drva3", "drvb1", "drvb2", "drvb3", "send1", "send2", "send3",
           "recv1", "recv2", "recv3", "cm1", "cm2", "cm3", "xu", "xl", "yu", "yl", "rpex"]
@@ -29,7 +30,7 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_array_count", "tp_get_scalars", "tp_set_scalars", "tp_mpigo", "tp_deriv",
            "tp_filte", "tp_filter_step", "tp_stage", "tp_step", "tp_step_graph", "tp_sync",
            "tp_profile", "tp_profile_read", "tp_info", "tp_set_option", "tp_autotune",
-           "tp_col_config"]
+           "tp_col_config", "tp_inflow_modes"]
 
 _lib = None
 
@@ -67,6 +68,7 @@ def load(path=LIB_PATH):
         "tp_set_option": [P, i32, i64],
         "tp_autotune": [P, i32, pi32, pd],
         "tp_col_config": [P, pi32],
+        "tp_inflow_modes": [i32, ctypes.c_uint64, dbl, dbl, pd, dbl, dbl, pd],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -89,6 +91,16 @@ def _dp(a):
 def _check(rc):
     if rc != 0:
         raise TPError(load().tp_last_error().decode())
+
+
+def inflow_modes(nmodes, seed, tu, lturb, uoo, slit, span):
+    """Seeded random Fourier modes of the inflow turbulence (tp_inflow_modes):
+    array (nmodes, 8) of k (3), a (3), phase, 0.  Needs no GPU."""
+    out = np.zeros((int(nmodes), 8))
+    u = np.ascontiguousarray(uoo, np.float64)
+    _check(load().tp_inflow_modes(int(nmodes), int(seed), float(tu), float(lturb), _dp(u), float(slit),
+                                  float(span), _dp(out)))
+    return out
 
 
 class Solver:

@@ -678,10 +678,78 @@ int block_index(tp_context *c, int cs, int b) {
 
 }  // namespace
 
+// ---- seeded random Fourier modes of the inflow turbulence ----
+namespace {
+struct SplitMix {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+}  // namespace
+
 // ----------------------------------------------------------------------------
 // C ABI
 // ----------------------------------------------------------------------------
 extern "C" {
+
+int32_t tp_inflow_modes(int32_t nmodes, uint64_t seed, double tu, double lturb, const double *uoo3,
+                        double slit, double span, double *out8) {
+  if (nmodes < 1 || !out8 || !uoo3 || !(lturb > 0) || !(slit > 0) || !(tu >= 0))
+    return fail(TP_E_ARG, "tp_inflow_modes: bad arguments");
+  const double ke = std::sqrt(kPi) * std::tgamma(5.0 / 6.0) / (std::tgamma(1.0 / 3.0) * lturb);
+  const double klo = ke / 5, khi = 40 * ke, lr = std::log(khi / klo);
+  const double dkx = 2 * kPi / slit, dkz = span > 0 ? 2 * kPi / span : 0.0;
+  SplitMix rng{seed};
+  std::vector<double> w(nmodes);
+  double esum = 0;
+  for (int n = 0; n < nmodes; n++) {
+    double k[3], kk, sg[3];
+    double kmag = 0;
+    for (;;) {
+      kmag = klo * std::exp(lr * rng.uniform());           // log-uniform magnitude
+      const double ct = 2 * rng.uniform() - 1, st = std::sqrt(std::fmax(0.0, 1 - ct * ct));
+      const double ph = 2 * kPi * rng.uniform();
+      k[0] = kmag * st * std::cos(ph); k[1] = kmag * ct; k[2] = kmag * st * std::sin(ph);
+      k[0] = std::nearbyint(k[0] / dkx) * dkx;              // periodic along x (period slit)
+      if (dkz > 0) k[2] = std::nearbyint(k[2] / dkz) * dkz; // and along the span
+      kk = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+      if (kk <= 0) continue;
+      // direction: a random vector with its component along k removed
+      double r[3] = {2 * rng.uniform() - 1, 2 * rng.uniform() - 1, 2 * rng.uniform() - 1};
+      const double rk = (r[0] * k[0] + r[1] * k[1] + r[2] * k[2]) / (kk * kk);
+      for (int d = 0; d < 3; d++) sg[d] = r[d] - rk * k[d];
+      const double sn = std::sqrt(sg[0] * sg[0] + sg[1] * sg[1] + sg[2] * sg[2]);
+      if (sn < 1e-3) continue;
+      for (int d = 0; d < 3; d++) sg[d] /= sn;
+      break;
+    }
+    // von Karman energy spectrum at |k| times the sampling density of kmag
+    const double x = kk / ke;
+    const double e = x * x * x * x / std::pow(1 + x * x, 17.0 / 6.0);
+    w[n] = e * kmag;
+    esum += w[n];
+    double *o = out8 + 8 * (size_t)n;
+    o[0] = k[0]; o[1] = k[1]; o[2] = k[2];
+    o[3] = sg[0]; o[4] = sg[1]; o[5] = sg[2];
+    o[6] = 2 * kPi * rng.uniform();
+    o[7] = 0;
+  }
+  // amplitudes: <u'.u'> = sum_n |a_n|^2 / 2 = 3 (tu |uoo|)^2, i.e. rms tu |uoo|
+  // in each component
+  const double u2 = tu * tu * (uoo3[0] * uoo3[0] + uoo3[1] * uoo3[1] + uoo3[2] * uoo3[2]);
+  for (int n = 0; n < nmodes; n++) {
+    const double amp = std::sqrt(6 * u2 * w[n] / esum);
+    double *o = out8 + 8 * (size_t)n;
+    for (int d = 0; d < 3; d++) o[3 + d] *= amp;
+  }
+  return 0;
+}
+
 
 const char *tp_last_error(void) { return g_err.c_str(); }
 
@@ -808,6 +876,17 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
   k.uoo[0] = m1 * aoo; k.uoo[1] = m2 * aoo; k.uoo[2] = m3 * aoo;
   k.nkrk = iparams[TP_I_NKRK]; k.nviscous = iparams[TP_I_NVISCOUS]; k.nsmf = iparams[TP_I_NSMF];
   k.les = iparams[TP_I_LES]; k.nextgcic = iparams[TP_I_NEXTGCIC];
+  k.inflow = iparams[TP_I_INFLOW];
+  if (k.inflow) {
+    k.ntz = iparams[TP_I_NTZ];
+    if (k.ntz < 4) return fail(TP_E_ARG, "inflow table needs TP_I_NTZ >= 4 time intervals");
+    if (iparams[TP_I_NMODES] < 1) return fail(TP_E_ARG, "inflow needs TP_I_NMODES >= 1");
+    if (!(k.uoo[0] > 0) || !(dparams[TP_D_SLIT] > 0) || !(dparams[TP_D_LTURB] > 0))
+      return fail(TP_E_ARG, "inflow needs u_inf along +x, TP_D_SLIT > 0 and TP_D_LTURB > 0");
+    k.tgusto = dparams[TP_D_TGUSTO];
+    k.period = dparams[TP_D_SLIT] / k.uoo[0];
+    k.fctr = dparams[TP_D_SLIT] / (k.ntz * k.uoo[0]);
+  }
   if (case_id) *case_id = (int)c->cases.size();
   c->cases.push_back(std::move(h));
   return 0;
@@ -924,8 +1003,8 @@ int32_t tp_finalize(tp_context *c) {
   std::vector<CaseCoef> coefs;
   for (auto &h : c->cases) coefs.push_back(h.coef);
   if ((rc = upload_vec(c, &c->d_coef, coefs))) return rc;
-  std::vector<double> sc(ncase * S_NSCALAR, 0.0);
-  for (int cs = 0; cs < ncase; cs++) sc[cs * S_NSCALAR + S_AOA] = c->cases[cs].dp[TP_D_AOA0];
+  std::vector<double> sc(ncase * S_STRIDE, 0.0);
+  for (int cs = 0; cs < ncase; cs++) sc[cs * S_STRIDE + S_AOA] = c->cases[cs].dp[TP_D_AOA0];
   if ((rc = upload_vec(c, &c->d_sc, sc))) return rc;
   // ---- work tables ----
   std::vector<Work> pts;
@@ -1123,6 +1202,46 @@ int32_t tp_setup_geometry(tp_context *c) {
       }
       b.nsponge = (int)lst.size();
       for (int a0 = 0; a0 < b.nsponge; a0 += 1024) sp.push_back({(int)g, a0, std::min(1024, b.nsponge - a0), 0});
+      // inflow points: sponge points upstream of the inflow sponge edge inside
+      // the band |y| < gustw, with the ramp sin^2 of SingleTurbulence3D.f90:253-268
+      if (h.coef.inflow) {
+        const double gw = h.dp[TP_D_GUSTW], ra0 = kPi / szth1;
+        std::vector<int> gix(lst.size(), -1);
+        std::vector<double> atz, pos;
+        std::vector<int> pts;
+        for (size_t q = 0; q < lst.size(); q++) {
+          const int l = lst[q];
+          if (x[l] < szp[0][0] && std::fabs(y[l]) < gw) {
+            gix[q] = (int)pts.size();
+            pts.push_back(l);
+            const double sn = std::sin(std::fmin(ra0 * (x[l] + domlen), kPi / 2));
+            atz.push_back(sn * sn);
+          }
+        }
+        b.ngust = (int)pts.size();
+        if ((rc = upload_vec(c, &b.gix, gix))) return rc;
+        if (b.ngust > 0) {
+          pos.resize(3 * (size_t)b.ngust);
+          for (int q = 0; q < b.ngust; q++) {
+            pos[q] = x[pts[q]]; pos[b.ngust + q] = y[pts[q]]; pos[2 * (size_t)b.ngust + q] = z[pts[q]];
+          }
+          if ((rc = upload_vec(c, &b.atz, atz))) return rc;
+          const int nm = h.ip[TP_I_NMODES];
+          std::vector<double> modes(8 * (size_t)nm);
+          if ((rc = tp_inflow_modes(nm, (uint64_t)(uint32_t)h.ip[TP_I_SEED], h.dp[TP_D_TU], h.dp[TP_D_LTURB],
+                                    h.coef.uoo, h.dp[TP_D_SLIT], h.dp[TP_D_SPAN], modes.data())))
+            return rc;
+          double *dmodes = nullptr, *dpos = nullptr;
+          if ((rc = upload_vec(c, &dmodes, modes))) return rc;
+          if ((rc = upload_vec(c, &dpos, pos))) return rc;
+          if ((rc = dalloc(c, (void **)&b.vito, sizeof(double) * 3 * (h.coef.ntz + 1) * (size_t)b.ngust)))
+            return rc;
+          launch_inflow_table(dmodes, nm, dpos, b.ngust, h.coef.ntz, h.coef.fctr, h.coef.uoo, b.vito,
+                              c->stream);
+          HIPCHK(hipStreamSynchronize(c->stream));
+          HIPCHK(hipGetLastError());
+        }
+      }
     }
   }
   HIPCHK(hipMemcpy(c->d_blocks, c->blocks.data(), sizeof(Block) * c->blocks.size(), hipMemcpyHostToDevice));
@@ -1140,7 +1259,7 @@ int32_t tp_initialo(tp_context *c, int32_t cs) {
   if (n) launch_point(PK_INIT, n, c->stream, a);
   double s[S_NSCALAR] = {0};
   s[S_AOA] = c->cases[cs].dp[TP_D_AOA0];
-  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_NSCALAR, s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_STRIDE, s, sizeof(s), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1183,7 +1302,7 @@ int32_t tp_download(tp_context *c, int32_t cs, int32_t lb, int32_t id, double *d
 int32_t tp_get_scalars(tp_context *c, int32_t cs, double *s16) {
   if (int rc = check_ready(c)) return rc;
   if (cs < 0 || cs >= (int)c->cases.size() || !s16) return fail(TP_E_ARG, "bad case");
-  HIPCHK(hipMemcpyAsync(s16, c->d_sc + cs * S_NSCALAR, sizeof(double) * S_NSCALAR, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(s16, c->d_sc + cs * S_STRIDE, sizeof(double) * S_NSCALAR, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1191,7 +1310,7 @@ int32_t tp_get_scalars(tp_context *c, int32_t cs, double *s16) {
 int32_t tp_set_scalars(tp_context *c, int32_t cs, const double *s16) {
   if (int rc = check_ready(c)) return rc;
   if (cs < 0 || cs >= (int)c->cases.size() || !s16) return fail(TP_E_ARG, "bad case");
-  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_NSCALAR, s16, sizeof(double) * S_NSCALAR, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_STRIDE, s16, sizeof(double) * S_NSCALAR, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

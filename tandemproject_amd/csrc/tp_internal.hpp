@@ -24,7 +24,9 @@ constexpr int MAX_CASE = 4096;
 // ---- per-case scalars on the device (src/MainVar3D.f90:139-142) ----
 enum Scalar {
   S_N = 0, S_NDT, S_DT, S_DTS, S_DTE, S_TIMO, S_UMF1, S_UMF2, S_UMF3,
-  S_DUDT1, S_DUDT2, S_DUDT3, S_DTK, S_DTKO, S_AOA, S_NSCALAR = 16
+  S_DUDT1, S_DUDT2, S_DUDT3, S_DTK, S_DTKO, S_AOA, S_NSCALAR = 16,
+  // internal: weights and table rows of the inflow's time interpolation for the current stage
+  S_LAGW = 16, S_LAGI = 21, S_LAGON = 26, S_STRIDE = 32
 };
 
 // ---- per-case constants (scheme coefficients and physics) ----
@@ -39,6 +41,9 @@ struct CaseCoef {
   double srefoo, srefp1dre, cfl, dto, wtemp, smago1, smago2;
   double timf, tp, tps, aoa0, aoa1, amachoo, uoo[3];
   int nkrk, nviscous, nsmf, les, nextgcic, pad;
+  // turbulent inflow through the sponge target (SingleTurbulence3D.f90:332-372)
+  int inflow, ntz;
+  double tgusto, period, fctr;   // table period slit/uoo(1) and spacing slit/(ntz uoo(1))
 };
 
 // ---- device view of one sub-block ----
@@ -59,6 +64,10 @@ struct Block {
   double *rpex;
   int *lcsz;
   double *asz, *bsz;
+  int ngust;                  // inflow points (a subset of the sponge points)
+  int *gix;                   // per sponge point: its inflow index, -1 if none
+  double *atz;                // per inflow point: ramp sin^2 of the reference
+  double *vito;               // [3][ntz+1][ngust] inflow velocity table
   const double *col[3][2];    // chunked-solver carry maps per direction, kind (deriv, filter)
 };
 
@@ -124,7 +133,7 @@ struct StageArgs {
   const Block *blocks;
   const Work *work;
   const CaseCoef *coef;
-  double *sc;        // [ncase][S_NSCALAR]
+  double *sc;        // [ncase][S_STRIDE]
   double *part;      // reduction partials [nwork][2]
   int nk, m, d, opt; // stage number, field, direction, option
   int arr, col;      // generic field selector
@@ -133,6 +142,8 @@ struct StageArgs {
 
 
 // launchers (tp_line.hip, tp_col.hip, tp_stage.hip)
+void launch_inflow_table(const double *modes, int nmodes, const double *pos, int ng, int ntz, double fctr,
+                         const double *uoo, double *vito, hipStream_t st);
 // chunked line solves: P waves per 64 lines, chunk capacity C points per lane
 int col_capacity(int chunk);
 size_t col_lds_bytes(int d, int P);

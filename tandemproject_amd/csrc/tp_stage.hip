@@ -78,11 +78,55 @@ __device__ void movef_dev(const CaseCoef &c, double *sc, double dtko, double dtk
 }
 
 // stage start: Main3D.f90:571-572
+// ---- time interpolation of the inflow table, SingleTurbulence3D.f90:336-356 ----
+// Five-point Lagrange interpolation in the periodic table time tt(i) = i*fctr,
+// i = 0..ntz, at ra3 = (timo - tgusto + dtk) mod period; nodes next to the
+// table ends wrap around by one period.  The weights and table rows are the
+// same for every inflow point of a case: stored in its scalar slots.
+__device__ void inflow_weights(const CaseCoef &c, double *s) {
+  s[S_LAGON] = 0.0;
+  const double timo = s[S_TIMO];
+  if (!(timo > c.tgusto)) return;
+  const double ra0 = timo - c.tgusto + s[S_DTK], ra1 = c.period, ra2 = ra0 / ra1;
+  const double ra3 = ra0 - ra1 * (double)(int)ra2;
+  const int is = 0, ie = c.ntz;
+  int ii = 0;
+  double best = fabs(0 * c.fctr - ra3);
+  for (int i = 1; i <= ie; i++) {
+    const double d = fabs(i * c.fctr - ra3);
+    if (d < best) { best = d; ii = i; }
+  }
+  int il[5];
+  double tl[5];
+  for (int jj = -2; jj <= 2; jj++) {
+    il[jj + 2] = min(max(ii + jj, is), ie);
+    tl[jj + 2] = il[jj + 2] * c.fctr;
+  }
+  if (ii - is == 0) { il[0] = ie - 2; il[1] = ie - 1; tl[0] = il[0] * c.fctr - ra1; tl[1] = il[1] * c.fctr - ra1; }
+  if (ii - is == 1) { il[0] = ie - 1; tl[0] = il[0] * c.fctr - ra1; }
+  if (ie - ii == 1) { il[4] = is + 1; tl[4] = il[4] * c.fctr + ra1; }
+  if (ie - ii == 0) { il[3] = is + 1; il[4] = is + 2; tl[3] = il[3] * c.fctr + ra1; tl[4] = il[4] * c.fctr + ra1; }
+  double al[5];
+  for (int q = 0; q < 5; q++) al[q] = ra3 - tl[q];
+  for (int jj = 0; jj < 5; jj++) {
+    double ao = 1.0, bo = 1.0;
+    for (int q = 0; q < 5; q++) {
+      if (q != jj) {
+        ao = ao * al[q];
+        bo = bo * (tl[jj] - tl[q]);
+      }
+    }
+    s[S_LAGW + jj] = ao / bo;
+    s[S_LAGI + jj] = (double)il[jj];
+  }
+  s[S_LAGON] = 1.0;
+}
+
 __global__ void k_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk) {
   int cs = blockIdx.x * blockDim.x + threadIdx.x;
   if (cs >= ncase) return;
   const CaseCoef &c = coef[cs];
-  double *s = sc + cs * S_NSCALAR;
+  double *s = sc + cs * S_STRIDE;
   int nkrk = c.nkrk;
   double dt = s[S_DT];
   int f1 = nk - 2 > 0 ? (nk - 2 < 1 ? nk - 2 : 1) : 0;
@@ -90,6 +134,7 @@ __global__ void k_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk) 
   double dtko = dt * (double)f1 / (nkrk - nk + 3);
   double dtk = dt * (double)f2 / (nkrk - nk + 2);
   movef_dev(c, s, dtko, dtk);
+  if (c.inflow) inflow_weights(c, s);
 }
 
 // time step size: Main3D.f90:587-610 (maxima reduced from the partials of the
@@ -100,7 +145,7 @@ __global__ void k_sc_dt(const CaseCoef *coef, double *sc, const double *part, co
   if (cs >= ncase) return;
   __shared__ double mi[256], mv[256];
   double fi = -INFINITY, fv = -INFINITY;
-  double *s = sc + cs * S_NSCALAR;
+  double *s = sc + cs * S_STRIDE;
   int n = (int)s[S_N];
   bool need = reduced && (n % 10 == 1) && coef[cs].dto < 0.0;
   if (need) {
@@ -141,7 +186,7 @@ __global__ void k_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk) {
   int cs = blockIdx.x * blockDim.x + threadIdx.x;
   if (cs >= ncase) return;
   const CaseCoef &c = coef[cs];
-  double *s = sc + cs * S_NSCALAR;
+  double *s = sc + cs * S_STRIDE;
   double dt = s[S_DT];
   int f2 = nk - 1 < 1 ? nk - 1 : 1;
   double dtko = dt * (double)f2 / (c.nkrk - nk + 2);
@@ -153,7 +198,7 @@ __global__ void k_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk) {
 __global__ void k_sc_end(double *sc, int ncase, const int *active) {
   int cs = blockIdx.x * blockDim.x + threadIdx.x;
   if (cs >= ncase || (active && !active[cs])) return;
-  double *s = sc + cs * S_NSCALAR;
+  double *s = sc + cs * S_STRIDE;
   s[S_N] = s[S_N] + 1;
   s[S_TIMO] = s[S_TIMO] + s[S_DT];
 }
@@ -179,7 +224,7 @@ __global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
 // ---- time-step maxima, Main3D.f90:590-603 (partials per workgroup) ----
 __global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
   WORK_POINTS
-  const double *sc = a.sc + b.cs * S_NSCALAR;
+  const double *sc = a.sc + b.cs * S_STRIDE;
   __shared__ double mi[256], mv[256];
   double fi = -INFINITY, fv = -INFINITY;
   int n = (int)sc[S_N];
@@ -284,7 +329,7 @@ __global__ void __launch_bounds__(256) k_grad_stress(StageArgs a) {
 __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
-  const double *sc = a.sc + b.cs * S_NSCALAR;
+  const double *sc = a.sc + b.cs * S_STRIDE;
   const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
   const int visc = c.nviscous == 1;
   const double *qa = b.qa, *xi = b.xim, *et = b.etm, *ze = b.zem, *p = b.p, *tv = b.tv, *de = b.de;
@@ -350,16 +395,71 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
 __global__ void __launch_bounds__(256) k_sponge(StageArgs a) {
   WORK_POINTS
   const Phys ph = phys();
-  const double target_p = ph.ham * pow(1.0, GAM);
+  const CaseCoef &c = a.coef[b.cs];
+  const double *sc = a.sc + b.cs * S_STRIDE;
+  const bool gust = c.inflow && sc[S_LAGON] != 0.0;
+  const int ng = b.ngust, nt1 = c.ntz + 1;
   for (int q = w.a + threadIdx.x; q < w.a + w.n; q += blockDim.x) {
-    int l = b.lcsz[q];
-    double as = b.asz[q];
-    b.de[l] = b.de[l] + as * (b.qa[l] - 1.0);
-    b.de[N + l] = b.de[N + l] + as * (b.qa[N + l] - 0.0);
-    b.de[2 * N + l] = b.de[2 * N + l] + as * (b.qa[2 * N + l] - 0.0);
-    b.de[3 * N + l] = b.de[3 * N + l] + as * (b.qa[3 * N + l] - 0.0);
-    b.de[4 * N + l] = b.de[4 * N + l] + b.bsz[q] * (b.p[l] - target_p);
+    const int l = b.lcsz[q];
+    double rt = 1.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    const int ll = c.inflow ? b.gix[q] : -1;
+    if (ll >= 0) {
+      // target of an inflow point: isentropic density and momentum of the
+      // interpolated velocity (SingleTurbulence3D.f90:361-366)
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+      if (gust) {
+        for (int jj = 0; jj < 5; jj++) {
+          const double wj = sc[S_LAGW + jj];
+          const int it = (int)sc[S_LAGI + jj];
+          v0 = v0 + wj * b.vito[((size_t)0 * nt1 + it) * ng + ll];
+          v1 = v1 + wj * b.vito[((size_t)1 * nt1 + it) * ng + ll];
+          v2 = v2 + wj * b.vito[((size_t)2 * nt1 + it) * ng + ll];
+        }
+      }
+      const double fctr = 0.5 * ph.gamm1;
+      rt = pow(1.0 - fctr * (v0 * v0 + v1 * v1 + v2 * v2), ph.hamm1);
+      const double at = b.atz[ll] * rt;
+      m0 = at * v0; m1 = at * v1; m2 = at * v2;
+    }
+    const double as = b.asz[q];
+    b.de[l] = b.de[l] + as * (b.qa[l] - rt);
+    b.de[N + l] = b.de[N + l] + as * (b.qa[N + l] - m0);
+    b.de[2 * N + l] = b.de[2 * N + l] + as * (b.qa[2 * N + l] - m1);
+    b.de[3 * N + l] = b.de[3 * N + l] + as * (b.qa[3 * N + l] - m2);
+    b.de[4 * N + l] = b.de[4 * N + l] + b.bsz[q] * (b.p[l] - ph.ham * pow(rt, GAM));
   }
+}
+
+// ---- inflow velocity table: frozen turbulence convected with uoo, sampled
+// at the inflow points at the table times tt(i) = i*fctr (the reference fills
+// vito(ll, i, :) the same way from its eddies, SingleTurbulence3D.f90:262-292) ----
+__global__ void __launch_bounds__(256) k_inflow_table(const double *modes, int nmodes, const double *pos,
+                                                      int ng, int ntz, double fctr, double u0, double u1,
+                                                      double u2, double *vito) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nt1 = ntz + 1;
+  if (e >= ng * nt1) return;
+  const int ll = e % ng, it = e / ng;
+  const double tt = it * fctr;
+  const double x0 = pos[ll] - tt * u0, x1 = pos[ng + ll] - tt * u1, x2 = pos[2 * (size_t)ng + ll] - tt * u2;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int n = 0; n < nmodes; n++) {
+    const double *m = modes + 8 * (size_t)n;
+    const double cs = cos(m[0] * x0 + m[1] * x1 + m[2] * x2 + m[6]);
+    s0 = s0 + m[3] * cs;
+    s1 = s1 + m[4] * cs;
+    s2 = s2 + m[5] * cs;
+  }
+  vito[((size_t)0 * nt1 + it) * ng + ll] = s0;
+  vito[((size_t)1 * nt1 + it) * ng + ll] = s1;
+  vito[((size_t)2 * nt1 + it) * ng + ll] = s2;
+}
+
+void launch_inflow_table(const double *modes, int nmodes, const double *pos, int ng, int ntz, double fctr,
+                         const double *uoo, double *vito, hipStream_t st) {
+  const int tot = ng * (ntz + 1);
+  hipLaunchKernelGGL(k_inflow_table, dim3((tot + 255) / 256), dim3(256), 0, st, modes, nmodes, pos, ng, ntz,
+                     fctr, uoo[0], uoo[1], uoo[2], vito);
 }
 
 // ---- characteristic algebra, Subroutines3D.f90:235-324 ----
@@ -436,7 +536,7 @@ __global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
   const Block &b = a.blocks[w.blk];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
   const int np = b.nbc[d][ip];
-  const double *sc = a.sc + b.cs * S_NSCALAR;
+  const double *sc = a.sc + b.cs * S_STRIDE;
   int j, k;
   face_jk(b, d, jk, j, k);
   int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
@@ -489,7 +589,7 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
   const CaseCoef &c = a.coef[b.cs];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
   const int np = b.nbc[d][ip];
-  const double *sc = a.sc + b.cs * S_NSCALAR;
+  const double *sc = a.sc + b.cs * S_STRIDE;
   int j, k;
   face_jk(b, d, jk, j, k);
   const int i = ip * (b.L[d] - 1), iq = 1 - 2 * ip;
@@ -538,7 +638,7 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
 // ---- RK update, Main3D.f90:876-881 ----
 __global__ void __launch_bounds__(256) k_rk(StageArgs a) {
   WORK_POINTS
-  const double dtk = a.sc[b.cs * S_NSCALAR + S_DTK];
+  const double dtk = a.sc[b.cs * S_STRIDE + S_DTK];
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
     double r1 = dtk * b.yaco[l];
     for (int v = 0; v < 5; v++) b.qa[(size_t)v * N + l] = b.qo[(size_t)v * N + l] - r1 * b.de[(size_t)v * N + l];
@@ -554,7 +654,7 @@ __global__ void __launch_bounds__(256) k_wall(StageArgs a) {
   const Phys ph = phys();
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x;
   const int np = b.nbc[d][ip];
-  const double *umf = a.sc + b.cs * S_NSCALAR + S_UMF1;
+  const double *umf = a.sc + b.cs * S_STRIDE + S_UMF1;
   int j, k;
   face_jk(b, d, jk, j, k);
   int l = indx3(b, ip * (b.L[d] - 1), j, k, d);

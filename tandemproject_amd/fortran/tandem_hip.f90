@@ -19,7 +19,8 @@ module tandem_hip
                                    TP_E_SIZE = 4, TP_E_LAYOUT = 5
   ! integer case parameters, iparams(1:16) (0-based slot + 1)
   integer, parameter :: TP_I_NKRK = 1, TP_I_NVISCOUS = 2, TP_I_NSMF = 3, TP_I_LES = 4, &
-                        TP_I_NEXTRABC = 5, TP_I_NEXTGCIC = 6, TP_I_NPARAM = 16
+                        TP_I_NEXTRABC = 5, TP_I_NEXTGCIC = 6, TP_I_INFLOW = 7, TP_I_NMODES = 8, &
+                        TP_I_SEED = 9, TP_I_NTZ = 10, TP_I_NPARAM = 16
   ! double case parameters, dparams(1:32)
   integer, parameter :: TP_D_REOO = 1, TP_D_TEMPOO = 2, TP_D_AMACH1 = 3, TP_D_AMACH2 = 4, &
                         TP_D_AMACH3 = 5, TP_D_WTEMP = 6, TP_D_CFL = 7, TP_D_TMAX = 8, &
@@ -27,7 +28,8 @@ module tandem_hip
                         TP_D_AOA0 = 13, TP_D_AOA1 = 14, TP_D_TPS = 15, TP_D_TP = 16, &
                         TP_D_SMAGO1 = 17, TP_D_SMAGO2 = 18, TP_D_DOMLEN = 19, TP_D_SZTH1 = 20, &
                         TP_D_SZTH2 = 21, TP_D_SZXT = 22, TP_D_SZCO = 23, TP_D_SPAN = 24, &
-                        TP_D_WLEW = 25, TP_D_WLEA = 26, TP_D_NPARAM = 32
+                        TP_D_WLEW = 25, TP_D_WLEA = 26, TP_D_TGUSTO = 27, TP_D_SLIT = 28, &
+                        TP_D_TU = 29, TP_D_LTURB = 30, TP_D_GUSTW = 31, TP_D_NPARAM = 32
   ! array ids of tp_upload / tp_download (names of src/MainVar3D.f90:78-102)
   integer(c_int32_t), parameter :: TP_QA = 0, TP_QO = 1, TP_DE = 2, TP_RR = 3, TP_SS = 4, &
                                    TP_XIM = 5, TP_ETM = 6, TP_ZEM = 7, TP_P = 8, TP_YACO = 9, &
@@ -193,6 +195,15 @@ module tandem_hip
       import :: c_ptr, c_int32_t, c_int64_t
       type(c_ptr), value :: ctx
       integer(c_int64_t), intent(out) :: out4(4)
+    end function
+    integer(c_int32_t) function tp_inflow_modes(nmodes, seed, tu, lturb, uoo3, slit, span, out8) &
+        bind(C, name="tp_inflow_modes")
+      import :: c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: nmodes
+      integer(c_int64_t), value :: seed
+      real(c_double), value :: tu, lturb, slit, span
+      real(c_double), intent(in) :: uoo3(3)
+      real(c_double), intent(out) :: out8(*)
     end function
     pure integer(c_int64_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_ptr, c_int64_t

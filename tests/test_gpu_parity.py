@@ -289,3 +289,31 @@ def test_ensemble_equals_alone(mode):
         np.testing.assert_array_equal(both.download(0, b, "qa"), a1.download(0, b, "qa"))
     for b in range(lay2.nb):
         np.testing.assert_array_equal(both.download(1, b, "qa"), a2.download(0, b, "qa"))
+
+
+@LINE_MODES
+def test_inflow_turbulence_steps(exact):
+    """Turbulent inflow through the sponge target: the device table (built
+    from the library's seeded Fourier modes), the per-stage time
+    interpolation and the relaxation against the oracle's restatement of
+    spongego over 40 steps; and the inflow does change the flow."""
+    say("inflow turbulence, 40 steps,", "sequential" if exact else "chunked")
+    lay, ss, p = aerofoil_case()
+    p = dict(p)
+    p.update(inflow=1, nmodes=48, seed=11, ntz=16, tgusto=0.0, slit=2.0, tu=0.05, lturb=0.3, gustw=1.5)
+    uoo = np.array([p["amach1"], p["amach2"], p["amach3"]]) * np.sqrt(1.4 * (1 / 1.4) / 1.0)
+    modes = api.inflow_modes(p["nmodes"], p["seed"], p["tu"], p["lturb"], uoo, p["slit"], p["span"])
+    o = po.Oracle(lay, p)
+    o.set_inflow_modes(modes)
+    s = api.Solver([(lay, p)], exact_lines=exact)
+    plain = api.Solver([(lay, dict(p, inflow=0))], exact_lines=exact)
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b]); s.set_grid(0, b, ss[b]); plain.set_grid(0, b, ss[b])
+    o.metrics(); s.setup_geometry(); plain.setup_geometry()
+    o.initialo(); s.initialo(0); plain.initialo(0)
+    o.step(40); s.step(40); plain.step(40)
+    s.sync()
+    compare_state(o, s, lay, 1e-10, acoustic=not exact)
+    moved = max(float(np.abs(s.download(0, b, "qa") - plain.download(0, b, "qa")).max()) for b in range(lay.nb))
+    say("  max change of the state by the inflow", moved)
+    assert moved > 1e-6
