@@ -38,10 +38,16 @@ def workload(scale=1.0):
     blocks = G.h_grid_blocks(lx, ly, lz, xu=7.0, xd=10.0, yb=7.0, yt=7.0, span=0.2,
                              thick=0.12, wall_spacing=4e-3)
     ss = G.subblock_grids(lay, blocks)
+    # inflow: synthetic isotropic turbulence from 4096 seeded random Fourier
+    # modes (von Karman spectrum, Tu = 2.5 %, integral length 0.1 chord),
+    # frozen in a periodic box of 4 chords convected with u_inf, imposed
+    # through the inflow sponge in the band |y| < 2
     params = dict(reoo=4e5 / 0.3, tempoo=273.0, amach1=0.3, amach2=0.0, amach3=0.0, wtemp=1.0,
                   cfl=1.0, tmax=1e9, timf=15.0, fltk=0.85, fltkbc=0.85, dto=-1.0, domlen=10.0,
                   szth1=2.0, szth2=2.0, szxt=4.0, szco=12.0, span=0.2, wlew=0.11, wlea=0.015,
-                  nkrk=4, nviscous=1, nsmf=0, les=0, nextrabc=0, nextgcic=0)
+                  nkrk=4, nviscous=1, nsmf=0, les=0, nextrabc=0, nextgcic=0,
+                  inflow=1, nmodes=4096, seed=2, ntz=64, tgusto=0.0, slit=4.0, tu=0.025, lturb=0.1,
+                  gustw=2.0)
     return lay, ss, params, dict(lxibk=lx, letbk=ly, lzebk=lz)
 
 
@@ -53,6 +59,11 @@ def cpu_baseline(seconds_target=15.0):
     po.set_threads(threads)
     lay, ss, params, dims = workload(scale=0.2)
     o = po.Oracle(lay, params)
+    if params.get("inflow"):
+        # free-stream velocity in units of aoo = sqrt(gam poo / rhooo) = 1
+        uoo = np.array([params["amach1"], params["amach2"], params["amach3"]])
+        o.set_inflow_modes(api.inflow_modes(params["nmodes"], params["seed"], params["tu"], params["lturb"],
+                                            uoo, params["slit"], params["span"]))
     for b in range(lay.nb):
         o.set_grid(b, ss[b])
     o.metrics()
@@ -176,10 +187,12 @@ def main():
         "metric": METRIC, "value": round(value, 2), "unit": "Mpoint-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: uniform-flow start with the moving-frame ramp on an algebraic 3x2 H-grid",
+        "data": "synthetic: uniform-flow start with the moving-frame ramp on an algebraic 3x2 H-grid, "
+                "seeded Fourier-mode turbulence inflow (4096 modes, von Karman, Tu 2.5 %, L 0.1c)",
         "config": {"workload": "naca0012-turbulence-interaction shape: 3x2 H blocks x 2 spanwise "
                                f"sub-blocks, lxibk={dims['lxibk']}, letbk={dims['letbk']}, nz={dims['lzebk'][0] + 1}, "
-                               f"{npts / 1e6:.2f} M points, M=0.3, Re_c=4e5, viscous, 4-stage RK",
+                               f"{npts / 1e6:.2f} M points, M=0.3, Re_c=4e5, viscous, 4-stage RK, "
+                               "turbulent inflow Tu=2.5%, L=0.1c, 4096 modes",
                    "points": npts, "sub_blocks": lay.nb, "graph": bool(args.graph),
                    "line_chunks": {f"{'xez'[d]}-{k}": c for (d, k), c in s.col_config().items()},
                    "ms_per_step_by_class": breakdown},
