@@ -37,11 +37,12 @@ enum {
 };
 
 /* integer case parameters, iparams[] (inputo.dat lines 5-10, 20; src/Main3D.f90:39-56).
- * TP_I_INFLOW = 1 switches on the turbulent inflow through the sponge target
+ * TP_I_INFLOW switches on an inflow disturbance through the sponge target
  * (the reference's spongego gust path, src/SingleTurbulence3D.f90:332-372, which
- * it ships disabled): a velocity table over the inflow-sponge points, filled
- * from TP_I_NMODES seeded random Fourier modes (tp_inflow_modes), sampled at
- * TP_I_NTZ+1 times over one period. */
+ * it ships disabled): a velocity table over the inflow-sponge points sampled
+ * at TP_I_NTZ+1 times over one period, filled from
+ *   1: TP_I_NMODES seeded random Fourier modes of turbulence (tp_inflow_modes);
+ *   2: the single mode of a sinusoidal transverse gust (tp_gust_mode). */
 enum { TP_I_NKRK = 0, TP_I_NVISCOUS, TP_I_NSMF, TP_I_LES, TP_I_NEXTRABC, TP_I_NEXTGCIC,
        TP_I_INFLOW, TP_I_NMODES, TP_I_SEED, TP_I_NTZ, TP_I_NPARAM = 16 };
 /* double case parameters, dparams[] (inputo.dat, inputp.dat; fltk/fltkbc in units of pi).
@@ -49,13 +50,15 @@ enum { TP_I_NKRK = 0, TP_I_NVISCOUS, TP_I_NSMF, TP_I_LES, TP_I_NEXTRABC, TP_I_NE
  * turbulence box along x (the table's period is slit/u_inf, as slit/uoo(1) in
  * the reference), TP_D_TU turbulence intensity (rms of each velocity
  * component over |u_inf|), TP_D_LTURB integral length scale, TP_D_GUSTW half
- * width in y of the band of inflow-sponge points that receive the turbulence. */
+ * width in y of the band of inflow-sponge points that receive the turbulence.
+ * Gust (TP_I_INFLOW = 2): TP_D_TU amplitude over |u_inf|, TP_D_GUSTK reduced
+ * frequency omega c / (2 |u_inf|) with chord c = 1; TP_D_SLIT is derived. */
 enum {
   TP_D_REOO = 0, TP_D_TEMPOO, TP_D_AMACH1, TP_D_AMACH2, TP_D_AMACH3, TP_D_WTEMP, TP_D_CFL,
   TP_D_TMAX, TP_D_TIMF, TP_D_FLTK, TP_D_FLTKBC, TP_D_DTO, TP_D_AOA0, TP_D_AOA1, TP_D_TPS,
   TP_D_TP, TP_D_SMAGO1, TP_D_SMAGO2, TP_D_DOMLEN, TP_D_SZTH1, TP_D_SZTH2, TP_D_SZXT,
   TP_D_SZCO, TP_D_SPAN, TP_D_WLEW, TP_D_WLEA, TP_D_TGUSTO, TP_D_SLIT, TP_D_TU, TP_D_LTURB,
-  TP_D_GUSTW, TP_D_NPARAM = 32
+  TP_D_GUSTW, TP_D_GUSTK, TP_D_NPARAM = 32
 };
 /* arrays reachable by tp_upload / tp_download (names of src/MainVar3D.f90:78-102) */
 enum {
@@ -89,19 +92,26 @@ const char *tp_last_error(void);
  *                           index, -1 none), sponge flag of face ip of direction d
  *   jct[j*12+mbr*3+{0,1,2}] (sub-block, ip, jp) of corner member mbr of junction j
  *   jvel[j*2+{0,1}]         members whose velocities are averaged
- * Several cases in one context are advanced together (ensembles). */
+ * Several cases in one context are advanced together (ensembles).  A case
+ * added after tp_finalize waits, pending, until tp_admit puts it on the
+ * device; until then it takes no device memory and is not advanced. */
 int32_t tp_add_case(tp_context *ctx, int32_t nblocks, const int32_t *dims, const int32_t *nbc,
                     const int32_t *ncd, const int32_t *nsz, int32_t njct, const int32_t *jct,
                     const int32_t *jvel, const int32_t *iparams, const double *dparams,
                     int32_t *case_id);
-/* allocate device state, compute the scheme coefficients and LU factors
- * (src/Main3D.f90:293-332), build the launch tables */
+/* allocate device state of the cases added so far, compute the scheme
+ * coefficients and LU factors (src/Main3D.f90:293-332), build the launch
+ * tables.  A context may be finalized with no case (cases then come through
+ * tp_admit). */
 int32_t tp_finalize(tp_context *ctx);
 
-/* grid of one sub-block, ss(0:lmx,1:3) = x, y, z (src/rpt.f90:673-696) */
+/* grid of one sub-block, ss(0:lmx,1:3) = x, y, z (src/rpt.f90:673-696); the
+ * host keeps a copy (a pending case's grid goes to the device on tp_admit) */
 int32_t tp_set_grid(tp_context *ctx, int32_t case_id, int32_t block, const double *xyz);
 /* metrics, Jacobian, face normals, wall-slot flags and sponge coefficients
- * for every case from its grid (src/Main3D.f90:365-480, src/SingleTurbulence3D.f90:209-249) */
+ * from the grid (src/Main3D.f90:365-480, src/SingleTurbulence3D.f90:209-249),
+ * for every case on the device whose geometry is not set yet; the fields of
+ * the other cases are not touched */
 int32_t tp_setup_geometry(tp_context *ctx);
 /* uniform initial state (src/SingleTurbulence3D.f90:179-205) and n = 0, t = 0 */
 int32_t tp_initialo(tp_context *ctx, int32_t case_id);
@@ -174,6 +184,78 @@ int32_t tp_col_config(tp_context *ctx, int32_t *chunks6);
 int32_t tp_inflow_modes(int32_t nmodes, uint64_t seed, double tu, double lturb, const double *uoo3,
                         double slit, double span, double *out8);
 
+/* Sinusoidal transverse gust as one Fourier mode in the layout of
+ * tp_inflow_modes: wavevector along u_inf with |k| = 2 kr (reduced frequency
+ * kr = omega c / (2 |u_inf|), chord 1), amplitude tu |u_inf| normal to u_inf
+ * in the x-y plane, phase 0; *slit receives the table's x-period, so that
+ * slit / u_x is the gust period.  Needs no device.  No counterpart in the
+ * reference. */
+int32_t tp_gust_mode(double tu, double kr, const double *uoo3, double *out8, double *slit);
+
+/* ---- cases joining and leaving a running context ----
+ * No counterpart in the reference (one case per MPI job there): these serve
+ * ensembles and the request loop below.  Cases on the device keep their state
+ * and scalars while others join or leave; a join or leave drops a captured
+ * step graph (tp_step_graph captures again). */
+/* device bytes case_id will hold once admitted: block arrays, solver tables,
+ * sponge and inflow arrays (needs the case's grids) -- the figure an HBM
+ * budget admits by */
+int32_t tp_case_bytes(tp_context *ctx, int32_t case_id, int64_t *bytes);
+/* put a pending case on the device: allocate, set up its geometry (metrics,
+ * sponge, inflow table) from the grids given by tp_set_grid, and include it
+ * in every later launch.  Its state is zero: follow with tp_initialo or
+ * tp_upload.  On failure the case stays pending and the context unchanged. */
+int32_t tp_admit(tp_context *ctx, int32_t case_id);
+/* drop a case (pending or on the device) and free its device memory */
+int32_t tp_remove_case(tp_context *ctx, int32_t case_id);
+/* out[0] state (0 pending, 1 on the device, 2 with geometry), out[1] points,
+ * out[2] sub-blocks, out[3] device bytes held */
+int32_t tp_case_info(tp_context *ctx, int32_t case_id, int64_t *out4);
+
+/* ---- request loop (no counterpart in the reference) ----
+ * A server owns a context.  Requests -- whole cases with their grids and a
+ * step count -- queue on the host with an arrival time.  Each iteration
+ *   1. admits arrived requests in arrival order while a slot is free and
+ *      the case's device bytes (tp_case_bytes) fit the HBM budget (a request
+ *      that does not fit is passed over for later, smaller ones), sets them
+ *      up on the device and gives them the uniform initial state;
+ *   2. advances every case on the device by the same number of steps (at
+ *      most max_steps, at most what any of them still needs) in shared
+ *      launches (captured step graph when use_graph);
+ *   3. copies the final qa of every case that made its steps to the host and
+ *      takes it off the device.
+ * Cases join and leave while others are resident; each case's result does
+ * not depend on the others (see tp_set_option on line-solver chunks for
+ * bit-identity).  Times are seconds on the server's clock. */
+typedef struct tp_server tp_server;
+/* budget_bytes <= 0: 90 % of the device memory free at creation */
+int32_t tp_server_create(int64_t budget_bytes, int32_t max_active, int32_t use_graph, tp_server **srv);
+int32_t tp_server_destroy(tp_server *srv);
+tp_context *tp_server_context(tp_server *srv);   /* for tp_set_option / tp_autotune */
+const char *tp_server_error(tp_server *srv);
+double tp_server_clock(tp_server *srv);
+/* queue a case (arguments of tp_add_case, then xyz[b] = grid of sub-block b);
+ * t_arrival < 0: now.  Fails with TP_E_SIZE if the case alone exceeds the budget. */
+int32_t tp_server_submit(tp_server *srv, int32_t nblocks, const int32_t *dims, const int32_t *nbc,
+                         const int32_t *ncd, const int32_t *nsz, int32_t njct, const int32_t *jct,
+                         const int32_t *jvel, const int32_t *iparams, const double *dparams,
+                         const double *const *xyz, int32_t nsteps, double t_arrival, int32_t *req);
+/* one iteration; status6 = on the device, queued (arrived), not yet arrived,
+ * finished, admitted in this iteration, steps made */
+int32_t tp_server_iterate(tp_server *srv, int32_t max_steps, int32_t *status6);
+/* iterate until every request has finished, idling until arrivals */
+int32_t tp_server_run(tp_server *srv, int32_t max_steps);
+/* info6 = state (0 queued, 1 on the device, 2 finished, 3 failed), arrival,
+ * admission and finish times, steps made, device bytes; qa (may be NULL):
+ * the final qa of every sub-block in order, count = tp_server_result_count */
+int32_t tp_server_result(tp_server *srv, int32_t req, double *qa, int64_t count, double *info6);
+int64_t tp_server_result_count(tp_server *srv, int32_t req);
+/* out12 = finished, failed, p50, p99, mean and max time to solution (arrival
+ * to final state on the host), wall time from the first arrival to the last
+ * finish, cases per hour over it, peak admitted bytes, peak cases on the
+ * device, admissions that joined running cases, time spent stepping */
+int32_t tp_server_stats(tp_server *srv, double *out12);
+
 /* ---- measurement ---- */
 /* record HIP events around every launch (on the context's stream) */
 int32_t tp_profile(tp_context *ctx, int32_t on);
@@ -182,7 +264,7 @@ int32_t tp_profile(tp_context *ctx, int32_t on);
  * the class times points); reset != 0 clears the counters afterwards */
 int32_t tp_profile_read(tp_context *ctx, int32_t kclass, double *ms, int64_t *launches,
                         double *bytes, int32_t reset);
-/* out[0] points, out[1] sub-blocks, out[2] cases, out[3] device bytes held */
+/* out[0] points, out[1] sub-blocks, out[2] cases on the device, out[3] device bytes held */
 int32_t tp_info(tp_context *ctx, int64_t *out4);
 
 #ifdef __cplusplus

@@ -18,7 +18,7 @@ I_NAMES = ["nkrk", "nviscous", "nsmf", "les", "nextrabc", "nextgcic", "inflow", 
 D_NAMES = ["reoo", "tempoo", "amach1", "amach2", "amach3", "wtemp", "cfl", "tmax",
            "timf", "fltk", "fltkbc", "dto", "aoa0", "aoa1", "tps", "tp", "smago1",
            "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea", "tgusto",
-           "slit", "tu", "lturb", "gustw"]
+           "slit", "tu", "lturb", "gustw", "gustk"]
 ARRAYS = ["qa", "qo", "de", "rr", "ss", "xim", "etm", "zem", "p", "yaco", "tv",
           "drva1", "drva2", "drva3", "drvb1", "drvb2", "drvb3", "send1", "send2", "send3",
           "recv1", "recv2", "recv3", "cm1", "cm2", "cm3", "xu", "xl", "yu", "yl", "rpex"]
@@ -30,7 +30,11 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_array_count", "tp_get_scalars", "tp_set_scalars", "tp_mpigo", "tp_deriv",
            "tp_filte", "tp_filter_step", "tp_stage", "tp_step", "tp_step_graph", "tp_sync",
            "tp_profile", "tp_profile_read", "tp_info", "tp_set_option", "tp_autotune",
-           "tp_col_config", "tp_inflow_modes"]
+           "tp_col_config", "tp_inflow_modes", "tp_gust_mode", "tp_case_bytes", "tp_admit",
+           "tp_remove_case", "tp_case_info", "tp_server_create", "tp_server_destroy",
+           "tp_server_context", "tp_server_error", "tp_server_clock", "tp_server_submit",
+           "tp_server_iterate", "tp_server_run", "tp_server_result", "tp_server_result_count",
+           "tp_server_stats"]
 
 _lib = None
 
@@ -69,6 +73,17 @@ def load(path=LIB_PATH):
         "tp_autotune": [P, i32, pi32, pd],
         "tp_col_config": [P, pi32],
         "tp_inflow_modes": [i32, ctypes.c_uint64, dbl, dbl, pd, dbl, dbl, pd],
+        "tp_gust_mode": [dbl, dbl, pd, pd, pd],
+        "tp_case_bytes": [P, i32, ctypes.POINTER(i64)],
+        "tp_admit": [P, i32], "tp_remove_case": [P, i32],
+        "tp_case_info": [P, i32, ctypes.POINTER(i64)],
+        "tp_server_create": [i64, i32, i32, ctypes.POINTER(P)],
+        "tp_server_destroy": [P],
+        "tp_server_submit": [P, i32, pi32, pi32, pi32, pi32, i32, pi32, pi32, pi32, pd,
+                             ctypes.POINTER(pd), i32, dbl, pi32],
+        "tp_server_iterate": [P, i32, pi32], "tp_server_run": [P, i32],
+        "tp_server_result": [P, i32, pd, i64, pd],
+        "tp_server_stats": [P, pd],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -76,6 +91,14 @@ def load(path=LIB_PATH):
         f.restype = i32
     L.tp_array_count.argtypes = [P, i32, i32, i32]
     L.tp_array_count.restype = i64
+    L.tp_server_context.argtypes = [P]
+    L.tp_server_context.restype = P
+    L.tp_server_error.argtypes = [P]
+    L.tp_server_error.restype = ctypes.c_char_p
+    L.tp_server_clock.argtypes = [P]
+    L.tp_server_clock.restype = dbl
+    L.tp_server_result_count.argtypes = [P, i32]
+    L.tp_server_result_count.restype = i64
     _lib = L
     return L
 
@@ -103,6 +126,31 @@ def inflow_modes(nmodes, seed, tu, lturb, uoo, slit, span):
     return out
 
 
+def gust_mode(tu, kr, uoo):
+    """Single Fourier mode of a sinusoidal transverse gust (tp_gust_mode):
+    (array (1, 8) in the layout of inflow_modes, x-period slit).  Needs no GPU."""
+    out = np.zeros((1, 8))
+    u = np.ascontiguousarray(uoo, np.float64)
+    slit = ctypes.c_double()
+    _check(load().tp_gust_mode(float(tu), float(kr), _dp(u), _dp(out), ctypes.byref(slit)))
+    return out, slit.value
+
+
+def case_args(layout, params):
+    """Arrays of tp_add_case for a block layout and a parameter dict: the
+    layout arrays, iparams and dparams (unnamed parameters are 0, nkrk 4,
+    nviscous 1)."""
+    ipar = np.zeros(16, np.int32)
+    for i, k in enumerate(I_NAMES):
+        ipar[i] = int(params.get(k, {"nkrk": 4, "nviscous": 1}.get(k, 0)))
+    dpar = np.zeros(32)
+    for i, k in enumerate(D_NAMES):
+        dpar[i] = float(params.get(k, 0.0))
+    arrs = [np.ascontiguousarray(a, np.int32) for a in
+            (layout.dims, layout.nbc, layout.ncd, layout.nsz, layout.jct, layout.jvel)]
+    return arrs, ipar, dpar
+
+
 class Solver:
     """Device-resident state of one or more cases (each a list of sub-blocks)."""
 
@@ -113,27 +161,46 @@ class Solver:
         L = load()
         self.h = ctypes.c_void_p()
         _check(L.tp_create(ctypes.byref(self.h)))
-        self.layouts = []
+        self.layouts = {}
         self._keep = []
         for layout, params in cases:
-            ipar = np.zeros(16, np.int32)
-            for i, k in enumerate(I_NAMES):
-                ipar[i] = int(params.get(k, {"nkrk": 4, "nviscous": 1}.get(k, 0)))
-            dpar = np.zeros(32)
-            for i, k in enumerate(D_NAMES):
-                dpar[i] = float(params.get(k, 0.0))
-            arrs = [np.ascontiguousarray(a, np.int32) for a in
-                    (layout.dims, layout.nbc, layout.ncd, layout.nsz, layout.jct, layout.jvel)]
-            self._keep.append((arrs, ipar, dpar))
-            cid = ctypes.c_int32()
-            d, b, n, s, j, v = arrs
-            _check(L.tp_add_case(self.h, layout.nb, _ip(d), _ip(b), _ip(n), _ip(s), layout.njct,
-                                 _ip(j), _ip(v), _ip(ipar), _dp(dpar), ctypes.byref(cid)))
-            self.layouts.append(layout)
+            self.add_case(layout, params)
         _check(L.tp_set_option(self.h, 1, int(chunks)))
         _check(L.tp_finalize(self.h))
         if exact_lines:
             self.set_exact_lines(True)
+
+    def add_case(self, layout, params):
+        """Add a case; returns its handle.  After construction the case is
+        pending until admit()."""
+        arrs, ipar, dpar = case_args(layout, params)
+        self._keep.append((arrs, ipar, dpar))
+        cid = ctypes.c_int32()
+        d, b, n, s, j, v = arrs
+        _check(load().tp_add_case(self.h, layout.nb, _ip(d), _ip(b), _ip(n), _ip(s), layout.njct,
+                                  _ip(j), _ip(v), _ip(ipar), _dp(dpar), ctypes.byref(cid)))
+        self.layouts[cid.value] = layout
+        return cid.value
+
+    def case_bytes(self, case):
+        """Device bytes the case holds once admitted (needs its grids)."""
+        n = ctypes.c_int64()
+        _check(load().tp_case_bytes(self.h, case, ctypes.byref(n)))
+        return n.value
+
+    def admit(self, case):
+        """Put a pending case on the device (allocation, geometry); its state
+        is zero until initialo() or upload()."""
+        _check(load().tp_admit(self.h, case))
+
+    def remove_case(self, case):
+        _check(load().tp_remove_case(self.h, case))
+        self.layouts.pop(case, None)
+
+    def case_info(self, case):
+        out = (ctypes.c_int64 * 4)()
+        _check(load().tp_case_info(self.h, case, out))
+        return dict(state=out[0], points=out[1], blocks=out[2], device_bytes=out[3])
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -245,3 +312,90 @@ class Solver:
         out = (ctypes.c_int64 * 4)()
         _check(load().tp_info(self.h, out))
         return dict(points=out[0], blocks=out[1], cases=out[2], device_bytes=out[3])
+
+
+class Server:
+    """Request loop over one context (tp_server_*): cases queue with an arrival
+    time, are admitted while their device bytes fit the HBM budget and a slot
+    is free, advance together in shared launches, and leave with their final
+    qa copied to the host."""
+
+    STATES = ("queued", "running", "done", "failed")
+
+    def __init__(self, budget_bytes=0, max_active=64, graph=True):
+        L = load()
+        self.h = ctypes.c_void_p()
+        rc = L.tp_server_create(int(budget_bytes), int(max_active), 1 if graph else 0, ctypes.byref(self.h))
+        if rc != 0:
+            raise TPError(L.tp_last_error().decode())
+        self.ctx = ctypes.c_void_p(L.tp_server_context(self.h))
+        self.layouts = {}
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().tp_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise TPError(load().tp_server_error(self.h).decode())
+
+    def set_option(self, option, value):
+        _check(load().tp_set_option(self.ctx, int(option), int(value)))
+
+    def clock(self):
+        return load().tp_server_clock(self.h)
+
+    def submit(self, layout, params, grids, nsteps, t_arrival=-1.0):
+        """Queue a case: grids[b] = (3, N_b) coordinates of sub-block b."""
+        arrs, ipar, dpar = case_args(layout, params)
+        g = [np.ascontiguousarray(x, np.float64) for x in grids]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(g))(*[_dp(x) for x in g])
+        req = ctypes.c_int32()
+        d, b, n, s, j, v = arrs
+        self._check(load().tp_server_submit(self.h, layout.nb, _ip(d), _ip(b), _ip(n), _ip(s), layout.njct,
+                                            _ip(j), _ip(v), _ip(ipar), _dp(dpar), ptrs, int(nsteps),
+                                            float(t_arrival), ctypes.byref(req)))
+        self.layouts[req.value] = layout
+        return req.value
+
+    def iterate(self, max_steps=10):
+        st = np.zeros(6, np.int32)
+        self._check(load().tp_server_iterate(self.h, int(max_steps), _ip(st)))
+        return dict(zip(("active", "queued", "waiting", "done", "admitted", "steps"), map(int, st)))
+
+    def run(self, max_steps=10):
+        self._check(load().tp_server_run(self.h, int(max_steps)))
+
+    def result(self, req, with_state=True):
+        """(info dict, list of per-block qa arrays or None)."""
+        info = np.zeros(6)
+        L = load()
+        self._check(L.tp_server_result(self.h, int(req), None, 0, _dp(info)))
+        d = dict(state=self.STATES[int(info[0])], arrival=info[1], admitted=info[2], done=info[3],
+                 steps=int(info[4]), device_bytes=int(info[5]))
+        if not with_state or d["state"] != "done":
+            return d, None
+        n = L.tp_server_result_count(self.h, int(req))
+        out = np.empty(n)
+        self._check(L.tp_server_result(self.h, int(req), _dp(out), n, None))
+        lay = self.layouts[req]
+        parts, off = [], 0
+        for b in range(lay.nb):
+            m = 5 * lay.npoints(b)
+            parts.append(out[off:off + m])
+            off += m
+        return d, parts
+
+    def stats(self):
+        o = np.zeros(12)
+        self._check(load().tp_server_stats(self.h, _dp(o)))
+        keys = ("finished", "failed", "p50_s", "p99_s", "mean_s", "max_s", "wall_s", "cases_per_hour",
+                "peak_bytes", "peak_cases", "joined_running", "stepping_s")
+        return dict(zip(keys, (float(x) for x in o)))

@@ -11,7 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["tp_line.hip", "tp_col.hip", "tp_stage.hip", "tp_capi.hip"]
+SOURCES = ["tp_line.hip", "tp_col.hip", "tp_stage.hip", "tp_capi.hip", "tp_serve.cpp"]
 LIB = os.path.join(HERE, "libtandem_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TP_OFFLOAD_ARCH", "gfx950")
@@ -26,7 +26,7 @@ def build(verbose=False):
     procs = []
     for s in SOURCES:
         src = os.path.join(CSRC, s)
-        obj = os.path.join(CSRC, s.replace(".hip", ".o"))
+        obj = os.path.join(CSRC, os.path.splitext(s)[0] + ".o")
         objs.append(obj)
         cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
         if verbose:

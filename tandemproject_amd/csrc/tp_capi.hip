@@ -365,19 +365,49 @@ double gcbc_weights(double sbci[MBCI]) {
 // ----------------------------------------------------------------------------
 // context
 // ----------------------------------------------------------------------------
+// device allocations with one owner: a case (its blocks, tables and sponge
+// lists) or the context (launch tables, descriptors, scalars)
+struct Mem {
+  std::vector<void *> p;
+  size_t bytes = 0;
+  void release() {
+    for (void *q : p) hipFree(q);
+    p.clear();
+    bytes = 0;
+  }
+};
+
+// chunked-solver tables of one block: direction x kind x chunk-count index
+using ColTab = std::array<const double *, 3 * 2 * 5>;
+
 struct HostCase {
-  int nb = 0, block0 = 0, njct = 0;
+  int id = 0;                         // stable handle returned by tp_add_case
+  int nb = 0, njct = 0;
+  int block0 = 0, slot = -1;          // first global block and scalar slot while linked
+  bool built = false, geom = false;   // device memory allocated / metrics and sponge set
   std::vector<int> dims, nbc, ncd, nsz, jct, jvel;
   int ip[TP_I_NPARAM] = {0};
   double dp[TP_D_NPARAM] = {0};
   CaseCoef coef;
   double cbca_ra0 = 0;
   double albed[2][3][5], albef[2][3][5], alphf = 0, betf = 0;
+  Mem mem;
+  std::vector<Block> blk;                   // host views of the case's blocks
+  std::vector<ColTab> coltab;               // per block
+  std::vector<std::vector<double>> grid;    // host copy of ss per block
+  double sc[S_STRIDE];                      // scalars kept across a relink
+  bool sc_saved = false;
 };
 
 struct DevList {
   Work *d = nullptr;
   int n = 0;
+};
+
+// launch tables over a set of blocks
+struct ListSet {
+  DevList points, sponge, lines[3], colwork[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  std::vector<int> wr;   // first point-work item of each linked case, then the end
 };
 
 struct ProfRec {
@@ -389,27 +419,25 @@ struct tp_context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool finalized = false;
-  std::vector<HostCase> cases;
+  int next_id = 0, nkrk = 0;
+  std::vector<HostCase> cases;        // pending and built cases, in order of addition
+  // linked view of the built cases, rebuilt whenever a case joins or leaves
   std::vector<Block> blocks;          // host copies of the device views
-  std::vector<std::vector<double>> grids;
-  std::vector<int> sponge_n;
+  std::vector<std::pair<int, int>> bsrc;   // (case index, block of the case) per global block
+  std::vector<int> linked;            // case ids in scalar-slot order
+  Mem link;
   Block *d_blocks = nullptr;
   CaseCoef *d_coef = nullptr;
   double *d_sc = nullptr, *d_part = nullptr;
   int *d_wr = nullptr, *d_jct = nullptr, *d_jvel = nullptr;
   int njct = 0, maxk = 0;
-  std::vector<int> wr;                // point-work ranges per case
-  std::vector<void *> allocs;
-  size_t dev_bytes = 0;
-  DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  ListSet L;
   int nlp[3] = {0, 0, 0};
   size_t lds[3] = {0, 0, 0};
   bool exact_lines = false;        // sequential one-lane-per-line sweeps
   int col_fixed = 0;               // requested chunks per line (0: chosen per direction)
+  int tuned[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};   // autotuned chunks per line (-1: none)
   int colP[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // chunks per line, direction x kind (0: sequential kernel)
-  DevList colwork[3];              // groups of 64 lines
-  // carry maps per block, direction, kind and chunk count index (kColP)
-  std::vector<std::array<const double *, 3 * 2 * 5>> coltab;
   long long npts = 0;
   bool any_visc = false;
   // measurement
@@ -425,25 +453,28 @@ struct tp_context {
 
 namespace {
 
-int dalloc(tp_context *c, void **p, size_t bytes) {
-  if (bytes == 0) bytes = 256;
+// bytes an allocation of n bytes is accounted with (dalloc)
+size_t acct(size_t n) { return n == 0 ? 256 : n; }
+
+int dalloc(Mem &m, void **p, size_t bytes) {
+  bytes = acct(bytes);
   HIPCHK(hipMalloc(p, bytes));
-  c->allocs.push_back(*p);
-  c->dev_bytes += bytes;
+  m.p.push_back(*p);
+  m.bytes += bytes;
   return 0;
 }
 
 template <class T>
-int upload_vec(tp_context *c, T **dptr, const std::vector<T> &v) {
-  int rc = dalloc(c, (void **)dptr, sizeof(T) * std::max<size_t>(v.size(), 1));
+int upload_vec(Mem &m, T **dptr, const std::vector<T> &v) {
+  int rc = dalloc(m, (void **)dptr, sizeof(T) * std::max<size_t>(v.size(), 1));
   if (rc) return rc;
   if (!v.empty()) HIPCHK(hipMemcpy(*dptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
-int upload_list(tp_context *c, DevList &L, const std::vector<Work> &v) {
+int upload_list(Mem &m, DevList &L, const std::vector<Work> &v) {
   L.n = (int)v.size();
-  return upload_vec(c, &L.d, v);
+  return upload_vec(m, &L.d, v);
 }
 
 hipEvent_t take_event(tp_context *c) {
@@ -481,10 +512,10 @@ struct Timed {
 
 // ---------------- launch helpers ----------------
 void run_points(tp_context *c, int which, StageArgs a, int cls, double bytes_per_point) {
-  if (!c->points.n) return;
+  if (!c->L.points.n) return;
   Timed t(c, cls, bytes_per_point * (double)c->npts);
-  a.work = c->points.d;
-  launch_point(which, c->points.n, c->stream, a);
+  a.work = c->L.points.d;
+  launch_point(which, c->L.points.n, c->stream, a);
 }
 
 void run_faces(tp_context *c, const DevList &L, int which, StageArgs a, int cls) {
@@ -505,31 +536,31 @@ StageArgs base_args(tp_context *c) {
 }
 
 void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0) {
-  if (!c->exch.n) return;
+  if (!c->L.exch.n) return;
   Timed t(c, K_EXCH, 0.0);
   ExchArgs a;
-  a.blocks = c->d_blocks; a.work = c->exch.d; a.coef = c->d_coef;
+  a.blocks = c->d_blocks; a.work = c->L.exch.d; a.coef = c->d_coef;
   a.nt = nt; a.nrt = nrt; a.n45 = n45; a.arr = arr; a.col0 = col0;
-  launch_exchange(c->exch.n, c->stream, a);
+  launch_exchange(c->L.exch.n, c->stream, a);
 }
 
 void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_arr, int out_col, int m,
               int acc) {
-  if (!c->lines[d].n) return;
+  if (!c->L.lines[d].n) return;
   // algorithmic bytes: one field in, one out; an accumulated output is also read
   Timed t(c, kind == 0 ? K_DERIV : K_FILTER, (acc ? 24.0 : 16.0) * (double)c->npts);
   LineArgs a;
-  a.blocks = c->d_blocks; a.work = c->lines[d].d; a.coef = c->d_coef;
+  a.blocks = c->d_blocks; a.work = c->L.lines[d].d; a.coef = c->d_coef;
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
   a.m = m; a.acc = acc; a.nlp = c->nlp[d];
   const int P = c->colP[d][kind];
   if (!c->exact_lines && P) {
-    a.work = c->colwork[d].d;
-    launch_col(d, kind, col_need(c->blocks, d, P), P, c->colwork[d].n, c->stream, a);
+    a.work = c->L.colwork[d].d;
+    launch_col(d, kind, col_need(c->blocks, d, P), P, c->L.colwork[d].n, c->stream, a);
   } else if (kind == 0) {
-    launch_deriv(d, c->lines[d].n, c->lds[d], c->stream, a);
+    launch_deriv(d, c->L.lines[d].n, c->lds[d], c->stream, a);
   } else {
-    launch_filter(d, c->lines[d].n, c->lds[d], c->stream, a);
+    launch_filter(d, c->L.lines[d].n, c->lds[d], c->stream, a);
   }
 }
 
@@ -550,8 +581,10 @@ void select_col(tp_context *c, int d, int kind, int P) {
     return;
   }
   c->colP[d][kind] = P;
-  for (size_t g = 0; g < c->blocks.size(); g++)
-    c->blocks[g].col[d][kind] = c->coltab[g][(d * 2 + kind) * kNColP + pi];
+  for (size_t g = 0; g < c->blocks.size(); g++) {
+    const auto &src = c->bsrc[g];
+    c->blocks[g].col[d][kind] = c->cases[src.first].coltab[src.second][(d * 2 + kind) * kNColP + pi];
+  }
 }
 
 // copy the host block views to the device after a selection change
@@ -573,7 +606,7 @@ void enqueue_filter_step(tp_context *c) {
 }
 
 void enqueue_stage(tp_context *c, int nk) {
-  const int ncase = (int)c->cases.size();
+  const int ncase = (int)c->linked.size();
   launch_sc_stage(c->d_coef, c->d_sc, ncase, nk, c->stream);
   StageArgs a = base_args(c);
   a.nk = nk;
@@ -604,29 +637,29 @@ void enqueue_stage(tp_context *c, int nk) {
     run_line(c, 0, 1, F_FX, f0 + 1, F_DE, m - 1, m, 1);
     run_line(c, 0, 2, F_FX, f0 + 2, F_DE, m - 1, m, 1);
   }
-  if (c->sponge.n) {
+  if (c->L.sponge.n) {
     Timed t(c, K_MISC, 0.0);
     StageArgs s = a;
-    s.work = c->sponge.d;
-    launch_point(PK_SPONGE, c->sponge.n, c->stream, s);
+    s.work = c->L.sponge.d;
+    launch_point(PK_SPONGE, c->L.sponge.n, c->stream, s);
   }
-  run_faces(c, c->gprep, FK_GCBC_PREP, a, K_GCBC);
-  run_faces(c, c->gcopy, FK_COPY, a, K_GCBC);
-  for (int d = 0; d < 3; d++) run_faces(c, c->gapply[d], FK_GCBC_APPLY, a, K_GCBC);
+  run_faces(c, c->L.gprep, FK_GCBC_PREP, a, K_GCBC);
+  run_faces(c, c->L.gcopy, FK_COPY, a, K_GCBC);
+  for (int d = 0; d < 3; d++) run_faces(c, c->L.gapply[d], FK_GCBC_APPLY, a, K_GCBC);
   launch_sc_rk(c->d_coef, c->d_sc, ncase, nk, c->stream);
   run_points(c, PK_RK, a, K_RK, 128.0);
-  run_faces(c, c->wall, FK_WALL, a, K_GCBC);
+  run_faces(c, c->L.wall, FK_WALL, a, K_GCBC);
   if (c->njct) launch_junction(c->d_blocks, c->d_jct, c->d_jvel, c->njct, c->maxk, c->stream);
-  run_faces(c, c->acap, FK_AVG_CAP, a, K_GCBC);
-  run_faces(c, c->acap, FK_COPY, a, K_GCBC);
-  for (int d = 0; d < 3; d++) run_faces(c, c->aapply[d], FK_AVG_APPLY, a, K_GCBC);
+  run_faces(c, c->L.acap, FK_AVG_CAP, a, K_GCBC);
+  run_faces(c, c->L.acap, FK_COPY, a, K_GCBC);
+  for (int d = 0; d < 3; d++) run_faces(c, c->L.aapply[d], FK_AVG_APPLY, a, K_GCBC);
 }
 
 void enqueue_step(tp_context *c) {
+  if (c->linked.empty()) return;
   enqueue_filter_step(c);
-  const int nkrk = c->cases[0].ip[TP_I_NKRK];
-  for (int nk = 1; nk <= nkrk; nk++) enqueue_stage(c, nk);
-  launch_sc_end(c->d_sc, (int)c->cases.size(), nullptr, c->stream);
+  for (int nk = 1; nk <= c->nkrk; nk++) enqueue_stage(c, nk);
+  launch_sc_end(c->d_sc, (int)c->linked.size(), nullptr, c->stream);
 }
 
 int check_ready(tp_context *c) {
@@ -670,10 +703,23 @@ double *array_ptr(const Block &b, int id, int64_t *count) {
   return nullptr;
 }
 
+// index in c->cases of the case with handle id (-1: none)
+int case_index(const tp_context *c, int id) {
+  for (size_t i = 0; i < c->cases.size(); i++)
+    if (c->cases[i].id == id) return (int)i;
+  return -1;
+}
+
+// the linked case with handle id (nullptr: unknown, or not on the device)
+HostCase *linked_case(tp_context *c, int id) {
+  const int i = case_index(c, id);
+  return i >= 0 && c->cases[i].built && c->cases[i].slot >= 0 ? &c->cases[i] : nullptr;
+}
+
 int block_index(tp_context *c, int cs, int b) {
-  if (cs < 0 || cs >= (int)c->cases.size()) return -1;
-  if (b < 0 || b >= c->cases[cs].nb) return -1;
-  return c->cases[cs].block0 + b;
+  const HostCase *h = linked_case(c, cs);
+  if (!h || b < 0 || b >= h->nb) return -1;
+  return h->block0 + b;
 }
 
 }  // namespace
@@ -690,6 +736,383 @@ struct SplitMix {
   }
   double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
 };
+}  // namespace
+
+// ---- a case's device memory, and the linked view of all built cases ----
+namespace {
+
+// sub-array offsets of one block's arena (256-byte aligned)
+struct Arena {
+  size_t qa, qo, de, rr, ss, xi, et, ze, p, ya, tv, fx, f[3][4], lud, luf, rp, total;
+};
+
+Arena arena_layout(const Block &b, int nwall) {
+  Arena A;
+  size_t off = 0;
+  auto take = [&](size_t n) { size_t o = off; off += ((n * sizeof(double) + 255) / 256) * 256; return o; };
+  const size_t N = b.N;
+  A.qa = take(5 * N); A.qo = take(5 * N); A.de = take(5 * N); A.rr = take(3 * N); A.ss = take(3 * N);
+  A.xi = take(3 * N); A.et = take(3 * N); A.ze = take(3 * N); A.p = take(N); A.ya = take(N);
+  A.tv = take(9 * N); A.fx = take(15 * N);
+  for (int d = 0; d < 3; d++) {
+    A.f[d][0] = take(10 * (size_t)b.F[d]); A.f[d][1] = take(10 * (size_t)b.F[d]);
+    A.f[d][2] = take(6 * (size_t)b.F[d]); A.f[d][3] = take(6 * (size_t)b.F[d]);
+  }
+  A.lud = take(5 * (size_t)b.nlu); A.luf = take(5 * (size_t)b.nlu); A.rp = take(std::max(nwall, 1));
+  A.total = off;
+  return A;
+}
+
+// sizes, boundary codes, LU offsets and wall slots of block lb of a case;
+// ncd holds the neighbour's block index within the case
+Block block_shape(const HostCase &h, int lb, int *nwall) {
+  Block b;
+  std::memset(&b, 0, sizeof(b));
+  b.nx = h.dims[3 * lb] + 1; b.ny = h.dims[3 * lb + 1] + 1; b.nz = h.dims[3 * lb + 2] + 1;
+  b.N = b.nx * b.ny * b.nz;
+  b.L[0] = b.nx; b.L[1] = b.ny; b.L[2] = b.nz;
+  b.F[0] = b.ny * b.nz; b.F[1] = b.nz * b.nx; b.F[2] = b.nx * b.ny;
+  b.cs = -1;
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++) {
+      b.nbc[d][s] = h.nbc[6 * lb + 2 * d + s];
+      b.ncd[d][s] = h.ncd[6 * lb + 2 * d + s];
+      const int np = b.nbc[d][s];
+      b.ndf[d][s] = (np == 35 || np == 40 || np == 45) ? 1 : 0;
+    }
+  b.nlu = b.nx + b.ny + b.nz;
+  b.lu_off[0] = 0; b.lu_off[1] = b.nx; b.lu_off[2] = b.nx + b.ny;
+  int nw = 0;
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++) {
+      if (b.nbc[d][s] == 20 || b.nbc[d][s] == 25) { b.wall_off[d][s] = nw; nw += b.F[d]; }
+      else b.wall_off[d][s] = -1;
+    }
+  *nwall = nw;
+  return b;
+}
+
+// chunk length of P chunks over L points, 0 if the chunked solver cannot run it
+int col_chunk(int L, int P) {
+  const int Cl = (L + P - 1) / P;
+  return col_capacity(Cl) ? Cl : 0;
+}
+
+// sponge points of block lb from its coordinates, SingleTurbulence3D.f90:209-249:
+// the point list with the strength sg and the streamwise factor lm of each
+// (the coefficients are (1/yaco) sg lm and (1/yaco) sg/(gam-1))
+void sponge_select(const HostCase &h, int lb, const Block &b, const double *x, const double *y,
+                   const double *z, std::vector<int> &lst, std::vector<double> &sgv, std::vector<double> &lmv) {
+  const double domlen = h.dp[TP_D_DOMLEN], szth1 = h.dp[TP_D_SZTH1], szth2 = h.dp[TP_D_SZTH2];
+  const double szxt = h.dp[TP_D_SZXT], szco = h.dp[TP_D_SZCO];
+  const double p07 = (double)0.7f;   // single-precision literal 0.7e0 of the reference
+  const double szr[2][3] = {{1 / szth1, 1 / szth2, 0}, {1 / (szth1 + szxt), 1 / szth2, 0}};
+  const double szp[2][3] = {{-p07 * domlen + szth1, -p07 * domlen + szth2, 0},
+                            {domlen - szth1, domlen + szxt - szth2, 0}};
+  int nsz[3][2];
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++) nsz[d][s] = h.nsz[6 * lb + 2 * d + s];
+  const double ra1 = 0.25, ra2 = kPi / (2 * domlen);
+  lst.clear(); sgv.clear(); lmv.clear();
+  for (int l = 0; l < b.N; l++) {
+    const double cx[3] = {x[l], y[l], z[l]};
+    double r[3];
+    for (int d = 0; d < 3; d++)
+      r[d] = nsz[d][0] * szr[0][d] * std::fmax(szp[0][d] - cx[d], 0.0) +
+             nsz[d][1] * szr[1][d] * std::fmax(cx[d] - szp[1][d], 0.0);
+    const double sg = szco * (1 - 0.125 * (1 + std::cos(kPi * r[0])) * (1 + std::cos(kPi * r[1])) *
+                                      (1 + std::cos(kPi * r[2])));
+    const double lm = ra1 + 0.5 * (1 - ra1) * (1 + std::cos(ra2 * (std::fmin(x[l], domlen) + domlen)));
+    if (sg > 0) {
+      lst.push_back(l);
+      sgv.push_back(sg);
+      lmv.push_back(lm);
+    }
+  }
+}
+
+// inflow points: sponge points upstream of the inflow sponge edge inside the
+// band |y| < gustw, with the ramp sin^2 of SingleTurbulence3D.f90:253-268
+void inflow_select(const HostCase &h, const double *x, const double *y, const std::vector<int> &lst,
+                   std::vector<int> &gix, std::vector<int> &pts, std::vector<double> &atz) {
+  const double domlen = h.dp[TP_D_DOMLEN], szth1 = h.dp[TP_D_SZTH1];
+  const double edge = -(double)0.7f * domlen + szth1;
+  const double gw = h.dp[TP_D_GUSTW], ra0 = kPi / szth1;
+  gix.assign(lst.size(), -1);
+  pts.clear(); atz.clear();
+  for (size_t q = 0; q < lst.size(); q++) {
+    const int l = lst[q];
+    if (x[l] < edge && std::fabs(y[l]) < gw) {
+      gix[q] = (int)pts.size();
+      pts.push_back(l);
+      const double sn = std::sin(std::fmin(ra0 * (x[l] + domlen), kPi / 2));
+      atz.push_back(sn * sn);
+    }
+  }
+}
+
+int inflow_nmodes(const HostCase &h) { return h.coef.inflow == 2 ? 1 : h.ip[TP_I_NMODES]; }
+
+// device bytes a case takes once admitted: block arenas, chunked-solver
+// tables, and its sponge and inflow arrays (from the host grid)
+size_t case_bytes(const HostCase &h) {
+  size_t total = 0;
+  for (int lb = 0; lb < h.nb; lb++) {
+    int nwall = 0;
+    const Block b = block_shape(h, lb, &nwall);
+    total += acct(arena_layout(b, nwall).total);
+    for (int d = 0; d < 3; d++)
+      for (int P : kColP) {
+        const int Cl = col_chunk(b.L[d], P);
+        if (Cl) total += 2 * acct(sizeof(double) * (128 + 192 * (size_t)((b.L[d] + Cl - 1) / Cl)));
+      }
+    if (lb >= (int)h.grid.size() || h.grid[lb].empty()) continue;
+    const double *x = h.grid[lb].data(), *y = x + b.N, *z = y + b.N;
+    std::vector<int> lst, gix, pts;
+    std::vector<double> sg, lm, atz;
+    sponge_select(h, lb, b, x, y, z, lst, sg, lm);
+    if (!lst.empty()) total += acct(4 * lst.size()) + 2 * acct(8 * lst.size());
+    if (h.coef.inflow) {
+      inflow_select(h, x, y, lst, gix, pts, atz);
+      total += acct(4 * std::max<size_t>(gix.size(), 1));
+      if (!pts.empty())
+        total += acct(8 * pts.size()) + acct(sizeof(double) * 3 * (h.coef.ntz + 1) * pts.size());
+    }
+  }
+  return total;
+}
+
+// device memory of a case: block arenas (zeroed), LU factors of the
+// derivative and filter matrices (src/Main3D.f90:316-332), chunked-solver
+// tables for every chunk count the block's lines allow, and the grid if set
+int build_case(tp_context *c, HostCase &h) {
+  h.blk.assign(h.nb, Block{});
+  h.coltab.assign(h.nb, ColTab{});
+  h.grid.resize(h.nb);
+  int rc;
+  for (int lb = 0; lb < h.nb; lb++) {
+    int nwall = 0;
+    Block b = block_shape(h, lb, &nwall);
+    const Arena A = arena_layout(b, nwall);
+    char *base = nullptr;
+    if ((rc = dalloc(h.mem, (void **)&base, A.total))) return rc;
+    HIPCHK(hipMemset(base, 0, A.total));
+    auto P = [&](size_t o) { return (double *)(base + o); };
+    b.qa = P(A.qa); b.qo = P(A.qo); b.de = P(A.de); b.rr = P(A.rr); b.ss = P(A.ss);
+    b.xim = P(A.xi); b.etm = P(A.et); b.zem = P(A.ze); b.p = P(A.p); b.yaco = P(A.ya); b.tv = P(A.tv);
+    b.fx = P(A.fx);
+    for (int d = 0; d < 3; d++) {
+      b.drva[d] = P(A.f[d][0]); b.drvb[d] = P(A.f[d][1]); b.recv[d] = P(A.f[d][2]); b.cm[d] = P(A.f[d][3]);
+    }
+    b.lud = P(A.lud); b.luf = P(A.luf); b.rpex = P(A.rp);
+    std::vector<double> lud(5 * b.nlu), luf(5 * b.nlu);
+    const int nl = b.nlu;
+    for (int d = 0; d < 3; d++) {
+      const int n = b.L[d], o = b.lu_off[d];
+      const int ns = b.ndf[d][0], ne = b.ndf[d][1];
+      band_lu(n, h.albed[ns], h.albed[ne], kAlpha, kBeta, &lud[0 * nl + o], &lud[1 * nl + o], &lud[2 * nl + o],
+              &lud[3 * nl + o], &lud[4 * nl + o]);
+      band_lu(n, h.albef[ns], h.albef[ne], h.alphf, h.betf, &luf[0 * nl + o], &luf[1 * nl + o], &luf[2 * nl + o],
+              &luf[3 * nl + o], &luf[4 * nl + o]);
+    }
+    HIPCHK(hipMemcpy(b.lud, lud.data(), sizeof(double) * lud.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.luf, luf.data(), sizeof(double) * luf.size(), hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; d++) {
+      const int L = b.L[d], o = b.lu_off[d];
+      for (int pi = 0; pi < kNColP; pi++) {
+        if (!col_chunk(L, kColP[pi])) continue;
+        for (int kind = 0; kind < 2; kind++) {
+          const std::vector<double> &lu = kind == 0 ? lud : luf;
+          std::vector<double> T = col_tables(L, kColP[pi], &lu[0 * nl + o], &lu[1 * nl + o], &lu[2 * nl + o],
+                                             &lu[3 * nl + o], &lu[4 * nl + o]);
+          double *dptr = nullptr;
+          if ((rc = upload_vec(h.mem, &dptr, T))) return rc;
+          h.coltab[lb][(d * 2 + kind) * kNColP + pi] = dptr;
+        }
+      }
+    }
+    if (!h.grid[lb].empty())
+      HIPCHK(hipMemcpy(b.ss, h.grid[lb].data(), sizeof(double) * 3 * b.N, hipMemcpyHostToDevice));
+    h.blk[lb] = b;
+  }
+  h.built = true;
+  h.geom = false;
+  h.sc_saved = false;
+  return 0;
+}
+
+// launch tables over the global blocks gs (sorted by case)
+int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
+  std::vector<Work> pts, sp, lw[3], cw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm;
+  L.wr.assign(c->linked.size() + 1, 0);
+  int slot = -1;
+  for (int g : gs) {
+    const Block &b = c->blocks[g];
+    while (slot < b.cs) L.wr[++slot] = (int)pts.size();
+    const int CH = 1024;
+    for (int a0 = 0; a0 < b.N; a0 += CH) pts.push_back({g, a0, std::min(CH, b.N - a0), 0});
+    for (int a0 = 0; a0 < b.nsponge; a0 += 1024) sp.push_back({g, a0, std::min(1024, b.nsponge - a0), 0});
+    for (int d = 0; d < 3; d++) {
+      const int nlines = b.N / b.L[d], nl = c->nlp[d] - 1;
+      for (int q = 0; q < nlines; q += 64) cw[d].push_back({g, q, std::min(64, nlines - q), d});
+      for (int q = 0; q < nlines; q += nl) lw[d].push_back({g, q, std::min(nl, nlines - q), d});
+    }
+    const HostCase &h = c->cases[c->bsrc[g].first];
+    const bool gcic = h.ip[TP_I_NEXTGCIC] == 1;
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) {
+        const int np = b.nbc[d][s], F = b.F[d];
+        auto add = [&](std::vector<Work> &v) {
+          for (int a0 = 0; a0 < F; a0 += 256) v.push_back({g, a0, std::min(256, F - a0), d | (s << 8)});
+        };
+        const bool gc_face = np == 10 || np == 20 || np == 25 || np == 30;
+        const bool wall = np == 20 || np == 25;
+        if (b.ndf[d][s]) add(ex);
+        if (gc_face) { add(gp); add(ga[d]); }
+        if (np == 30 || (gcic && wall)) add(gc);
+        if (wall) add(wl);
+        if (np == 30 || np == 35 || np == 45 || (gcic && wall)) { add(ac); add(aa[d]); }
+        add(nm);
+      }
+  }
+  while (slot < (int)c->linked.size()) L.wr[++slot] = (int)pts.size();
+  int rc;
+  if ((rc = upload_list(m, L.points, pts))) return rc;
+  if ((rc = upload_list(m, L.sponge, sp))) return rc;
+  if ((rc = upload_list(m, L.exch, ex))) return rc;
+  if ((rc = upload_list(m, L.gprep, gp))) return rc;
+  if ((rc = upload_list(m, L.gcopy, gc))) return rc;
+  if ((rc = upload_list(m, L.wall, wl))) return rc;
+  if ((rc = upload_list(m, L.acap, ac))) return rc;
+  if ((rc = upload_list(m, L.normals, nm))) return rc;
+  for (int d = 0; d < 3; d++) {
+    if ((rc = upload_list(m, L.lines[d], lw[d]))) return rc;
+    if ((rc = upload_list(m, L.colwork[d], cw[d]))) return rc;
+    if ((rc = upload_list(m, L.gapply[d], ga[d]))) return rc;
+    if ((rc = upload_list(m, L.aapply[d], aa[d]))) return rc;
+  }
+  return 0;
+}
+
+// rebuild the linked view after a case joined or left: global block
+// numbering, scalar slots (each case keeps its scalars), chunk counts,
+// descriptors, coefficients and launch tables
+int link(tp_context *c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->d_sc && !c->linked.empty()) {
+    std::vector<double> sc(c->linked.size() * S_STRIDE);
+    HIPCHK(hipMemcpy(sc.data(), c->d_sc, sizeof(double) * sc.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < c->linked.size(); i++) {
+      const int ci = case_index(c, c->linked[i]);
+      if (ci < 0 || !c->cases[ci].built) continue;
+      std::memcpy(c->cases[ci].sc, &sc[i * S_STRIDE], sizeof(double) * S_STRIDE);
+      c->cases[ci].sc_saved = true;
+    }
+  }
+  drop_graph(c);
+  c->link.release();
+  c->d_blocks = nullptr; c->d_coef = nullptr; c->d_sc = nullptr; c->d_part = nullptr;
+  c->d_wr = nullptr; c->d_jct = nullptr; c->d_jvel = nullptr;
+  c->L = ListSet();
+  c->blocks.clear(); c->bsrc.clear(); c->linked.clear();
+  c->npts = 0; c->any_visc = false; c->njct = 0; c->maxk = 0;
+  std::vector<CaseCoef> coefs;
+  std::vector<double> sc;
+  for (size_t i = 0; i < c->cases.size(); i++) {
+    HostCase &h = c->cases[i];
+    h.slot = -1;
+    if (!h.built) continue;
+    h.block0 = (int)c->blocks.size();
+    h.slot = (int)c->linked.size();
+    c->linked.push_back(h.id);
+    coefs.push_back(h.coef);
+    if (h.sc_saved) {
+      sc.insert(sc.end(), h.sc, h.sc + S_STRIDE);
+    } else {
+      sc.insert(sc.end(), S_STRIDE, 0.0);
+      sc[h.slot * S_STRIDE + S_AOA] = h.dp[TP_D_AOA0];
+    }
+    if (h.ip[TP_I_NVISCOUS] == 1) c->any_visc = true;
+    for (int lb = 0; lb < h.nb; lb++) {
+      Block b = h.blk[lb];
+      b.cs = h.slot;
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++) {
+          const int nbr = h.ncd[6 * lb + 2 * d + s];
+          b.ncd[d][s] = nbr >= 0 ? h.block0 + nbr : -1;
+        }
+      c->blocks.push_back(b);
+      c->bsrc.push_back({(int)i, lb});
+      c->npts += b.N;
+    }
+  }
+  const int nb = (int)c->blocks.size();
+  int rc;
+  // sequential kernels: NL lines of one block per 64-lane workgroup, LDS budget ~40 KB
+  for (int d = 0; d < 3; d++) {
+    int Lmax = 1;
+    for (auto &b : c->blocks) Lmax = std::max(Lmax, b.L[d]);
+    int nl = std::max(1, std::min(64, (int)(40960 / (8 * Lmax)) - 1));
+    if (nl >= 16) nl = (nl / 16) * 16;
+    c->nlp[d] = nl + 1;
+    c->lds[d] = (size_t)Lmax * (nl + 1) * sizeof(double);
+    if (c->lds[d] > 64 * 1024) return fail(TP_E_SIZE, "line too long for the LDS tile");
+  }
+  // chunked solver: the tuned chunk count if it still fits every block, else
+  // the requested one, else the default
+  for (int d = 0; d < 3; d++)
+    for (int kind = 0; kind < 2; kind++) {
+      int P = col_default(c->blocks, d);
+      if (c->col_fixed && col_need(c->blocks, d, c->col_fixed)) P = c->col_fixed;
+      const int t = c->tuned[d][kind];
+      if (t == 0 || (t > 0 && col_need(c->blocks, d, t))) P = t;
+      select_col(c, d, kind, P);
+    }
+  std::vector<int> gs(nb);
+  for (int g = 0; g < nb; g++) gs[g] = g;
+  if ((rc = make_lists(c, gs, c->link, c->L))) return rc;
+  if ((rc = upload_vec(c->link, &c->d_blocks, c->blocks))) return rc;
+  if ((rc = upload_vec(c->link, &c->d_coef, coefs))) return rc;
+  if ((rc = upload_vec(c->link, &c->d_sc, sc))) return rc;
+  std::vector<double> part(2 * std::max(c->L.points.n, 1), 0.0);
+  if ((rc = upload_vec(c->link, &c->d_part, part))) return rc;
+  if ((rc = upload_vec(c->link, &c->d_wr, c->L.wr))) return rc;
+  // junctions
+  std::vector<int> jct, jvel;
+  for (const HostCase &h : c->cases) {
+    if (!h.built) continue;
+    for (int j = 0; j < h.njct; j++) {
+      for (int q = 0; q < 4; q++) {
+        const int lb = h.jct[12 * j + 3 * q];
+        jct.push_back(h.block0 + lb);
+        jct.push_back(h.jct[12 * j + 3 * q + 1]);
+        jct.push_back(h.jct[12 * j + 3 * q + 2]);
+        c->maxk = std::max(c->maxk, c->blocks[h.block0 + lb].nz);
+      }
+      jvel.push_back(h.jvel[2 * j]);
+      jvel.push_back(h.jvel[2 * j + 1]);
+      c->njct++;
+    }
+  }
+  if ((rc = upload_vec(c->link, &c->d_jct, jct))) return rc;
+  if ((rc = upload_vec(c->link, &c->d_jvel, jvel))) return rc;
+  return 0;
+}
+
+// the single Fourier mode of a sinusoidal transverse gust: wavevector along
+// the free stream with |k| = 2 kr (reduced frequency kr = omega c / (2 |u|),
+// chord 1), velocity amplitude tu |u| normal to the free stream in the x-y
+// plane; slit is the table's x-period, so that slit / u_x is the gust period
+void gust_mode(double tu, double kr, const double *u, double *m8, double *slit) {
+  const double um = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const double uxy = std::sqrt(u[0] * u[0] + u[1] * u[1]);
+  const double ca = u[0] / uxy, sa = u[1] / uxy;
+  m8[0] = 2 * kr * u[0] / um; m8[1] = 2 * kr * u[1] / um; m8[2] = 2 * kr * u[2] / um;
+  m8[3] = -tu * um * sa; m8[4] = tu * um * ca; m8[5] = 0;
+  m8[6] = 0; m8[7] = 0;
+  *slit = kPi * u[0] / (kr * um);
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------------------
@@ -776,7 +1199,8 @@ int32_t tp_destroy(tp_context *c) {
   for (int k = 0; k < K_NCLASS; k++)
     for (auto &r : c->pend[k]) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
   for (auto e : c->pool) hipEventDestroy(e);
-  for (void *p : c->allocs) hipFree(p);
+  for (auto &h : c->cases) h.mem.release();
+  c->link.release();
   hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -786,10 +1210,17 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
                     const int32_t *nsz, int32_t njct, const int32_t *jct, const int32_t *jvel,
                     const int32_t *iparams, const double *dparams, int32_t *case_id) {
   if (!c) return fail(TP_E_ARG, "null context");
-  if (c->finalized) return fail(TP_E_STATE, "cases must be added before tp_finalize");
   if (nb <= 0 || !dims || !nbc || !ncd || !nsz || !iparams || !dparams)
     return fail(TP_E_ARG, "tp_add_case: missing arrays");
   if ((int)c->cases.size() >= MAX_CASE) return fail(TP_E_SIZE, "too many cases");
+  if (njct > 0 && (!jct || !jvel)) return fail(TP_E_ARG, "junction arrays missing");
+  for (int j = 0; j < njct; j++)
+    for (int q = 0; q < 4; q++) {
+      const int lb = jct[12 * j + 3 * q];
+      if (lb < 0 || lb >= nb) return fail(TP_E_LAYOUT, "junction member outside its case");
+      if (dims[3 * lb + 2] != dims[3 * jct[12 * j] + 2])
+        return fail(TP_E_LAYOUT, "junction members differ in zeta size");
+    }
   HostCase h;
   h.nb = nb;
   h.njct = njct;
@@ -834,10 +1265,9 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
                                      std::to_string(L) + " points; its faces need >= " + std::to_string(need));
     }
   }
-  if (!c->cases.empty() && c->cases[0].ip[TP_I_NKRK] != iparams[TP_I_NKRK])
+  if (c->nkrk && c->nkrk != iparams[TP_I_NKRK])
     return fail(TP_E_ARG, "all cases of a context must use the same number of RK stages");
   if (iparams[TP_I_NKRK] < 1) return fail(TP_E_ARG, "nkrk must be >= 1");
-  h.block0 = c->cases.empty() ? 0 : c->cases.back().block0 + c->cases.back().nb;
   // coefficients, src/Main3D.f90:60-63, 71-73, 295-312
   CaseCoef &k = h.coef;
   std::memset(&k, 0, sizeof(k));
@@ -877,17 +1307,28 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
   k.nkrk = iparams[TP_I_NKRK]; k.nviscous = iparams[TP_I_NVISCOUS]; k.nsmf = iparams[TP_I_NSMF];
   k.les = iparams[TP_I_LES]; k.nextgcic = iparams[TP_I_NEXTGCIC];
   k.inflow = iparams[TP_I_INFLOW];
+  if (k.inflow < 0 || k.inflow > 2) return fail(TP_E_ARG, "TP_I_INFLOW must be 0, 1 or 2");
   if (k.inflow) {
     k.ntz = iparams[TP_I_NTZ];
     if (k.ntz < 4) return fail(TP_E_ARG, "inflow table needs TP_I_NTZ >= 4 time intervals");
-    if (iparams[TP_I_NMODES] < 1) return fail(TP_E_ARG, "inflow needs TP_I_NMODES >= 1");
-    if (!(k.uoo[0] > 0) || !(dparams[TP_D_SLIT] > 0) || !(dparams[TP_D_LTURB] > 0))
-      return fail(TP_E_ARG, "inflow needs u_inf along +x, TP_D_SLIT > 0 and TP_D_LTURB > 0");
+    if (!(k.uoo[0] > 0)) return fail(TP_E_ARG, "inflow needs u_inf with a positive x component");
+    double slit = dparams[TP_D_SLIT];
+    if (k.inflow == 1) {
+      if (iparams[TP_I_NMODES] < 1) return fail(TP_E_ARG, "inflow needs TP_I_NMODES >= 1");
+      if (!(slit > 0) || !(dparams[TP_D_LTURB] > 0))
+        return fail(TP_E_ARG, "turbulent inflow needs TP_D_SLIT > 0 and TP_D_LTURB > 0");
+    } else {
+      if (!(dparams[TP_D_GUSTK] > 0)) return fail(TP_E_ARG, "gust inflow needs TP_D_GUSTK > 0");
+      double m8[8];
+      gust_mode(dparams[TP_D_TU], dparams[TP_D_GUSTK], k.uoo, m8, &slit);
+    }
     k.tgusto = dparams[TP_D_TGUSTO];
-    k.period = dparams[TP_D_SLIT] / k.uoo[0];
-    k.fctr = dparams[TP_D_SLIT] / (k.ntz * k.uoo[0]);
+    k.period = slit / k.uoo[0];
+    k.fctr = slit / (k.ntz * k.uoo[0]);
   }
-  if (case_id) *case_id = (int)c->cases.size();
+  h.id = c->next_id++;
+  c->nkrk = iparams[TP_I_NKRK];
+  if (case_id) *case_id = h.id;
   c->cases.push_back(std::move(h));
   return 0;
 }
@@ -895,371 +1336,183 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
 int32_t tp_finalize(tp_context *c) {
   if (!c) return fail(TP_E_ARG, "null context");
   if (c->finalized) return fail(TP_E_STATE, "already finalized");
-  if (c->cases.empty()) return fail(TP_E_STATE, "no case added");
   HIPCHK(hipSetDevice(c->device));
-  const int ncase = (int)c->cases.size();
-  std::vector<std::vector<double>> lu_d, lu_f;   // host LU factors per block
-  // ---- blocks and their device memory ----
-  for (int cs = 0; cs < ncase; cs++) {
-    HostCase &h = c->cases[cs];
-    if (h.ip[TP_I_NVISCOUS] == 1) c->any_visc = true;
-    for (int lb = 0; lb < h.nb; lb++) {
-      Block b;
-      std::memset(&b, 0, sizeof(b));
-      b.nx = h.dims[3 * lb] + 1; b.ny = h.dims[3 * lb + 1] + 1; b.nz = h.dims[3 * lb + 2] + 1;
-      b.N = b.nx * b.ny * b.nz;
-      b.L[0] = b.nx; b.L[1] = b.ny; b.L[2] = b.nz;
-      b.F[0] = b.ny * b.nz; b.F[1] = b.nz * b.nx; b.F[2] = b.nx * b.ny;
-      b.cs = cs;
-      for (int d = 0; d < 3; d++)
-        for (int s = 0; s < 2; s++) {
-          b.nbc[d][s] = h.nbc[6 * lb + 2 * d + s];
-          const int nbr = h.ncd[6 * lb + 2 * d + s];
-          b.ncd[d][s] = nbr >= 0 ? h.block0 + nbr : -1;
-          const int np = b.nbc[d][s];
-          b.ndf[d][s] = (np == 35 || np == 40 || np == 45) ? 1 : 0;
-        }
-      b.nlu = b.nx + b.ny + b.nz;
-      b.lu_off[0] = 0; b.lu_off[1] = b.nx; b.lu_off[2] = b.nx + b.ny;
-      int nwall = 0;
-      for (int d = 0; d < 3; d++)
-        for (int s = 0; s < 2; s++) {
-          if (b.nbc[d][s] == 20 || b.nbc[d][s] == 25) { b.wall_off[d][s] = nwall; nwall += b.F[d]; }
-          else b.wall_off[d][s] = -1;
-        }
-      // one allocation per block, 256-byte aligned sub-arrays
-      size_t off = 0;
-      auto take = [&](size_t n) { size_t o = off; off += ((n * sizeof(double) + 255) / 256) * 256; return o; };
-      const size_t N = b.N;
-      size_t o_qa = take(5 * N), o_qo = take(5 * N), o_de = take(5 * N), o_rr = take(3 * N), o_ss = take(3 * N);
-      size_t o_xi = take(3 * N), o_et = take(3 * N), o_ze = take(3 * N), o_p = take(N), o_ya = take(N);
-      size_t o_tv = take(9 * N), o_fx = take(15 * N);
-      size_t o_f[3][4];
-      for (int d = 0; d < 3; d++) {
-        o_f[d][0] = take(10 * (size_t)b.F[d]); o_f[d][1] = take(10 * (size_t)b.F[d]);
-        o_f[d][2] = take(6 * (size_t)b.F[d]); o_f[d][3] = take(6 * (size_t)b.F[d]);
-      }
-      size_t o_lud = take(5 * (size_t)b.nlu), o_luf = take(5 * (size_t)b.nlu), o_rp = take(std::max(nwall, 1));
-      char *base = nullptr;
-      int rc = dalloc(c, (void **)&base, off);
-      if (rc) return rc;
-      HIPCHK(hipMemset(base, 0, off));
-      auto P = [&](size_t o) { return (double *)(base + o); };
-      b.qa = P(o_qa); b.qo = P(o_qo); b.de = P(o_de); b.rr = P(o_rr); b.ss = P(o_ss);
-      b.xim = P(o_xi); b.etm = P(o_et); b.zem = P(o_ze); b.p = P(o_p); b.yaco = P(o_ya); b.tv = P(o_tv);
-      b.fx = P(o_fx);
-      for (int d = 0; d < 3; d++) {
-        b.drva[d] = P(o_f[d][0]); b.drvb[d] = P(o_f[d][1]); b.recv[d] = P(o_f[d][2]); b.cm[d] = P(o_f[d][3]);
-      }
-      b.lud = P(o_lud); b.luf = P(o_luf); b.rpex = P(o_rp);
-      // LU factors of the derivative and filter matrices, src/Main3D.f90:316-332
-      std::vector<double> lud(5 * b.nlu), luf(5 * b.nlu);
-      for (int d = 0; d < 3; d++) {
-        const int n = b.L[d], o = b.lu_off[d], nl = b.nlu;
-        const int ns = b.ndf[d][0], ne = b.ndf[d][1];
-        band_lu(n, h.albed[ns], h.albed[ne], kAlpha, kBeta, &lud[0 * nl + o], &lud[1 * nl + o], &lud[2 * nl + o],
-                &lud[3 * nl + o], &lud[4 * nl + o]);
-        band_lu(n, h.albef[ns], h.albef[ne], h.alphf, h.betf, &luf[0 * nl + o], &luf[1 * nl + o], &luf[2 * nl + o],
-                &luf[3 * nl + o], &luf[4 * nl + o]);
-      }
-      HIPCHK(hipMemcpy(b.lud, lud.data(), sizeof(double) * lud.size(), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(b.luf, luf.data(), sizeof(double) * luf.size(), hipMemcpyHostToDevice));
-      lu_d.push_back(lud);
-      lu_f.push_back(luf);
-      c->blocks.push_back(b);
-      c->grids.emplace_back();
-      c->npts += b.N;
-    }
-  }
-  const int nb = (int)c->blocks.size();
   int rc;
-  // chunked solver: carry maps for every usable chunk count, 64-line groups
-  c->coltab.assign(nb, {});
-  for (int d = 0; d < 3; d++) {
-    std::vector<Work> cw;
-    for (int g = 0; g < nb; g++) {
-      Block &b = c->blocks[g];
-      const int L = b.L[d], nl = b.nlu, o = b.lu_off[d];
-      for (int pi = 0; pi < kNColP; pi++) {
-        if (!col_need(c->blocks, d, kColP[pi])) continue;
-        for (int kind = 0; kind < 2; kind++) {
-          const std::vector<double> &lu = kind == 0 ? lu_d[g] : lu_f[g];
-          std::vector<double> T = col_tables(L, kColP[pi], &lu[0 * nl + o], &lu[1 * nl + o],
-                                             &lu[2 * nl + o], &lu[3 * nl + o], &lu[4 * nl + o]);
-          double *dptr = nullptr;
-          if ((rc = upload_vec(c, &dptr, T))) return rc;
-          c->coltab[g][(d * 2 + kind) * kNColP + pi] = dptr;
-        }
-      }
-      const int nlines = b.N / L;
-      for (int q = 0; q < nlines; q += 64) cw.push_back({g, q, std::min(64, nlines - q), d});
-    }
-    if ((rc = upload_list(c, c->colwork[d], cw))) return rc;
-    int P = col_default(c->blocks, d);
-    if (c->col_fixed && col_need(c->blocks, d, c->col_fixed)) P = c->col_fixed;
-    for (int kind = 0; kind < 2; kind++) select_col(c, d, kind, P);
-  }
-  if ((rc = upload_vec(c, &c->d_blocks, c->blocks))) return rc;
-  std::vector<CaseCoef> coefs;
-  for (auto &h : c->cases) coefs.push_back(h.coef);
-  if ((rc = upload_vec(c, &c->d_coef, coefs))) return rc;
-  std::vector<double> sc(ncase * S_STRIDE, 0.0);
-  for (int cs = 0; cs < ncase; cs++) sc[cs * S_STRIDE + S_AOA] = c->cases[cs].dp[TP_D_AOA0];
-  if ((rc = upload_vec(c, &c->d_sc, sc))) return rc;
-  // ---- work tables ----
-  std::vector<Work> pts;
-  c->wr.assign(ncase + 1, 0);
-  for (int cs = 0; cs < ncase; cs++) {
-    c->wr[cs] = (int)pts.size();
-    const HostCase &h = c->cases[cs];
-    for (int lb = 0; lb < h.nb; lb++) {
-      const int g = h.block0 + lb;
-      const int CH = 1024;
-      for (int a0 = 0; a0 < c->blocks[g].N; a0 += CH) pts.push_back({g, a0, std::min(CH, c->blocks[g].N - a0), 0});
-    }
-  }
-  c->wr[ncase] = (int)pts.size();
-  if ((rc = upload_list(c, c->points, pts))) return rc;
-  std::vector<double> part(2 * std::max<size_t>(pts.size(), 1), 0.0);
-  if ((rc = upload_vec(c, &c->d_part, part))) return rc;
-  if ((rc = upload_vec(c, &c->d_wr, c->wr))) return rc;
-  // line tiles: NL lines of one block per 64-lane workgroup, LDS budget ~40 KB
-  for (int d = 0; d < 3; d++) {
-    int Lmax = 0;
-    for (auto &b : c->blocks) Lmax = std::max(Lmax, b.L[d]);
-    int nl = std::max(1, std::min(64, (int)(40960 / (8 * Lmax)) - 1));
-    if (nl >= 16) nl = (nl / 16) * 16;
-    c->nlp[d] = nl + 1;
-    c->lds[d] = (size_t)Lmax * (nl + 1) * sizeof(double);
-    if (c->lds[d] > 64 * 1024) return fail(TP_E_SIZE, "line too long for the LDS tile");
-    std::vector<Work> lw;
-    for (int g = 0; g < nb; g++) {
-      const Block &b = c->blocks[g];
-      const int nlines = b.N / b.L[d];
-      for (int q = 0; q < nlines; q += nl) lw.push_back({g, q, std::min(nl, nlines - q), d});
-    }
-    if ((rc = upload_list(c, c->lines[d], lw))) return rc;
-  }
-  // face tables
-  std::vector<Work> ex, gp, ga[3], gc, wl, ac, aa[3], nm;
-  for (int g = 0; g < nb; g++) {
-    const Block &b = c->blocks[g];
-    const bool gcic = c->cases[b.cs].ip[TP_I_NEXTGCIC] == 1;
-    for (int d = 0; d < 3; d++)
-      for (int s = 0; s < 2; s++) {
-        const int np = b.nbc[d][s], F = b.F[d];
-        auto add = [&](std::vector<Work> &v) {
-          for (int a0 = 0; a0 < F; a0 += 256) v.push_back({g, a0, std::min(256, F - a0), d | (s << 8)});
-        };
-        const bool gc_face = np == 10 || np == 20 || np == 25 || np == 30;
-        const bool wall = np == 20 || np == 25;
-        if (b.ndf[d][s]) add(ex);
-        if (gc_face) { add(gp); add(ga[d]); }
-        if (np == 30 || (gcic && wall)) add(gc);
-        if (wall) add(wl);
-        if (np == 30 || np == 35 || np == 45 || (gcic && wall)) { add(ac); add(aa[d]); }
-        add(nm);
-      }
-  }
-  if ((rc = upload_list(c, c->exch, ex))) return rc;
-  if ((rc = upload_list(c, c->gprep, gp))) return rc;
-  if ((rc = upload_list(c, c->gcopy, gc))) return rc;
-  if ((rc = upload_list(c, c->wall, wl))) return rc;
-  if ((rc = upload_list(c, c->acap, ac))) return rc;
-  if ((rc = upload_list(c, c->normals, nm))) return rc;
-  for (int d = 0; d < 3; d++) {
-    if ((rc = upload_list(c, c->gapply[d], ga[d]))) return rc;
-    if ((rc = upload_list(c, c->aapply[d], aa[d]))) return rc;
-  }
-  // junctions
-  std::vector<int> jct, jvel;
   for (auto &h : c->cases)
-    for (int j = 0; j < h.njct; j++) {
-      for (int q = 0; q < 4; q++) {
-        const int lb = h.jct[12 * j + 3 * q];
-        if (lb < 0 || lb >= h.nb) return fail(TP_E_LAYOUT, "junction member outside its case");
-        if (c->blocks[h.block0 + lb].nz != c->blocks[h.block0 + h.jct[12 * j]].nz)
-          return fail(TP_E_LAYOUT, "junction members differ in zeta size");
-        jct.push_back(h.block0 + lb);
-        jct.push_back(h.jct[12 * j + 3 * q + 1]);
-        jct.push_back(h.jct[12 * j + 3 * q + 2]);
-        c->maxk = std::max(c->maxk, c->blocks[h.block0 + lb].nz);
-      }
-      jvel.push_back(h.jvel[2 * j]);
-      jvel.push_back(h.jvel[2 * j + 1]);
-      c->njct++;
-    }
-  if ((rc = upload_vec(c, &c->d_jct, jct))) return rc;
-  if ((rc = upload_vec(c, &c->d_jvel, jvel))) return rc;
+    if ((rc = build_case(c, h))) return rc;
+  if ((rc = link(c))) return rc;
   c->finalized = true;
   return 0;
 }
 
 int32_t tp_set_grid(tp_context *c, int32_t cs, int32_t lb, const double *xyz) {
-  if (int rc = check_ready(c)) return rc;
-  const int g = block_index(c, cs, lb);
-  if (g < 0 || !xyz) return fail(TP_E_ARG, "tp_set_grid: bad case/block");
-  const Block &b = c->blocks[g];
-  c->grids[g].assign(xyz, xyz + 3 * (size_t)b.N);
-  HIPCHK(hipMemcpyAsync(b.ss, xyz, sizeof(double) * 3 * b.N, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c) return fail(TP_E_ARG, "null context");
+  const int ci = case_index(c, cs);
+  if (ci < 0 || !xyz || lb < 0 || lb >= c->cases[ci].nb) return fail(TP_E_ARG, "tp_set_grid: bad case/block");
+  HostCase &h = c->cases[ci];
+  const size_t N = (size_t)(h.dims[3 * lb] + 1) * (h.dims[3 * lb + 1] + 1) * (h.dims[3 * lb + 2] + 1);
+  h.grid.resize(h.nb);
+  h.grid[lb].assign(xyz, xyz + 3 * N);
+  h.geom = false;
+  if (h.built) {
+    HIPCHK(hipMemcpyAsync(h.blk[lb].ss, xyz, sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   return 0;
 }
 
 int32_t tp_setup_geometry(tp_context *c) {
   if (int rc = check_ready(c)) return rc;
-  for (size_t g = 0; g < c->blocks.size(); g++)
-    if (c->grids[g].empty()) return fail(TP_E_STATE, "grid of block " + std::to_string(g) + " not set");
-  // coordinate derivatives, src/Main3D.f90:365-375 (periodic jump correction on)
-  const int dst[3] = {F_QO, F_QA, F_DE};
-  for (int m = 0; m < 3; m++) {
-    run_exchange(c, 0, 1, 1, F_SS, m);
-    run_line(c, 0, 2, F_SS, m, dst[m], 2, 0, 0);
-    run_line(c, 0, 1, F_SS, m, dst[m], 1, 0, 0);
-    run_line(c, 0, 0, F_SS, m, dst[m], 0, 0, 0);
+  // the built cases whose metrics are not set yet, and their global blocks
+  std::vector<int> todo, gs;
+  for (size_t i = 0; i < c->cases.size(); i++) {
+    const HostCase &h = c->cases[i];
+    if (!h.built || h.geom) continue;
+    for (int lb = 0; lb < h.nb; lb++)
+      if ((int)h.grid.size() <= lb || h.grid[lb].empty())
+        return fail(TP_E_STATE, "grid of case " + std::to_string(h.id) + " block " + std::to_string(lb) +
+                                    " not set");
+    todo.push_back((int)i);
+    for (int lb = 0; lb < h.nb; lb++) gs.push_back(h.block0 + lb);
   }
-  run_points(c, PK_METRICS, base_args(c), K_MISC, 0.0);
-  run_faces(c, c->normals, FK_NORMALS, base_args(c), K_MISC);
-  HIPCHK(hipStreamSynchronize(c->stream));
-  // wall-slot flags (SingleTurbulence3D.f90:447-464) and sponge coefficients
-  // (SingleTurbulence3D.f90:209-249) on the host from the grid and Jacobian
-  std::vector<Work> sp;
-  for (size_t g = 0; g < c->blocks.size(); g++) {
-    Block &b = c->blocks[g];
-    const HostCase &h = c->cases[b.cs];
-    const double *x = c->grids[g].data(), *y = x + b.N, *z = y + b.N;
-    std::vector<double> yaco(b.N);
-    HIPCHK(hipMemcpy(yaco.data(), b.yaco, sizeof(double) * b.N, hipMemcpyDeviceToHost));
-    // wall flags in the reference's face order
-    {
-      int64_t cnt;
-      array_ptr(b, TP_RPEX, &cnt);
-      std::vector<double> rp(std::max<int64_t>(cnt, 1), 0.0);
-      double state = h.cbca_ra0;
-      const double wlea = h.dp[TP_D_WLEA], wlew = h.dp[TP_D_WLEW], span = h.dp[TP_D_SPAN];
-      int lp = 0;
-      for (int d = 0; d < 3; d++)
-        for (int s = 0; s < 2; s++) {
-          const int np = b.nbc[d][s];
-          if (!(np == 10 || np == 20 || np == 25 || np == 30)) continue;
-          const int fd = d == 0 ? b.ny : d == 1 ? b.nz : b.nx, fo = b.F[d] / fd;
-          for (int kk = 0; kk < fo; kk++)
-            for (int jj = 0; jj < fd; jj++) {
-              if (!(np == 20 || np == 25)) continue;
-              const int l = indx3(b, s * (b.L[d] - 1), jj, kk, d);
-              double flag = 0;
-              if (x[l] < state) {
-                const double slitw = 0.4 * wlea;
-                state = -0.5 + wlea + wlea;
-                const double r1 = -0.5 * span + 0.25 * wlew - 0.5 * slitw, r2 = r1 + slitw;
-                const int ie = (int)((span + 1.0e-6) / wlew) - 1;
-                for (int ii = 0; ii <= ie; ii++) {
-                  const double r3 = ii * wlew, res = (z[l] - (r1 + r3)) * (z[l] - (r2 + r3));
-                  if (res < 0 && h.ip[TP_I_NEXTRABC] == 1) flag = 1;
-                }
-              }
-              rp[lp++] = flag;
-            }
-        }
-      HIPCHK(hipMemcpy(b.rpex, rp.data(), sizeof(double) * rp.size(), hipMemcpyHostToDevice));
+  if (todo.empty()) return 0;
+  // coordinate derivatives, src/Main3D.f90:365-375 (periodic jump correction
+  // on), metrics and face normals, over these cases' blocks only: the other
+  // cases' fields are left as they are
+  {
+    Mem tmp;
+    ListSet sub;
+    int rc = make_lists(c, gs, tmp, sub);
+    if (rc) { tmp.release(); return rc; }
+    std::swap(c->L, sub);
+    const int dst[3] = {F_QO, F_QA, F_DE};
+    for (int m = 0; m < 3; m++) {
+      run_exchange(c, 0, 1, 1, F_SS, m);
+      run_line(c, 0, 2, F_SS, m, dst[m], 2, 0, 0);
+      run_line(c, 0, 1, F_SS, m, dst[m], 1, 0, 0);
+      run_line(c, 0, 0, F_SS, m, dst[m], 0, 0, 0);
     }
-    // sponge
-    {
-      const double domlen = h.dp[TP_D_DOMLEN], szth1 = h.dp[TP_D_SZTH1], szth2 = h.dp[TP_D_SZTH2];
-      const double szxt = h.dp[TP_D_SZXT], szco = h.dp[TP_D_SZCO];
-      const double p07 = (double)0.7f;   // single-precision literal 0.7e0 of the reference
-      const double szr[2][3] = {{1 / szth1, 1 / szth2, 0}, {1 / (szth1 + szxt), 1 / szth2, 0}};
-      const double szp[2][3] = {{-p07 * domlen + szth1, -p07 * domlen + szth2, 0},
-                                {domlen - szth1, domlen + szxt - szth2, 0}};
-      const int lb = (int)g - h.block0;
-      int nsz[3][2];
-      for (int d = 0; d < 3; d++)
-        for (int s = 0; s < 2; s++) nsz[d][s] = h.nsz[6 * lb + 2 * d + s];
-      const double ra1 = 0.25, ra2 = kPi / (2 * domlen);
-      std::vector<int> lst;
-      std::vector<double> as, bs;
-      for (int l = 0; l < b.N; l++) {
-        const double cx[3] = {x[l], y[l], z[l]};
-        double r[3];
+    run_points(c, PK_METRICS, base_args(c), K_MISC, 0.0);
+    run_faces(c, c->L.normals, FK_NORMALS, base_args(c), K_MISC);
+    std::swap(c->L, sub);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    tmp.release();
+    if (e != hipSuccess) return fail(TP_E_HIP, std::string("metrics: ") + hipGetErrorString(e));
+  }
+  // wall-slot flags (SingleTurbulence3D.f90:447-464), sponge coefficients
+  // (SingleTurbulence3D.f90:209-249) and the inflow table, on the host from
+  // the grid and Jacobian
+  for (int ci : todo) {
+    HostCase &h = c->cases[ci];
+    for (int lb = 0; lb < h.nb; lb++) {
+      Block &b = h.blk[lb];
+      const double *x = h.grid[lb].data(), *y = x + b.N, *z = y + b.N;
+      std::vector<double> yaco(b.N);
+      HIPCHK(hipMemcpy(yaco.data(), b.yaco, sizeof(double) * b.N, hipMemcpyDeviceToHost));
+      // wall flags in the reference's face order
+      {
+        int64_t cnt;
+        array_ptr(b, TP_RPEX, &cnt);
+        std::vector<double> rp(std::max<int64_t>(cnt, 1), 0.0);
+        double state = h.cbca_ra0;
+        const double wlea = h.dp[TP_D_WLEA], wlew = h.dp[TP_D_WLEW], span = h.dp[TP_D_SPAN];
+        int lp = 0;
         for (int d = 0; d < 3; d++)
-          r[d] = nsz[d][0] * szr[0][d] * std::fmax(szp[0][d] - cx[d], 0.0) +
-                 nsz[d][1] * szr[1][d] * std::fmax(cx[d] - szp[1][d], 0.0);
-        const double sg = szco * (1 - 0.125 * (1 + std::cos(kPi * r[0])) * (1 + std::cos(kPi * r[1])) *
-                                          (1 + std::cos(kPi * r[2])));
-        const double lm = ra1 + 0.5 * (1 - ra1) * (1 + std::cos(ra2 * (std::fmin(x[l], domlen) + domlen)));
-        if (sg > 0) {
-          const double res = 1 / yaco[l];
-          lst.push_back(l);
-          as.push_back(res * sg * lm);
-          bs.push_back(res * (1 / (1.4 - 1)) * sg);
-        }
+          for (int s = 0; s < 2; s++) {
+            const int np = b.nbc[d][s];
+            if (!(np == 10 || np == 20 || np == 25 || np == 30)) continue;
+            const int fd = d == 0 ? b.ny : d == 1 ? b.nz : b.nx, fo = b.F[d] / fd;
+            for (int kk = 0; kk < fo; kk++)
+              for (int jj = 0; jj < fd; jj++) {
+                if (!(np == 20 || np == 25)) continue;
+                const int l = indx3(b, s * (b.L[d] - 1), jj, kk, d);
+                double flag = 0;
+                if (x[l] < state) {
+                  const double slitw = 0.4 * wlea;
+                  state = -0.5 + wlea + wlea;
+                  const double r1 = -0.5 * span + 0.25 * wlew - 0.5 * slitw, r2 = r1 + slitw;
+                  const int ie = (int)((span + 1.0e-6) / wlew) - 1;
+                  for (int ii = 0; ii <= ie; ii++) {
+                    const double r3 = ii * wlew, res = (z[l] - (r1 + r3)) * (z[l] - (r2 + r3));
+                    if (res < 0 && h.ip[TP_I_NEXTRABC] == 1) flag = 1;
+                  }
+                }
+                rp[lp++] = flag;
+              }
+          }
+        HIPCHK(hipMemcpy(b.rpex, rp.data(), sizeof(double) * rp.size(), hipMemcpyHostToDevice));
+      }
+      // sponge
+      std::vector<int> lst;
+      std::vector<double> sgv, lmv, as, bs;
+      sponge_select(h, lb, b, x, y, z, lst, sgv, lmv);
+      for (size_t q = 0; q < lst.size(); q++) {
+        const double res = 1 / yaco[lst[q]], sg = sgv[q], lm = lmv[q];
+        as.push_back(res * sg * lm);
+        bs.push_back(res * (1 / (1.4 - 1)) * sg);
       }
       int rc;
       if (!lst.empty()) {
-        if ((rc = upload_vec(c, &b.lcsz, lst))) return rc;
-        if ((rc = upload_vec(c, &b.asz, as))) return rc;
-        if ((rc = upload_vec(c, &b.bsz, bs))) return rc;
+        if ((rc = upload_vec(h.mem, &b.lcsz, lst))) return rc;
+        if ((rc = upload_vec(h.mem, &b.asz, as))) return rc;
+        if ((rc = upload_vec(h.mem, &b.bsz, bs))) return rc;
       }
       b.nsponge = (int)lst.size();
-      for (int a0 = 0; a0 < b.nsponge; a0 += 1024) sp.push_back({(int)g, a0, std::min(1024, b.nsponge - a0), 0});
-      // inflow points: sponge points upstream of the inflow sponge edge inside
-      // the band |y| < gustw, with the ramp sin^2 of SingleTurbulence3D.f90:253-268
       if (h.coef.inflow) {
-        const double gw = h.dp[TP_D_GUSTW], ra0 = kPi / szth1;
-        std::vector<int> gix(lst.size(), -1);
+        std::vector<int> gix, pts;
         std::vector<double> atz, pos;
-        std::vector<int> pts;
-        for (size_t q = 0; q < lst.size(); q++) {
-          const int l = lst[q];
-          if (x[l] < szp[0][0] && std::fabs(y[l]) < gw) {
-            gix[q] = (int)pts.size();
-            pts.push_back(l);
-            const double sn = std::sin(std::fmin(ra0 * (x[l] + domlen), kPi / 2));
-            atz.push_back(sn * sn);
-          }
-        }
+        inflow_select(h, x, y, lst, gix, pts, atz);
         b.ngust = (int)pts.size();
-        if ((rc = upload_vec(c, &b.gix, gix))) return rc;
+        if ((rc = upload_vec(h.mem, &b.gix, gix))) return rc;
         if (b.ngust > 0) {
           pos.resize(3 * (size_t)b.ngust);
           for (int q = 0; q < b.ngust; q++) {
             pos[q] = x[pts[q]]; pos[b.ngust + q] = y[pts[q]]; pos[2 * (size_t)b.ngust + q] = z[pts[q]];
           }
-          if ((rc = upload_vec(c, &b.atz, atz))) return rc;
-          const int nm = h.ip[TP_I_NMODES];
+          if ((rc = upload_vec(h.mem, &b.atz, atz))) return rc;
+          const int nm = inflow_nmodes(h);
           std::vector<double> modes(8 * (size_t)nm);
-          if ((rc = tp_inflow_modes(nm, (uint64_t)(uint32_t)h.ip[TP_I_SEED], h.dp[TP_D_TU], h.dp[TP_D_LTURB],
-                                    h.coef.uoo, h.dp[TP_D_SLIT], h.dp[TP_D_SPAN], modes.data())))
+          if (h.coef.inflow == 2) {
+            double slit;
+            gust_mode(h.dp[TP_D_TU], h.dp[TP_D_GUSTK], h.coef.uoo, modes.data(), &slit);
+          } else if ((rc = tp_inflow_modes(nm, (uint64_t)(uint32_t)h.ip[TP_I_SEED], h.dp[TP_D_TU],
+                                           h.dp[TP_D_LTURB], h.coef.uoo, h.dp[TP_D_SLIT], h.dp[TP_D_SPAN],
+                                           modes.data()))) {
             return rc;
+          }
+          if ((rc = dalloc(h.mem, (void **)&b.vito, sizeof(double) * 3 * (h.coef.ntz + 1) * (size_t)b.ngust)))
+            return rc;
+          Mem tmp;
           double *dmodes = nullptr, *dpos = nullptr;
-          if ((rc = upload_vec(c, &dmodes, modes))) return rc;
-          if ((rc = upload_vec(c, &dpos, pos))) return rc;
-          if ((rc = dalloc(c, (void **)&b.vito, sizeof(double) * 3 * (h.coef.ntz + 1) * (size_t)b.ngust)))
+          if ((rc = upload_vec(tmp, &dmodes, modes)) || (rc = upload_vec(tmp, &dpos, pos))) {
+            tmp.release();
             return rc;
+          }
           launch_inflow_table(dmodes, nm, dpos, b.ngust, h.coef.ntz, h.coef.fctr, h.coef.uoo, b.vito,
                               c->stream);
-          HIPCHK(hipStreamSynchronize(c->stream));
+          const hipError_t e = hipStreamSynchronize(c->stream);
+          tmp.release();
+          if (e != hipSuccess) return fail(TP_E_HIP, std::string("inflow table: ") + hipGetErrorString(e));
           HIPCHK(hipGetLastError());
         }
       }
     }
+    h.geom = true;
   }
-  HIPCHK(hipMemcpy(c->d_blocks, c->blocks.data(), sizeof(Block) * c->blocks.size(), hipMemcpyHostToDevice));
-  int rc;
-  if ((rc = upload_list(c, c->sponge, sp))) return rc;
-  return 0;
+  return link(c);
 }
 
 int32_t tp_initialo(tp_context *c, int32_t cs) {
   if (int rc = check_ready(c)) return rc;
-  if (cs < 0 || cs >= (int)c->cases.size()) return fail(TP_E_ARG, "bad case");
+  const HostCase *h = linked_case(c, cs);
+  if (!h) return fail(TP_E_ARG, "bad case");
   StageArgs a = base_args(c);
-  a.work = c->points.d + c->wr[cs];
-  const int n = c->wr[cs + 1] - c->wr[cs];
+  a.work = c->L.points.d + c->L.wr[h->slot];
+  const int n = c->L.wr[h->slot + 1] - c->L.wr[h->slot];
   if (n) launch_point(PK_INIT, n, c->stream, a);
   double s[S_NSCALAR] = {0};
-  s[S_AOA] = c->cases[cs].dp[TP_D_AOA0];
-  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_STRIDE, s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  s[S_AOA] = h->dp[TP_D_AOA0];
+  HIPCHK(hipMemcpyAsync(c->d_sc + h->slot * S_STRIDE, s, sizeof(s), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1301,16 +1554,18 @@ int32_t tp_download(tp_context *c, int32_t cs, int32_t lb, int32_t id, double *d
 
 int32_t tp_get_scalars(tp_context *c, int32_t cs, double *s16) {
   if (int rc = check_ready(c)) return rc;
-  if (cs < 0 || cs >= (int)c->cases.size() || !s16) return fail(TP_E_ARG, "bad case");
-  HIPCHK(hipMemcpyAsync(s16, c->d_sc + cs * S_STRIDE, sizeof(double) * S_NSCALAR, hipMemcpyDeviceToHost, c->stream));
+  const HostCase *h = linked_case(c, cs);
+  if (!h || !s16) return fail(TP_E_ARG, "bad case");
+  HIPCHK(hipMemcpyAsync(s16, c->d_sc + h->slot * S_STRIDE, sizeof(double) * S_NSCALAR, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int32_t tp_set_scalars(tp_context *c, int32_t cs, const double *s16) {
   if (int rc = check_ready(c)) return rc;
-  if (cs < 0 || cs >= (int)c->cases.size() || !s16) return fail(TP_E_ARG, "bad case");
-  HIPCHK(hipMemcpyAsync(c->d_sc + cs * S_STRIDE, s16, sizeof(double) * S_NSCALAR, hipMemcpyHostToDevice, c->stream));
+  const HostCase *h = linked_case(c, cs);
+  if (!h || !s16) return fail(TP_E_ARG, "bad case");
+  HIPCHK(hipMemcpyAsync(c->d_sc + h->slot * S_STRIDE, s16, sizeof(double) * S_NSCALAR, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1348,7 +1603,8 @@ int32_t tp_filter_step(tp_context *c) {
 
 int32_t tp_stage(tp_context *c, int32_t nk) {
   if (int rc = check_ready(c)) return rc;
-  if (nk < 1 || nk > c->cases[0].ip[TP_I_NKRK]) return fail(TP_E_ARG, "tp_stage: bad stage");
+  if (c->linked.empty()) return 0;
+  if (nk < 1 || nk > c->nkrk) return fail(TP_E_ARG, "tp_stage: bad stage");
   enqueue_stage(c, nk);
   HIPCHK(hipGetLastError());
   return 0;
@@ -1396,6 +1652,7 @@ int32_t tp_set_option(tp_context *c, int32_t option, int64_t value) {
     if (P != 0 && P != 1 && P != 2 && P != 4 && P != 8 && P != 16)
       return fail(TP_E_ARG, "chunks per line must be 0, 1, 2, 4, 8 or 16, not " + std::to_string(value));
     c->col_fixed = P;
+    for (int d = 0; d < 3; d++) c->tuned[d][0] = c->tuned[d][1] = -1;
     if (!c->finalized) return 0;
     for (int d = 0; d < 3; d++) {
       int q = P && col_need(c->blocks, d, P) ? P : col_default(c->blocks, d);
@@ -1442,6 +1699,7 @@ int32_t tp_autotune(tp_context *c, int32_t reps, int32_t *chosen, double *us) {
       }
       if (rc) break;
       select_col(c, d, kind, best);
+      c->tuned[d][kind] = best;
       if (chosen) chosen[d * 2 + kind] = c->colP[d][kind];
       if (us) us[d * 2 + kind] = bestms < 1e29f ? 1e3 * bestms / reps : 0.0;
     }
@@ -1492,10 +1750,84 @@ int32_t tp_profile_read(tp_context *c, int32_t k, double *ms, int64_t *launches,
 
 int32_t tp_info(tp_context *c, int64_t *out) {
   if (!c || !out) return fail(TP_E_ARG, "null argument");
+  size_t bytes = c->link.bytes;
+  for (const auto &h : c->cases) bytes += h.mem.bytes;
   out[0] = c->npts;
   out[1] = (int64_t)c->blocks.size();
-  out[2] = (int64_t)c->cases.size();
-  out[3] = (int64_t)c->dev_bytes;
+  out[2] = (int64_t)c->linked.size();
+  out[3] = (int64_t)bytes;
+  return 0;
+}
+
+int32_t tp_case_bytes(tp_context *c, int32_t cs, int64_t *bytes) {
+  if (!c || !bytes) return fail(TP_E_ARG, "null argument");
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  const HostCase &h = c->cases[ci];
+  for (int lb = 0; lb < h.nb; lb++)
+    if ((int)h.grid.size() <= lb || h.grid[lb].empty())
+      return fail(TP_E_STATE, "tp_case_bytes needs the grid of every block (tp_set_grid)");
+  *bytes = (int64_t)case_bytes(h);
+  return 0;
+}
+
+int32_t tp_admit(tp_context *c, int32_t cs) {
+  if (int rc = check_ready(c)) return rc;
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  if (c->cases[ci].built) return fail(TP_E_STATE, "case already admitted");
+  for (int lb = 0; lb < c->cases[ci].nb; lb++)
+    if ((int)c->cases[ci].grid.size() <= lb || c->cases[ci].grid[lb].empty())
+      return fail(TP_E_STATE, "tp_admit needs the grid of every block (tp_set_grid)");
+  int rc = build_case(c, c->cases[ci]);
+  if (!rc) rc = link(c);
+  if (!rc) rc = tp_setup_geometry(c);
+  if (rc) {
+    // leave the context as it was: the case stays pending
+    const std::string msg = g_err;
+    hipStreamSynchronize(c->stream);
+    HostCase &h = c->cases[ci];
+    h.mem.release();
+    h.built = false;
+    h.geom = false;
+    link(c);
+    g_err = msg;
+  }
+  return rc;
+}
+
+int32_t tp_remove_case(tp_context *c, int32_t cs) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const bool built = c->cases[ci].built;
+  c->cases[ci].mem.release();
+  c->cases.erase(c->cases.begin() + ci);
+  if (c->cases.empty()) c->nkrk = 0;
+  if (built && c->finalized) return link(c);
+  return 0;
+}
+
+int32_t tp_case_info(tp_context *c, int32_t cs, int64_t *out4) {
+  if (!c || !out4) return fail(TP_E_ARG, "null argument");
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  const HostCase &h = c->cases[ci];
+  int64_t np = 0;
+  for (int lb = 0; lb < h.nb; lb++)
+    np += (int64_t)(h.dims[3 * lb] + 1) * (h.dims[3 * lb + 1] + 1) * (h.dims[3 * lb + 2] + 1);
+  out4[0] = h.geom ? 2 : h.built ? 1 : 0;
+  out4[1] = np;
+  out4[2] = h.nb;
+  out4[3] = (int64_t)h.mem.bytes;
+  return 0;
+}
+
+int32_t tp_gust_mode(double tu, double kr, const double *uoo3, double *out8, double *slit) {
+  if (!uoo3 || !out8 || !slit || !(kr > 0) || !(tu >= 0)) return fail(TP_E_ARG, "tp_gust_mode: bad arguments");
+  if (!(uoo3[0] > 0)) return fail(TP_E_ARG, "tp_gust_mode: u_inf needs a positive x component");
+  gust_mode(tu, kr, uoo3, out8, slit);
   return 0;
 }
 

@@ -29,7 +29,8 @@ module tandem_hip
                         TP_D_SMAGO1 = 17, TP_D_SMAGO2 = 18, TP_D_DOMLEN = 19, TP_D_SZTH1 = 20, &
                         TP_D_SZTH2 = 21, TP_D_SZXT = 22, TP_D_SZCO = 23, TP_D_SPAN = 24, &
                         TP_D_WLEW = 25, TP_D_WLEA = 26, TP_D_TGUSTO = 27, TP_D_SLIT = 28, &
-                        TP_D_TU = 29, TP_D_LTURB = 30, TP_D_GUSTW = 31, TP_D_NPARAM = 32
+                        TP_D_TU = 29, TP_D_LTURB = 30, TP_D_GUSTW = 31, TP_D_GUSTK = 32, &
+                        TP_D_NPARAM = 32
   ! array ids of tp_upload / tp_download (names of src/MainVar3D.f90:78-102)
   integer(c_int32_t), parameter :: TP_QA = 0, TP_QO = 1, TP_DE = 2, TP_RR = 3, TP_SS = 4, &
                                    TP_XIM = 5, TP_ETM = 6, TP_ZEM = 7, TP_P = 8, TP_YACO = 9, &
@@ -204,6 +205,100 @@ module tandem_hip
       real(c_double), value :: tu, lturb, slit, span
       real(c_double), intent(in) :: uoo3(3)
       real(c_double), intent(out) :: out8(*)
+    end function
+    integer(c_int32_t) function tp_gust_mode(tu, kr, uoo3, out8, slit) bind(C, name="tp_gust_mode")
+      import :: c_int32_t, c_double
+      real(c_double), value :: tu, kr
+      real(c_double), intent(in) :: uoo3(3)
+      real(c_double), intent(out) :: out8(8), slit
+    end function
+    ! cases joining and leaving a running context
+    integer(c_int32_t) function tp_case_bytes(ctx, case_id, bytes) bind(C, name="tp_case_bytes")
+      import :: c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      integer(c_int64_t), intent(out) :: bytes
+    end function
+    integer(c_int32_t) function tp_admit(ctx, case_id) bind(C, name="tp_admit")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+    end function
+    integer(c_int32_t) function tp_remove_case(ctx, case_id) bind(C, name="tp_remove_case")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+    end function
+    integer(c_int32_t) function tp_case_info(ctx, case_id, out4) bind(C, name="tp_case_info")
+      import :: c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      integer(c_int64_t), intent(out) :: out4(4)
+    end function
+    ! request loop
+    integer(c_int32_t) function tp_server_create(budget_bytes, max_active, use_graph, srv) &
+        bind(C, name="tp_server_create")
+      import :: c_ptr, c_int32_t, c_int64_t
+      integer(c_int64_t), value :: budget_bytes
+      integer(c_int32_t), value :: max_active, use_graph
+      type(c_ptr), intent(out) :: srv
+    end function
+    integer(c_int32_t) function tp_server_destroy(srv) bind(C, name="tp_server_destroy")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: srv
+    end function
+    type(c_ptr) function tp_server_context(srv) bind(C, name="tp_server_context")
+      import :: c_ptr
+      type(c_ptr), value :: srv
+    end function
+    type(c_ptr) function tp_server_error(srv) bind(C, name="tp_server_error")
+      import :: c_ptr
+      type(c_ptr), value :: srv
+    end function
+    real(c_double) function tp_server_clock(srv) bind(C, name="tp_server_clock")
+      import :: c_ptr, c_double
+      type(c_ptr), value :: srv
+    end function
+    integer(c_int32_t) function tp_server_submit(srv, nblocks, dims, nbc, ncd, nsz, njct, jct, jvel, &
+                                                 iparams, dparams, xyz, nsteps, t_arrival, req) &
+        bind(C, name="tp_server_submit")
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: srv
+      integer(c_int32_t), value :: nblocks, njct, nsteps
+      integer(c_int32_t), intent(in) :: dims(*), nbc(*), ncd(*), nsz(*), jct(*), jvel(*), iparams(16)
+      real(c_double), intent(in) :: dparams(32)
+      type(c_ptr), intent(in) :: xyz(*)
+      real(c_double), value :: t_arrival
+      integer(c_int32_t), intent(out) :: req
+    end function
+    integer(c_int32_t) function tp_server_iterate(srv, max_steps, status6) bind(C, name="tp_server_iterate")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: srv
+      integer(c_int32_t), value :: max_steps
+      integer(c_int32_t), intent(out) :: status6(6)
+    end function
+    integer(c_int32_t) function tp_server_run(srv, max_steps) bind(C, name="tp_server_run")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: srv
+      integer(c_int32_t), value :: max_steps
+    end function
+    integer(c_int32_t) function tp_server_result(srv, req, qa, count, info6) bind(C, name="tp_server_result")
+      import :: c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: srv
+      integer(c_int32_t), value :: req
+      real(c_double), intent(out) :: qa(*)
+      integer(c_int64_t), value :: count
+      real(c_double), intent(out) :: info6(6)
+    end function
+    integer(c_int64_t) function tp_server_result_count(srv, req) bind(C, name="tp_server_result_count")
+      import :: c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: srv
+      integer(c_int32_t), value :: req
+    end function
+    integer(c_int32_t) function tp_server_stats(srv, out12) bind(C, name="tp_server_stats")
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: srv
+      real(c_double), intent(out) :: out12(12)
     end function
     pure integer(c_int64_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_ptr, c_int64_t

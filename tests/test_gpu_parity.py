@@ -317,3 +317,27 @@ def test_inflow_turbulence_steps(exact):
     moved = max(float(np.abs(s.download(0, b, "qa") - plain.download(0, b, "qa")).max()) for b in range(lay.nb))
     say("  max change of the state by the inflow", moved)
     assert moved > 1e-6
+
+
+def test_inflow_gust_steps():
+    """Sinusoidal transverse gust (TP_I_INFLOW = 2) at 5 degrees incidence:
+    the library derives the single mode and the table period itself; the
+    oracle is given the same mode (tp_gust_mode) and period, 40 steps."""
+    say("inflow gust, 40 steps, sequential")
+    lay, ss, p = aerofoil_case()
+    a = np.radians(5.0)
+    p = dict(p)
+    p.update(amach1=0.4 * np.cos(a), amach2=0.4 * np.sin(a), inflow=2, ntz=24, tgusto=0.0, tu=0.04,
+             gustk=1.2, gustw=1.5)
+    uoo = np.array([p["amach1"], p["amach2"], 0.0])
+    mode, slit = api.gust_mode(p["tu"], p["gustk"], uoo)
+    o = po.Oracle(lay, dict(p, slit=slit, nmodes=1))
+    o.set_inflow_modes(mode)
+    s = api.Solver([(lay, p)], exact_lines=True)
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b]); s.set_grid(0, b, ss[b])
+    o.metrics(); s.setup_geometry()
+    o.initialo(); s.initialo(0)
+    o.step(40); s.step(40)
+    s.sync()
+    compare_state(o, s, lay, 1e-10)

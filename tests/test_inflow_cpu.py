@@ -93,3 +93,56 @@ def test_table_periodic_over_one_period():
         np.testing.assert_allclose(v[:, 0], v[:, -1], rtol=0, atol=1e-12 * np.abs(v).max())
         assert np.all((o.array(b, "atz") >= 0) & (o.array(b, "atz") <= 1))
     assert seen > 0
+
+
+def test_gust_mode_along_the_free_stream():
+    """tp_gust_mode: wavevector 2 kr along u_inf, amplitude tu |u_inf| normal
+    to u_inf in the x-y plane, and a table x-period whose convection time
+    slit / u_x is the gust period pi c / (kr |u_inf|) (omega = 2 kr |u| / c)."""
+    tu, kr = 0.03, 2.5
+    for aoa in (0.0, 5.0, 8.0):
+        a = np.radians(aoa)
+        uoo = np.array([0.4 * np.cos(a), 0.4 * np.sin(a), 0.0]) * AOO
+        um = np.linalg.norm(uoo)
+        m, slit = api.gust_mode(tu, kr, uoo)
+        k, amp = m[0, :3], m[0, 3:6]
+        np.testing.assert_allclose(k, 2 * kr * uoo / um, rtol=0, atol=1e-14)
+        assert abs(amp @ uoo) <= 1e-16 and amp[2] == 0 and m[0, 6] == 0
+        assert np.linalg.norm(amp) == pytest.approx(tu * um, rel=1e-14)
+        assert slit / uoo[0] == pytest.approx(np.pi / (kr * um), rel=1e-14)
+    with pytest.raises(api.TPError):
+        api.gust_mode(tu, 0.0, np.array([0.3, 0.0, 0.0]))
+
+
+def test_gust_table_is_a_travelling_sinusoid():
+    """The oracle's inflow table for the gust mode: at every inflow point the
+    transverse velocity over one period is A cos(theta_p - 2 pi it / ntz)
+    with A = tu |u_inf| (the gust convects one wavelength per period), and
+    the other two components vanish."""
+    lay, ss, p = aerofoil_case()
+    tu, kr, ntz = 0.02, 1.5, 32
+    uoo = np.array([p["amach1"], 0.0, 0.0]) * AOO
+    m, slit = api.gust_mode(tu, kr, uoo)
+    p = inflow_params(p, inflow=2, nmodes=1, tu=tu, gustk=kr, ntz=ntz, slit=slit)
+    o = po.Oracle(lay, p)
+    o.set_inflow_modes(m)
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b])
+    o.metrics()
+    A = tu * np.linalg.norm(uoo)
+    ph = 2 * np.pi * np.arange(ntz) / ntz
+    seen = 0
+    for b in range(lay.nb):
+        v = o.array(b, "vito")
+        if v.size == 0:
+            continue
+        seen += 1
+        v = v.reshape(3, ntz + 1, -1)
+        assert np.abs(v[0]).max() <= 1e-15 * A and np.abs(v[2]).max() == 0
+        c = 2 / ntz * (np.cos(ph) @ v[1, :ntz])
+        s = 2 / ntz * (np.sin(ph) @ v[1, :ntz])
+        np.testing.assert_allclose(np.hypot(c, s), A, rtol=1e-12)
+        fit = np.cos(ph)[:, None] * c[None] + np.sin(ph)[:, None] * s[None]
+        assert np.abs(v[1, :ntz] - fit).max() <= 1e-12 * A
+        np.testing.assert_allclose(v[1, ntz], v[1, 0], rtol=0, atol=1e-13 * A)
+    assert seen > 0

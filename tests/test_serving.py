@@ -1,0 +1,180 @@
+"""Cases joining and leaving a running context, and the request loop over it
+(tp_case_bytes / tp_admit / tp_remove_case / tp_server_*).
+
+No counterpart in the reference (one case per MPI job).  The property under
+test is that a case computes what it computes alone while other cases are
+admitted and retired around it: bit-identical with the sequential line
+sweeps, within 1e-10 of the conserved-variable scale with the chunked sweeps
+(whose chunk count may differ between a case alone and the mix).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from caseutil import aerofoil_case
+from tandemproject_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def alone(lay, ss, p, nsteps, exact=True):
+    s = api.Solver([(lay, p)], exact_lines=exact)
+    for b in range(lay.nb):
+        s.set_grid(0, b, ss[b])
+    s.setup_geometry()
+    s.initialo(0)
+    s.step(nsteps)
+    s.sync()
+    out = [s.download(0, b, "qa") for b in range(lay.nb)], s.scalars(0)
+    s.close()
+    return out
+
+
+def state_diff(qs, rs):
+    """Max difference of the conserved variables over the case, each variable
+    relative to the largest of rho, |rho u| and E (rho a_inf scale)."""
+    qs = [q.reshape(5, -1) for q in qs]
+    rs = [r.reshape(5, -1) for r in rs]
+    scale = [max(np.abs(r[v]).max() for r in rs) for v in range(5)]
+    mom = max(max(scale[1:4]), scale[0])
+    scale = [scale[0], mom, mom, mom, scale[4]]
+    worst = 0.0
+    for q, r in zip(qs, rs):
+        assert np.isfinite(q).all()
+        for v in range(5):
+            worst = max(worst, float(np.abs(q[v] - r[v]).max() / scale[v]))
+    return worst
+
+
+CASES = [
+    aerofoil_case(lxibk=(16, 20, 16), letbk=(14, 14)),
+    aerofoil_case(lxibk=(20, 24, 18), letbk=(16, 12), amach1=0.3, reoo=4e4),
+    aerofoil_case(lxibk=(18, 22, 16), letbk=(12, 14), lzebk=(12,), amach1=0.35, inflow=2, ntz=16, tu=0.03,
+                  gustk=1.5, gustw=1.5),
+    aerofoil_case(lxibk=(24, 30, 20), letbk=(16, 16), amach1=0.45, inflow=1, nmodes=32, seed=7, ntz=16,
+                  slit=2.0, tu=0.04, lturb=0.3, gustw=1.5),
+]
+
+
+def test_admit_and_remove_around_a_running_case():
+    say("case B admitted after 5 steps of case A, A removed after 10; each equals its run alone")
+    (la, sa, pa), (lb, sb, pb) = CASES[0], CASES[1]
+    s = api.Solver([(la, pa)], exact_lines=True)
+    for b in range(la.nb):
+        s.set_grid(0, b, sa[b])
+    s.setup_geometry()
+    s.initialo(0)
+    s.step(5)
+    cb = s.add_case(lb, pb)
+    assert s.case_info(cb)["state"] == 0 and s.case_info(cb)["device_bytes"] == 0
+    for b in range(lb.nb):
+        s.set_grid(cb, b, sb[b])
+    need = s.case_bytes(cb)
+    s.admit(cb)
+    info = s.case_info(cb)
+    assert info["state"] == 2 and info["device_bytes"] == need, (info, need)
+    assert s.info()["cases"] == 2
+    s.initialo(cb)
+    s.step(5)
+    qa_a = [s.download(0, b, "qa") for b in range(la.nb)]
+    sc_a = s.scalars(0)
+    s.remove_case(0)
+    assert s.info()["cases"] == 1
+    with pytest.raises(api.TPError):
+        s.scalars(0)
+    s.step(3, graph=True)
+    s.step(2)
+    s.sync()
+    qa_b = [s.download(cb, b, "qa") for b in range(lb.nb)]
+    sc_b = s.scalars(cb)
+    ref_a, rsc_a = alone(la, sa, pa, 10)
+    ref_b, rsc_b = alone(lb, sb, pb, 10)
+    for q, r in zip(qa_a, ref_a):
+        np.testing.assert_array_equal(q, r)
+    for q, r in zip(qa_b, ref_b):
+        np.testing.assert_array_equal(q, r)
+    for k in ("n", "dt", "timo", "umf1"):
+        assert sc_a[k] == rsc_a[k] and sc_b[k] == rsc_b[k], k
+    assert sc_a["n"] == 10 and sc_b["n"] == 10
+
+
+@pytest.mark.parametrize("mode", ["sequential-graph", "sequential-eager", "chunked-graph"])
+def test_server_mix_equals_alone(mode):
+    """Four cases (two with inflow disturbances) through the request loop with
+    two slots: the later cases join while earlier ones run, and each result
+    equals the case run alone."""
+    exact = mode.startswith("sequential")
+    say("request loop, 4 cases, 2 slots:", mode)
+    srv = api.Server(budget_bytes=0, max_active=2, graph=mode.endswith("graph"))
+    if exact:
+        srv.set_option(0, 1)
+    nsteps = [12, 7, 9, 10]
+    reqs = [srv.submit(lay, p, ss, n, 0.0) for (lay, ss, p), n in zip(CASES, nsteps)]
+    srv.run(max_steps=5)
+    st = srv.stats()
+    say("  stats", {k: round(v, 4) for k, v in st.items()})
+    assert st["finished"] == 4 and st["failed"] == 0
+    assert st["peak_cases"] == 2 and st["joined_running"] >= 1
+    assert 0 < st["p50_s"] <= st["p99_s"] <= st["max_s"]
+    worst = 0.0
+    for r, (lay, ss, p), n in zip(reqs, CASES, nsteps):
+        info, qa = srv.result(r)
+        assert info["state"] == "done" and info["steps"] == n
+        assert info["arrival"] <= info["admitted"] <= info["done"]
+        ref, _ = alone(lay, ss, p, n, exact=exact)
+        if exact:
+            for q, rr in zip(qa, ref):
+                np.testing.assert_array_equal(q, rr)
+        else:
+            worst = max(worst, state_diff(qa, ref))
+    say("  max rel diff mix vs alone", worst)
+    assert worst <= 1e-10
+    srv.close()
+
+
+def test_server_admits_by_hbm_budget():
+    """A budget that holds the two small cases together but the large one
+    only alone: the large case runs first, the small ones after it, together;
+    a case over the budget is refused at submission."""
+    say("request loop admission by device bytes")
+    big, s1, s2 = CASES[3], CASES[0], CASES[1]
+    probe = api.Solver([])
+    need = []
+    for lay, ss, p in (big, s1, s2):
+        c = probe.add_case(lay, p)
+        for b in range(lay.nb):
+            probe.set_grid(c, b, ss[b])
+        need.append(probe.case_bytes(c))
+    probe.close()
+    assert need[0] > max(need[1], need[2])
+    budget = need[0] + min(need[1], need[2]) - 1
+    assert need[1] + need[2] <= budget
+    srv = api.Server(budget_bytes=budget, max_active=8, graph=True)
+    srv.set_option(0, 1)
+    r_big = srv.submit(big[0], big[2], big[1], 6, 0.0)
+    r1 = srv.submit(s1[0], s1[2], s1[1], 4, 0.0)
+    r2 = srv.submit(s2[0], s2[2], s2[1], 5, 0.0)
+    srv.run(max_steps=3)
+    st = srv.stats()
+    say("  budget", budget, "peak", st["peak_bytes"], "stats", {k: round(v, 4) for k, v in st.items()})
+    assert st["finished"] == 3 and st["peak_bytes"] <= budget and st["peak_cases"] == 2
+    ib, _ = srv.result(r_big, with_state=False)
+    i1, q1 = srv.result(r1)
+    i2, q2 = srv.result(r2)
+    assert i1["admitted"] >= ib["done"] and i2["admitted"] >= ib["done"]
+    assert i1["device_bytes"] == need[1] and ib["device_bytes"] == need[0]
+    for q, (lay, ss, p), n in ((q1, s1, 4), (q2, s2, 5)):
+        ref, _ = alone(lay, ss, p, n)
+        for a, b in zip(q, ref):
+            np.testing.assert_array_equal(a, b)
+    tiny = api.Server(budget_bytes=need[1] // 2, max_active=2)
+    with pytest.raises(api.TPError, match="over the budget"):
+        tiny.submit(s1[0], s1[2], s1[1], 3, 0.0)
+    tiny.close()
+    srv.close()
