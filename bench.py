@@ -51,6 +51,203 @@ def workload(scale=1.0):
     return lay, ss, params, dict(lxibk=lx, letbk=ly, lzebk=lz)
 
 
+def aerofoil_params(**over):
+    """Deck of a single-aerofoil case on the bench's H-grid domain (grid
+    x in [-7, 10], |y| <= 7; inflow sponge x < -5)."""
+    p = dict(reoo=4e5 / 0.3, tempoo=273.0, amach1=0.3, amach2=0.0, amach3=0.0, wtemp=1.0,
+             cfl=1.0, tmax=1e9, timf=15.0, fltk=0.85, fltkbc=0.85, dto=-1.0, domlen=10.0,
+             szth1=2.0, szth2=2.0, szxt=4.0, szco=12.0, span=0.2, wlew=0.11, wlea=0.015,
+             nkrk=4, nviscous=1, nsmf=0, les=0, nextrabc=0, nextgcic=0)
+    p.update(over)
+    return p
+
+
+def drawn_case(rng, npts_lo, npts_hi, nz_lo, nz_hi):
+    """Block layout and grid of a single-aerofoil case with about N points, N
+    log-uniform in [npts_lo, npts_hi], nz uniform in [nz_lo, nz_hi]; the
+    in-plane blocks keep the proportions of the 20 M-point shape
+    (lxibk = [a, 1.7a, a], letbk = [1.43a, 1.43a])."""
+    n = float(np.exp(rng.uniform(np.log(npts_lo), np.log(npts_hi))))
+    nz = int(rng.integers(nz_lo, nz_hi + 1))
+    a = max(16, int(round(np.sqrt(n / nz / 10.6))))
+    lx = [a, int(round(1.7 * a)), a]
+    ly = [int(round(1.43 * a))] * 2
+    lz = [nz - 1]
+    lay = L.h_layout(lx, ly, lz, nviscous=1)
+    blocks = G.h_grid_blocks(lx, ly, lz, xu=7.0, xd=10.0, yb=7.0, yt=7.0, span=0.2, thick=0.12,
+                             wall_spacing=4e-3)
+    return lay, G.subblock_grids(lay, blocks)
+
+
+def ensemble_cases(n, seed=4):
+    """Batched-ensemble cases: single-aerofoil gust response, 0.15-2 M points,
+    nz 13-32 (the spanwise-periodic closure spans 13 points, so the 8-12
+    points of the quasi-3D range are raised to 13), AoA U(0, 8 deg),
+    M U(0.2, 0.5), Re_c = 4e5, sinusoidal transverse gust with amplitude
+    U(1 %, 5 %) of |u_inf| and reduced frequency U(0.5, 8)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        lay, ss = drawn_case(rng, 1.5e5, 2.0e6, 13, 32)
+        aoa = np.radians(rng.uniform(0.0, 8.0))
+        mach = rng.uniform(0.2, 0.5)
+        p = aerofoil_params(reoo=4e5 / mach, amach1=mach * np.cos(aoa), amach2=mach * np.sin(aoa),
+                            inflow=2, ntz=32, tu=rng.uniform(0.01, 0.05), gustk=rng.uniform(0.5, 8.0),
+                            gustw=2.0)
+        out.append((lay, ss, p))
+    return out
+
+
+def served_cases(n, seed=5):
+    """Served turbulence-response cases: single aerofoil, 0.2-20 M points, nz
+    16-96, M = 0.3, Re_c = 4e5, turbulent inflow of 1024 seeded Fourier modes
+    with Tu U(1 %, 5 %) and integral length U(0.05, 0.2) chords."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        lay, ss = drawn_case(rng, 2.0e5, 2.0e7, 16, 96)
+        p = aerofoil_params(inflow=1, nmodes=1024, seed=100 + i, ntz=32, tgusto=0.0, slit=4.0,
+                            tu=rng.uniform(0.01, 0.05), lturb=rng.uniform(0.05, 0.2), gustw=2.0)
+        out.append((lay, ss, p))
+    return out
+
+
+def run_plain(lay, ss, p, nsteps):
+    """One case alone on the plain path: its own context, set-up, nsteps
+    eager steps, final qa to the host.  Returns (seconds, qa per block)."""
+    t0 = time.perf_counter()
+    s = api.Solver([(lay, p)])
+    for b in range(lay.nb):
+        s.set_grid(0, b, ss[b])
+    s.setup_geometry()
+    s.initialo(0)
+    s.step(nsteps)
+    qa = [s.download(0, b, "qa") for b in range(lay.nb)]
+    dt = time.perf_counter() - t0
+    s.close()
+    return dt, qa
+
+
+def state_diff(qs, rs):
+    """Max difference of the conserved variables, each relative to the
+    largest of rho, |rho u| and E over the case."""
+    qs = [q.reshape(5, -1) for q in qs]
+    rs = [r.reshape(5, -1) for r in rs]
+    scale = [max(np.abs(r[v]).max() for r in rs) for v in range(5)]
+    mom = max(max(scale[1:4]), scale[0])
+    scale = [scale[0], mom, mom, mom, scale[4]]
+    return max(float(np.abs(q[v] - r[v]).max() / scale[v]) for q, r in zip(qs, rs) for v in range(5))
+
+
+def ensemble_bench(ncase, nsteps, nplain):
+    """cases/h of the batched ensemble (every case in one request loop,
+    advanced in shared launches) against the plain path (one case at a time,
+    measured on a seeded sample of the same cases).  Both are timed from the
+    cases' grids on the host to their final states on the host; grid
+    generation is outside."""
+    cases = ensemble_cases(ncase)
+    pts = [lay.npoints() for lay, _, _ in cases]
+    log(f"ensemble: {ncase} cases, {sum(pts) / 1e6:.1f} M points, {nsteps} steps each")
+    srv = api.Server(budget_bytes=0, max_active=ncase, graph=True)
+    t0 = time.perf_counter()
+    reqs = [srv.submit(lay, p, ss, nsteps, 0.0) for lay, ss, p in cases]
+    srv.run(max_steps=nsteps)
+    wall_b = time.perf_counter() - t0
+    st = srv.stats()
+    step_s = st["stepping_s"] / nsteps
+    setup_b = wall_b - st["stepping_s"]
+    sample = np.random.default_rng(41).choice(ncase, size=min(nplain, ncase), replace=False)
+    t_plain, diff = [], 0.0
+    for i in sample:
+        lay, ss, p = cases[i]
+        dt, qa = run_plain(lay, ss, p, nsteps)
+        t_plain.append(dt)
+        _, qb = srv.result(reqs[i])
+        diff = max(diff, state_diff(qb, qa))
+    srv.close()
+    mean_plain = float(np.mean(t_plain))
+    # per-case plain time split into set-up and steps for the projection to
+    # the configured 2000 steps: the steps part scales with the step count
+    setup_p = []
+    for i in sample[:2]:
+        lay, ss, p = cases[i]
+        setup_p.append(run_plain(lay, ss, p, 0)[0])
+    setup_plain = float(np.mean(setup_p))
+    cph_b = 3600.0 * ncase / wall_b
+    cph_p = 3600.0 / mean_plain
+    proj_b = 3600.0 * ncase / (setup_b + 2000 * step_s)
+    proj_p = 3600.0 / (setup_plain + 2000 * max(mean_plain - setup_plain, 1e-9) / nsteps)
+    return {"cases": ncase, "steps_per_case": nsteps, "points_total": int(sum(pts)),
+            "points_range": [int(min(pts)), int(max(pts))],
+            "cases_per_hour": round(cph_b, 1), "plain_cases_per_hour": round(cph_p, 1),
+            "speedup_vs_plain": round(cph_b / cph_p, 2),
+            "wall_s": round(wall_b, 3), "setup_s": round(setup_b, 3), "ms_per_step": round(1e3 * step_s, 3),
+            "mpoint_steps_per_s": round(sum(pts) / step_s / 1e6, 1),
+            "plain_sample": int(len(sample)), "plain_mean_case_s": round(mean_plain, 4),
+            "max_rel_diff_vs_plain": diff,
+            "projected_2000_steps": {"cases_per_hour": round(proj_b, 1), "plain_cases_per_hour": round(proj_p, 1),
+                                     "speedup_vs_plain": round(proj_b / proj_p, 2)},
+            "workload": "single-aerofoil sinusoidal-gust cases (AoA 0-8 deg, M 0.2-0.5, amplitude 1-5 %, "
+                        "reduced frequency 0.5-8), 0.15-2 M points, nz 13-32, seed 4"}
+
+
+def serving_bench(nreq, nsteps, load, fill):
+    """Poisson stream of served cases through the request loop: p50/p99 time
+    to solution (arrival to final state on the host) and cases/h, against
+    the plain path serving the same stream one case at a time (first come,
+    first served, each case's plain time measured alone)."""
+    cases = served_cases(nreq)
+    pts = [lay.npoints() for lay, _, _ in cases]
+    log(f"serving: {nreq} requests, {sum(pts) / 1e6:.1f} M points, {nsteps} steps each")
+    # service time of each request alone on the plain path
+    t_plain, qa_plain = [], []
+    for lay, ss, p in cases:
+        dt, qa = run_plain(lay, ss, p, nsteps)
+        t_plain.append(dt)
+        qa_plain.append(qa)
+    # arrivals: Poisson at the rate that offers `load` times what the plain
+    # path can serve
+    rate = load * nreq / sum(t_plain)
+    arr = np.cumsum(np.random.default_rng(55).exponential(1.0 / rate, nreq))
+    arr -= arr[0]
+    srv = api.Server(budget_bytes=0, max_active=64, graph=True, fill_points=fill)
+    t0 = srv.clock() + 0.05
+    reqs = [srv.submit(lay, p, ss, nsteps, t0 + a) for (lay, ss, p), a in zip(cases, arr)]
+    srv.run(max_steps=10)
+    st = srv.stats()
+    diff = 0.0
+    for r, qa in zip(reqs, qa_plain):
+        _, qb = srv.result(r)
+        diff = max(diff, state_diff(qb, qa))
+    srv.close()
+    # the plain path serving the same arrivals, one at a time in arrival order
+    done, tts = 0.0, []
+    for a, t in zip(arr, t_plain):
+        done = max(done, a) + t
+        tts.append(done - a)
+    wall_p = done - arr[0]
+    tts = np.sort(np.array(tts))
+    pct = lambda v, q: float(v[max(int(np.ceil(q * len(v))), 1) - 1])
+    return {"requests": nreq, "steps_per_case": nsteps, "arrival_rate_per_s": round(rate, 3),
+            "points_range": [int(min(pts)), int(max(pts))],
+            "p50_s": round(st["p50_s"], 3), "p99_s": round(st["p99_s"], 3), "mean_s": round(st["mean_s"], 3),
+            "cases_per_hour": round(st["cases_per_hour"], 1),
+            "plain_p50_s": round(pct(tts, 0.5), 3), "plain_p99_s": round(pct(tts, 0.99), 3),
+            "plain_cases_per_hour": round(3600.0 * nreq / wall_p, 1),
+            "speedup_vs_plain": round(st["cases_per_hour"] / (3600.0 * nreq / wall_p), 2),
+            "peak_cases_on_device": int(st["peak_cases"]), "peak_device_gb": round(st["peak_bytes"] / 1e9, 2),
+            "joined_running": int(st["joined_running"]), "max_rel_diff_vs_plain": diff,
+            "fill_points": int(fill), "admit_s": round(st["admit_s"], 3), "retire_s": round(st["retire_s"], 3),
+            "stepping_s": round(st["stepping_s"], 3),
+            "workload": "single-aerofoil turbulence-response cases (1024 Fourier modes, Tu 1-5 %, "
+                        "L 0.05-0.2c), 0.2-20 M points, nz 16-96, seed 5; Poisson arrivals offering "
+                        f"{load:g}x the plain path's capacity"}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(seconds_target=15.0):
     """The oracle (C restatement, OpenMP) on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -122,6 +319,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel events)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default line-solver launch shapes")
+    ap.add_argument("--ensemble-cases", type=int, default=64, help="0: skip the ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=40)
+    ap.add_argument("--ensemble-plain", type=int, default=8, help="cases timed on the plain path")
+    ap.add_argument("--serve-requests", type=int, default=24, help="0: skip the serving measurement")
+    ap.add_argument("--serve-steps", type=int, default=40)
+    ap.add_argument("--serve-load", type=float, default=0.9,
+                    help="offered load as a multiple of the plain path's capacity")
+    ap.add_argument("--serve-fill", type=float, default=24e6,
+                    help="admission stops while the cases on the device hold this many points")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="after the run, one plain device-to-device copy of MB MiB (a reference kernel "
                          "of known traffic for scaling hardware byte counters)")
@@ -161,6 +367,7 @@ def main():
     prof = {k: s.profile_read(k) for k in api.KCLASS}
     sc = s.scalars(0)
     q = s.download(0, 0, "qa")
+    line_chunks = {f"{'xez'[d]}-{k}": c for (d, k), c in s.col_config().items()}
     finite = bool(np.isfinite(q).all())
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s; n={sc['n']:.0f} dt={sc['dt']:.3e} finite={finite}",
           file=sys.stderr, flush=True)
@@ -182,6 +389,17 @@ def main():
     breakdown = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1]}
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+    s.close()
+    ens = ensemble_bench(args.ensemble_cases, args.ensemble_steps, args.ensemble_plain) \
+        if args.ensemble_cases > 0 else None
+    if ens:
+        log(f"ensemble: {ens['cases_per_hour']} cases/h vs plain {ens['plain_cases_per_hour']} "
+            f"({ens['speedup_vs_plain']}x)")
+    srv = serving_bench(args.serve_requests, args.serve_steps, args.serve_load, args.serve_fill) \
+        if args.serve_requests > 0 else None
+    if srv:
+        log(f"serving: p50 {srv['p50_s']} s p99 {srv['p99_s']} s, {srv['cases_per_hour']} cases/h "
+            f"vs plain {srv['plain_cases_per_hour']} ({srv['speedup_vs_plain']}x)")
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpoint-steps/s", "n_gpus": 1,
@@ -194,9 +412,9 @@ def main():
                                f"{npts / 1e6:.2f} M points, M=0.3, Re_c=4e5, viscous, 4-stage RK, "
                                "turbulent inflow Tu=2.5%, L=0.1c, 4096 modes",
                    "points": npts, "sub_blocks": lay.nb, "graph": bool(args.graph),
-                   "line_chunks": {f"{'xez'[d]}-{k}": c for (d, k), c in s.col_config().items()},
+                   "line_chunks": line_chunks,
                    "ms_per_step_by_class": breakdown},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "ensemble": ens, "serving": srv,
     }
     print(json.dumps(out))
 

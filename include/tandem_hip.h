@@ -152,22 +152,27 @@ int32_t tp_sync(tp_context *ctx);
  *   the sequential sweep up to rounding of the chunk carries (~1e-16
  *   relative).  May be changed between steps.
  * TP_OPT_CHUNKS = P (1, 2, 4, 8, 16) fixes the chunks per line of the
- *   chunked sweeps for every direction where chunks of ceil(L/P) <= 32
- *   points fit; 0 (default) lets the library choose per direction.  With a
- *   fixed P a case computes bit-identical results alone or batched with
- *   others whose lines need the same P. */
+ *   chunked sweeps for every block and direction whose lines L allow chunks
+ *   of ceil(L/P) <= 32 points; 0 (default) lets the library choose per block
+ *   and direction.  The chunk count of a block depends only on its own line
+ *   lengths (and on the autotuner's choice for its line-length class), so a
+ *   case computes bit-identical results alone or batched with others when
+ *   the chunk counts are fixed or left at their defaults. */
 enum { TP_OPT_EXACT_LINES = 0, TP_OPT_CHUNKS = 1 };
 int32_t tp_set_option(tp_context *ctx, int32_t option, int64_t value);
 
 /* Time every usable line-solver configuration (sequential kernel and P =
  * 1..16 chunks) on the context's own blocks, reps launches each, and keep
- * the fastest per direction and kind.  Overwrites the scratch array rr, so
- * call it between steps.  chosen[d*2+kind] (kind 0 deriv, 1 filter) receives
- * the chunk count (0: sequential kernel), us[d*2+kind] the time per launch;
- * either may be NULL.  No counterpart in the reference: it replaces the
- * fixed 1-D loop structure of deriv/filte with a measured launch shape. */
+ * the fastest per direction, kind and line-length class (blocks whose lines
+ * have the same default chunk count form a class; the blocks of one chunk
+ * count share a launch).  Overwrites the scratch array rr, so call it
+ * between steps.  chosen[d*2+kind] (kind 0 deriv, 1 filter) receives the
+ * chunk count used for the most points (0: sequential kernel), us[d*2+kind]
+ * the time of the direction's launches; either may be NULL.  No counterpart
+ * in the reference: it replaces the fixed 1-D loop structure of deriv/filte
+ * with a measured launch shape. */
 int32_t tp_autotune(tp_context *ctx, int32_t reps, int32_t *chosen, double *us);
-/* chunks per line in use, [d*2+kind] (0: sequential kernel) */
+/* chunks per line used for the most points, [d*2+kind] (0: sequential kernel) */
 int32_t tp_col_config(tp_context *ctx, int32_t *chunks6);
 
 /* ---- turbulent inflow ----
@@ -215,10 +220,12 @@ int32_t tp_case_info(tp_context *ctx, int32_t case_id, int64_t *out4);
 /* ---- request loop (no counterpart in the reference) ----
  * A server owns a context.  Requests -- whole cases with their grids and a
  * step count -- queue on the host with an arrival time.  Each iteration
- *   1. admits arrived requests in arrival order while a slot is free and
- *      the case's device bytes (tp_case_bytes) fit the HBM budget (a request
- *      that does not fit is passed over for later, smaller ones), sets them
- *      up on the device and gives them the uniform initial state;
+ *   1. admits arrived requests in arrival order while a slot is free, the
+ *      cases on the device hold fewer points than the fill target
+ *      (tp_server_set_fill) and the case's device bytes (tp_case_bytes) fit
+ *      the HBM budget (a request that does not fit the budget is passed over
+ *      for later, smaller ones), sets them up on the device and gives them
+ *      the uniform initial state;
  *   2. advances every case on the device by the same number of steps (at
  *      most max_steps, at most what any of them still needs) in shared
  *      launches (captured step graph when use_graph);
@@ -232,6 +239,11 @@ typedef struct tp_server tp_server;
 int32_t tp_server_create(int64_t budget_bytes, int32_t max_active, int32_t use_graph, tp_server **srv);
 int32_t tp_server_destroy(tp_server *srv);
 tp_context *tp_server_context(tp_server *srv);   /* for tp_set_option / tp_autotune */
+/* admit no further request while the cases on the device hold at least
+ * `points` grid points (one case is always admitted; 0: no limit, the
+ * default).  Enough points to fill the GPU keep its throughput while large
+ * cases finish one after another instead of sharing it. */
+int32_t tp_server_set_fill(tp_server *srv, int64_t points);
 const char *tp_server_error(tp_server *srv);
 double tp_server_clock(tp_server *srv);
 /* queue a case (arguments of tp_add_case, then xyz[b] = grid of sub-block b);
@@ -250,11 +262,12 @@ int32_t tp_server_run(tp_server *srv, int32_t max_steps);
  * the final qa of every sub-block in order, count = tp_server_result_count */
 int32_t tp_server_result(tp_server *srv, int32_t req, double *qa, int64_t count, double *info6);
 int64_t tp_server_result_count(tp_server *srv, int32_t req);
-/* out12 = finished, failed, p50, p99, mean and max time to solution (arrival
+/* out16 = finished, failed, p50, p99, mean and max time to solution (arrival
  * to final state on the host), wall time from the first arrival to the last
  * finish, cases per hour over it, peak admitted bytes, peak cases on the
- * device, admissions that joined running cases, time spent stepping */
-int32_t tp_server_stats(tp_server *srv, double *out12);
+ * device, admissions that joined running cases, time spent stepping, in
+ * admissions, in retirements, in step-graph captures, number of captures */
+int32_t tp_server_stats(tp_server *srv, double *out16);
 
 /* ---- measurement ---- */
 /* record HIP events around every launch (on the context's stream) */

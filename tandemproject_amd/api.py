@@ -34,7 +34,7 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_remove_case", "tp_case_info", "tp_server_create", "tp_server_destroy",
            "tp_server_context", "tp_server_error", "tp_server_clock", "tp_server_submit",
            "tp_server_iterate", "tp_server_run", "tp_server_result", "tp_server_result_count",
-           "tp_server_stats"]
+           "tp_server_stats", "tp_server_set_fill"]
 
 _lib = None
 
@@ -83,7 +83,7 @@ def load(path=LIB_PATH):
                              ctypes.POINTER(pd), i32, dbl, pi32],
         "tp_server_iterate": [P, i32, pi32], "tp_server_run": [P, i32],
         "tp_server_result": [P, i32, pd, i64, pd],
-        "tp_server_stats": [P, pd],
+        "tp_server_stats": [P, pd], "tp_server_set_fill": [P, i64],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -322,7 +322,7 @@ class Server:
 
     STATES = ("queued", "running", "done", "failed")
 
-    def __init__(self, budget_bytes=0, max_active=64, graph=True):
+    def __init__(self, budget_bytes=0, max_active=64, graph=True, fill_points=0):
         L = load()
         self.h = ctypes.c_void_p()
         rc = L.tp_server_create(int(budget_bytes), int(max_active), 1 if graph else 0, ctypes.byref(self.h))
@@ -330,6 +330,8 @@ class Server:
             raise TPError(L.tp_last_error().decode())
         self.ctx = ctypes.c_void_p(L.tp_server_context(self.h))
         self.layouts = {}
+        if fill_points:
+            self._check(L.tp_server_set_fill(self.h, int(fill_points)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -394,8 +396,9 @@ class Server:
         return d, parts
 
     def stats(self):
-        o = np.zeros(12)
+        o = np.zeros(16)
         self._check(load().tp_server_stats(self.h, _dp(o)))
         keys = ("finished", "failed", "p50_s", "p99_s", "mean_s", "max_s", "wall_s", "cases_per_hour",
-                "peak_bytes", "peak_cases", "joined_running", "stepping_s")
+                "peak_bytes", "peak_cases", "joined_running", "stepping_s", "admit_s", "retire_s",
+                "capture_s", "captures")
         return dict(zip(keys, (float(x) for x in o)))

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tandem_hip.h"
@@ -320,24 +321,30 @@ std::vector<double> col_tables(int L, int P, const double *l1, const double *l2,
 constexpr int kColP[] = {1, 2, 4, 8, 16};
 constexpr int kNColP = 5;
 
-// capacity needed by P chunks over the lines of direction d (0: P unusable)
-int col_need(const std::vector<Block> &blocks, int d, int P) {
-  int cmax = 0;
-  for (const Block &b : blocks) cmax = std::max(cmax, (b.L[d] + P - 1) / P);
-  return col_capacity(cmax);
+// chunk length of P chunks over L points, 0 if the chunked solver cannot run it
+int col_chunk(int L, int P) {
+  const int Cl = (L + P - 1) / P;
+  return col_capacity(Cl) ? Cl : 0;
 }
 
-// default chunk count: the fewest chunks whose length stays within 24 points
-// (registers for 4+ waves per SIMD), else within 32
-int col_default(const std::vector<Block> &blocks, int d) {
+// default chunk count of lines of L points: the fewest chunks whose length
+// stays within 24 points (registers for 4+ waves per SIMD), else within 32;
+// 0 (the sequential kernel) if none fits
+int col_default(int L) {
   for (int lim : {24, 32})
-    for (int P : kColP) {
-      int cmax = 0;
-      for (const Block &b : blocks) cmax = std::max(cmax, (b.L[d] + P - 1) / P);
-      if (cmax <= lim) return P;
-    }
+    for (int P : kColP)
+      if ((L + P - 1) / P <= lim) return P;
   return 0;
 }
+
+// index of a chunk count in {0 (sequential), kColP...}: the line-length class
+// of blocks whose default is that count
+int col_class(int P) {
+  for (int i = 0; i < kNColP; i++)
+    if (kColP[i] == P) return i + 1;
+  return 0;
+}
+constexpr int kNClass = kNColP + 1;
 
 // GCBC spreading weights over the MBCI points inside a characteristic face
 double gcbc_weights(double sbci[MBCI]) {
@@ -405,8 +412,17 @@ struct DevList {
 };
 
 // launch tables over a set of blocks
+// one launch of the line solves of a direction and kind: the blocks whose
+// lines use P chunks (P = 0: the sequential kernel), chunk capacity C
+struct ColGroup {
+  int P = 0, C = 0;
+  long long npts = 0;
+  DevList work;
+};
+
 struct ListSet {
-  DevList points, sponge, lines[3], colwork[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  std::vector<ColGroup> cg[3][2];   // line-solve launches per direction and kind
   std::vector<int> wr;   // first point-work item of each linked case, then the end
 };
 
@@ -436,8 +452,9 @@ struct tp_context {
   size_t lds[3] = {0, 0, 0};
   bool exact_lines = false;        // sequential one-lane-per-line sweeps
   int col_fixed = 0;               // requested chunks per line (0: chosen per direction)
-  int tuned[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};   // autotuned chunks per line (-1: none)
-  int colP[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // chunks per line, direction x kind (0: sequential kernel)
+  // autotuned chunks per line for each direction, kind and line-length class (-1: none)
+  int tuned[3][2][kNClass];
+  Mem lmem;                        // line-solve launch tables (rebuilt when the chunk choice changes)
   long long npts = 0;
   bool any_visc = false;
   // measurement
@@ -449,6 +466,11 @@ struct tp_context {
   // graph of one step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  tp_context() {
+    for (auto &a : tuned)
+      for (auto &b : a)
+        for (int &v : b) v = -1;
+  }
 };
 
 namespace {
@@ -553,14 +575,17 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   a.blocks = c->d_blocks; a.work = c->L.lines[d].d; a.coef = c->d_coef;
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
   a.m = m; a.acc = acc; a.nlp = c->nlp[d];
-  const int P = c->colP[d][kind];
-  if (!c->exact_lines && P) {
-    a.work = c->L.colwork[d].d;
-    launch_col(d, kind, col_need(c->blocks, d, P), P, c->L.colwork[d].n, c->stream, a);
-  } else if (kind == 0) {
-    launch_deriv(d, c->L.lines[d].n, c->lds[d], c->stream, a);
-  } else {
-    launch_filter(d, c->L.lines[d].n, c->lds[d], c->stream, a);
+  if (c->exact_lines) {
+    if (kind == 0) launch_deriv(d, c->L.lines[d].n, c->lds[d], c->stream, a);
+    else launch_filter(d, c->L.lines[d].n, c->lds[d], c->stream, a);
+    return;
+  }
+  for (const ColGroup &g : c->L.cg[d][kind]) {
+    if (!g.work.n) continue;
+    a.work = g.work.d;
+    if (g.P) launch_col(d, kind, g.C, g.P, g.work.n, c->stream, a);
+    else if (kind == 0) launch_deriv(d, g.work.n, c->lds[d], c->stream, a);
+    else launch_filter(d, g.work.n, c->lds[d], c->stream, a);
   }
 }
 
@@ -570,28 +595,78 @@ void drop_graph(tp_context *c) {
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
 }
 
-// use P chunks per line for the line solves of direction d, kind (0: the
-// sequential one-lane-per-line kernel); host copies of the blocks only
-void select_col(tp_context *c, int d, int kind, int P) {
-  int pi = -1;
-  for (int i = 0; i < kNColP; i++)
-    if (kColP[i] == P) pi = i;
-  if (pi < 0 || !col_need(c->blocks, d, P)) {
-    c->colP[d][kind] = 0;
-    return;
-  }
-  c->colP[d][kind] = P;
+// chunks per line of block g's line solves in direction d, kind: the tuned
+// count of its line-length class if that fits its lines, else the requested
+// count if it fits, else the default (0: the sequential kernel)
+int block_chunks(const tp_context *c, int g, int d, int kind) {
+  const int L = c->blocks[g].L[d];
+  const int def = col_default(L);
+  auto fits = [&](int P) { return P == 0 || col_chunk(L, P) != 0; };
+  const int t = c->tuned[d][kind][col_class(def)];
+  if (t >= 0 && fits(t)) return t;
+  if (c->col_fixed && fits(c->col_fixed)) return c->col_fixed;
+  return def;
+}
+
+// point every block's chunked-solver table pointers at its chosen chunk
+// count (host copies of the blocks only)
+void choose_lines(tp_context *c) {
   for (size_t g = 0; g < c->blocks.size(); g++) {
     const auto &src = c->bsrc[g];
-    c->blocks[g].col[d][kind] = c->cases[src.first].coltab[src.second][(d * 2 + kind) * kNColP + pi];
+    for (int d = 0; d < 3; d++)
+      for (int kind = 0; kind < 2; kind++) {
+        const int ci = col_class(block_chunks(c, (int)g, d, kind)) - 1;
+        c->blocks[g].col[d][kind] =
+            ci < 0 ? nullptr : c->cases[src.first].coltab[src.second][(d * 2 + kind) * kNColP + ci];
+      }
   }
 }
 
-// copy the host block views to the device after a selection change
-int push_blocks(tp_context *c) {
+// line-solve launches over the global blocks gs: one per distinct chunk
+// count of each direction and kind
+int make_groups(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
+  for (int d = 0; d < 3; d++)
+    for (int kind = 0; kind < 2; kind++) {
+      std::vector<ColGroup> groups;
+      std::vector<std::vector<Work>> works;
+      std::vector<int> cmax;
+      for (int g : gs) {
+        const Block &b = c->blocks[g];
+        const int P = block_chunks(c, g, d, kind);
+        size_t k = 0;
+        while (k < groups.size() && groups[k].P != P) k++;
+        if (k == groups.size()) {
+          groups.emplace_back();
+          groups.back().P = P;
+          works.emplace_back();
+          cmax.push_back(0);
+        }
+        const int nlines = b.N / b.L[d];
+        const int step = P ? 64 : c->nlp[d] - 1;
+        for (int q = 0; q < nlines; q += step) works[k].push_back({g, q, std::min(step, nlines - q), d});
+        if (P) cmax[k] = std::max(cmax[k], (b.L[d] + P - 1) / P);
+        groups[k].npts += b.N;
+      }
+      for (size_t k = 0; k < groups.size(); k++) {
+        groups[k].C = groups[k].P ? col_capacity(cmax[k]) : 0;
+        if (int rc = upload_list(m, groups[k].work, works[k])) return rc;
+      }
+      L.cg[d][kind] = std::move(groups);
+    }
+  return 0;
+}
+
+// rebuild the line-solve launches of the linked blocks after the chunk
+// choice changed, and refresh the device block views
+int regroup(tp_context *c) {
   drop_graph(c);
-  if (!c->d_blocks) return 0;
   HIPCHK(hipStreamSynchronize(c->stream));
+  choose_lines(c);
+  c->lmem.release();
+  std::vector<int> gs(c->blocks.size());
+  for (size_t g = 0; g < gs.size(); g++) gs[g] = (int)g;
+  if (int rc = make_groups(c, gs, c->lmem, c->L)) return rc;
+  if (!c->d_blocks) return 0;
   HIPCHK(hipMemcpy(c->d_blocks, c->blocks.data(), sizeof(Block) * c->blocks.size(), hipMemcpyHostToDevice));
   return 0;
 }
@@ -792,15 +867,30 @@ Block block_shape(const HostCase &h, int lb, int *nwall) {
   return b;
 }
 
-// chunk length of P chunks over L points, 0 if the chunked solver cannot run it
-int col_chunk(int L, int P) {
-  const int Cl = (L + P - 1) / P;
-  return col_capacity(Cl) ? Cl : 0;
-}
-
 // sponge points of block lb from its coordinates, SingleTurbulence3D.f90:209-249:
 // the point list with the strength sg and the streamwise factor lm of each
 // (the coefficients are (1/yaco) sg lm and (1/yaco) sg/(gam-1))
+// run fn(lo, hi, t) over [0, n) in contiguous ranges on nt host threads
+// (range t covers [t n/nt, (t+1) n/nt))
+template <class F>
+void host_parallel(int n, int nt, F fn) {
+  if (nt <= 1 || n < 65536) {
+    fn(0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) {
+    const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+    th.emplace_back([=]() { fn(lo, hi, t); });
+  }
+  for (auto &t : th) t.join();
+}
+
+int host_threads() {
+  const unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc, 16u));
+}
+
 void sponge_select(const HostCase &h, int lb, const Block &b, const double *x, const double *y,
                    const double *z, std::vector<int> &lst, std::vector<double> &sgv, std::vector<double> &lmv) {
   const double domlen = h.dp[TP_D_DOMLEN], szth1 = h.dp[TP_D_SZTH1], szth2 = h.dp[TP_D_SZTH2];
@@ -813,21 +903,31 @@ void sponge_select(const HostCase &h, int lb, const Block &b, const double *x, c
   for (int d = 0; d < 3; d++)
     for (int s = 0; s < 2; s++) nsz[d][s] = h.nsz[6 * lb + 2 * d + s];
   const double ra1 = 0.25, ra2 = kPi / (2 * domlen);
-  lst.clear(); sgv.clear(); lmv.clear();
-  for (int l = 0; l < b.N; l++) {
-    const double cx[3] = {x[l], y[l], z[l]};
-    double r[3];
-    for (int d = 0; d < 3; d++)
-      r[d] = nsz[d][0] * szr[0][d] * std::fmax(szp[0][d] - cx[d], 0.0) +
-             nsz[d][1] * szr[1][d] * std::fmax(cx[d] - szp[1][d], 0.0);
-    const double sg = szco * (1 - 0.125 * (1 + std::cos(kPi * r[0])) * (1 + std::cos(kPi * r[1])) *
-                                      (1 + std::cos(kPi * r[2])));
-    const double lm = ra1 + 0.5 * (1 - ra1) * (1 + std::cos(ra2 * (std::fmin(x[l], domlen) + domlen)));
-    if (sg > 0) {
-      lst.push_back(l);
-      sgv.push_back(sg);
-      lmv.push_back(lm);
+  const int nt = host_threads();
+  std::vector<std::vector<int>> tl(nt);
+  std::vector<std::vector<double>> ts(nt), tm(nt);
+  host_parallel(b.N, nt, [&](int lo, int hi, int t) {
+    for (int l = lo; l < hi; l++) {
+      const double cx[3] = {x[l], y[l], z[l]};
+      double r[3];
+      for (int d = 0; d < 3; d++)
+        r[d] = nsz[d][0] * szr[0][d] * std::fmax(szp[0][d] - cx[d], 0.0) +
+               nsz[d][1] * szr[1][d] * std::fmax(cx[d] - szp[1][d], 0.0);
+      const double sg = szco * (1 - 0.125 * (1 + std::cos(kPi * r[0])) * (1 + std::cos(kPi * r[1])) *
+                                        (1 + std::cos(kPi * r[2])));
+      if (sg > 0) {
+        const double lm = ra1 + 0.5 * (1 - ra1) * (1 + std::cos(ra2 * (std::fmin(x[l], domlen) + domlen)));
+        tl[t].push_back(l);
+        ts[t].push_back(sg);
+        tm[t].push_back(lm);
+      }
     }
+  });
+  lst.clear(); sgv.clear(); lmv.clear();
+  for (int t = 0; t < nt; t++) {
+    lst.insert(lst.end(), tl[t].begin(), tl[t].end());
+    sgv.insert(sgv.end(), ts[t].begin(), ts[t].end());
+    lmv.insert(lmv.end(), tm[t].begin(), tm[t].end());
   }
 }
 
@@ -943,7 +1043,7 @@ int build_case(tp_context *c, HostCase &h) {
 
 // launch tables over the global blocks gs (sorted by case)
 int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
-  std::vector<Work> pts, sp, lw[3], cw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm;
+  std::vector<Work> pts, sp, lw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm;
   L.wr.assign(c->linked.size() + 1, 0);
   int slot = -1;
   for (int g : gs) {
@@ -954,7 +1054,6 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
     for (int a0 = 0; a0 < b.nsponge; a0 += 1024) sp.push_back({g, a0, std::min(1024, b.nsponge - a0), 0});
     for (int d = 0; d < 3; d++) {
       const int nlines = b.N / b.L[d], nl = c->nlp[d] - 1;
-      for (int q = 0; q < nlines; q += 64) cw[d].push_back({g, q, std::min(64, nlines - q), d});
       for (int q = 0; q < nlines; q += nl) lw[d].push_back({g, q, std::min(nl, nlines - q), d});
     }
     const HostCase &h = c->cases[c->bsrc[g].first];
@@ -987,7 +1086,6 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
   if ((rc = upload_list(m, L.normals, nm))) return rc;
   for (int d = 0; d < 3; d++) {
     if ((rc = upload_list(m, L.lines[d], lw[d]))) return rc;
-    if ((rc = upload_list(m, L.colwork[d], cw[d]))) return rc;
     if ((rc = upload_list(m, L.gapply[d], ga[d]))) return rc;
     if ((rc = upload_list(m, L.aapply[d], aa[d]))) return rc;
   }
@@ -1011,6 +1109,7 @@ int link(tp_context *c) {
   }
   drop_graph(c);
   c->link.release();
+  c->lmem.release();
   c->d_blocks = nullptr; c->d_coef = nullptr; c->d_sc = nullptr; c->d_part = nullptr;
   c->d_wr = nullptr; c->d_jct = nullptr; c->d_jvel = nullptr;
   c->L = ListSet();
@@ -1058,19 +1157,12 @@ int link(tp_context *c) {
     c->lds[d] = (size_t)Lmax * (nl + 1) * sizeof(double);
     if (c->lds[d] > 64 * 1024) return fail(TP_E_SIZE, "line too long for the LDS tile");
   }
-  // chunked solver: the tuned chunk count if it still fits every block, else
-  // the requested one, else the default
-  for (int d = 0; d < 3; d++)
-    for (int kind = 0; kind < 2; kind++) {
-      int P = col_default(c->blocks, d);
-      if (c->col_fixed && col_need(c->blocks, d, c->col_fixed)) P = c->col_fixed;
-      const int t = c->tuned[d][kind];
-      if (t == 0 || (t > 0 && col_need(c->blocks, d, t))) P = t;
-      select_col(c, d, kind, P);
-    }
+  // chunked solver: chunks per line per block, one launch per chunk count
+  choose_lines(c);
   std::vector<int> gs(nb);
   for (int g = 0; g < nb; g++) gs[g] = g;
   if ((rc = make_lists(c, gs, c->link, c->L))) return rc;
+  if ((rc = make_groups(c, gs, c->lmem, c->L))) return rc;
   if ((rc = upload_vec(c->link, &c->d_blocks, c->blocks))) return rc;
   if ((rc = upload_vec(c->link, &c->d_coef, coefs))) return rc;
   if ((rc = upload_vec(c->link, &c->d_sc, sc))) return rc;
@@ -1201,6 +1293,7 @@ int32_t tp_destroy(tp_context *c) {
   for (auto e : c->pool) hipEventDestroy(e);
   for (auto &h : c->cases) h.mem.release();
   c->link.release();
+  c->lmem.release();
   hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1383,6 +1476,7 @@ int32_t tp_setup_geometry(tp_context *c) {
     Mem tmp;
     ListSet sub;
     int rc = make_lists(c, gs, tmp, sub);
+    if (!rc) rc = make_groups(c, gs, tmp, sub);
     if (rc) { tmp.release(); return rc; }
     std::swap(c->L, sub);
     const int dst[3] = {F_QO, F_QA, F_DE};
@@ -1652,13 +1746,11 @@ int32_t tp_set_option(tp_context *c, int32_t option, int64_t value) {
     if (P != 0 && P != 1 && P != 2 && P != 4 && P != 8 && P != 16)
       return fail(TP_E_ARG, "chunks per line must be 0, 1, 2, 4, 8 or 16, not " + std::to_string(value));
     c->col_fixed = P;
-    for (int d = 0; d < 3; d++) c->tuned[d][0] = c->tuned[d][1] = -1;
+    for (auto &a : c->tuned)
+      for (auto &b : a)
+        for (int &v : b) v = -1;
     if (!c->finalized) return 0;
-    for (int d = 0; d < 3; d++) {
-      int q = P && col_need(c->blocks, d, P) ? P : col_default(c->blocks, d);
-      for (int kind = 0; kind < 2; kind++) select_col(c, d, kind, q);
-    }
-    return push_blocks(c);
+    return regroup(c);
   }
   return fail(TP_E_ARG, "unknown option " + std::to_string(option));
 }
@@ -1674,37 +1766,61 @@ int32_t tp_autotune(tp_context *c, int32_t reps, int32_t *chosen, double *us) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   int rc = 0;
+  // time the line solves of one direction and kind over all blocks
+  auto timed = [&](int d, int kind, float *ms) -> int {
+    // rr is scratch between stages: deriv rr(:,1) -> rr(:,2), filter rr(:,3) in place
+    auto once = [&]() {
+      if (kind == 0) run_line(c, 0, d, F_RR, 0, F_RR, 1, 0, 0);
+      else run_line(c, 1, d, F_RR, 2, F_RR, 2, 0, 0);
+    };
+    once(); once();
+    if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(TP_E_HIP, "event record");
+    for (int r = 0; r < reps; r++) once();
+    if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+      return fail(TP_E_HIP, "autotune launch failed");
+    hipEventElapsedTime(ms, e0, e1);
+    return 0;
+  };
   for (int d = 0; d < 3 && !rc; d++) {
     for (int kind = 0; kind < 2 && !rc; kind++) {
-      int best = c->colP[d][kind];
+      // the line-length classes present, each tuned in turn with the others fixed
+      bool present[kNClass] = {false};
+      for (const Block &b : c->blocks) present[col_class(col_default(b.L[d]))] = true;
       float bestms = 1e30f;
-      for (int pi = -1; pi < kNColP && !rc; pi++) {
-        const int P = pi < 0 ? 0 : kColP[pi];   // 0: the sequential kernel
-        if (P && !col_need(c->blocks, d, P)) continue;
-        select_col(c, d, kind, P);
-        if ((rc = push_blocks(c))) break;
-        // rr is scratch between stages: deriv rr(:,1) -> rr(:,2), filter rr(:,3) in place
-        auto once = [&]() {
-          if (kind == 0) run_line(c, 0, d, F_RR, 0, F_RR, 1, 0, 0);
-          else run_line(c, 1, d, F_RR, 2, F_RR, 2, 0, 0);
-        };
-        once(); once();
-        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(TP_E_HIP, "event record"); break; }
-        for (int r = 0; r < reps; r++) once();
-        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipGetLastError() != hipSuccess) { rc = fail(TP_E_HIP, "autotune launch failed"); break; }
-        float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
-        if (ms < bestms) { bestms = ms; best = P; }
+      for (int cls = 0; cls < kNClass && !rc; cls++) {
+        if (!present[cls]) continue;
+        int best = -2;
+        bestms = 1e30f;
+        for (int pi = -1; pi < kNColP && !rc; pi++) {
+          const int P = pi < 0 ? 0 : kColP[pi];   // 0: the sequential kernel
+          bool ok = true;
+          for (const Block &b : c->blocks)
+            if (col_class(col_default(b.L[d])) == cls && P && !col_chunk(b.L[d], P)) ok = false;
+          if (!ok) continue;
+          c->tuned[d][kind][cls] = P;
+          if ((rc = regroup(c))) break;
+          float ms = 0;
+          if ((rc = timed(d, kind, &ms))) break;
+          if (ms < bestms) { bestms = ms; best = P; }
+        }
+        if (rc) break;
+        c->tuned[d][kind][cls] = best >= 0 ? best : -1;
       }
       if (rc) break;
-      select_col(c, d, kind, best);
-      c->tuned[d][kind] = best;
-      if (chosen) chosen[d * 2 + kind] = c->colP[d][kind];
-      if (us) us[d * 2 + kind] = bestms < 1e29f ? 1e3 * bestms / reps : 0.0;
+      if ((rc = regroup(c))) break;
+      float ms = 0;
+      if ((rc = timed(d, kind, &ms))) break;
+      // report the chunk count that covers the most points
+      long long top = -1;
+      int P = 0;
+      for (const ColGroup &g : c->L.cg[d][kind])
+        if (g.npts > top) { top = g.npts; P = g.P; }
+      if (chosen) chosen[d * 2 + kind] = P;
+      if (us) us[d * 2 + kind] = 1e3 * ms / reps;
     }
   }
-  if (!rc) rc = push_blocks(c);
+  if (!rc) rc = regroup(c);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   c->prof = prof;
@@ -1713,8 +1829,15 @@ int32_t tp_autotune(tp_context *c, int32_t reps, int32_t *chosen, double *us) {
 }
 
 int32_t tp_col_config(tp_context *c, int32_t *chunks6) {
-  if (!c) return fail(TP_E_ARG, "null context");
-  for (int i = 0; i < 6; i++) chunks6[i] = c->exact_lines ? 0 : c->colP[i / 2][i % 2];
+  if (!c || !chunks6) return fail(TP_E_ARG, "null argument");
+  for (int i = 0; i < 6; i++) {
+    long long top = -1;
+    int P = 0;
+    if (!c->exact_lines)
+      for (const ColGroup &g : c->L.cg[i / 2][i % 2])
+        if (g.npts > top) { top = g.npts; P = g.P; }
+    chunks6[i] = P;
+  }
   return 0;
 }
 

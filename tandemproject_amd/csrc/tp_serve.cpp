@@ -22,6 +22,7 @@ namespace {
 struct Request {
   int cid = -1;                  // case handle in the context
   int nb = 0;
+  int64_t points = 0;
   int nsteps = 0, steps = 0;     // requested and made
   int state = 0;                 // 0 queued, 1 on the device, 2 done, 3 failed
   double t_arr = 0, t_adm = -1, t_done = -1;
@@ -44,6 +45,7 @@ double percentile(std::vector<double> v, double q) {
 struct tp_server {
   tp_context *ctx = nullptr;
   int64_t budget = 0, used = 0, peak_used = 0;
+  int64_t fill = 0, active_points = 0;   // admission stops at fill points on the device (0: no limit)
   int max_active = 0, max_seen = 0;
   bool graph = true;
   std::chrono::steady_clock::time_point t0;
@@ -51,6 +53,9 @@ struct tp_server {
   std::vector<int> active;       // request ids on the device, in admission order
   int64_t admitted_while_busy = 0;   // admissions that joined running cases
   double busy_s = 0;             // wall time spent stepping
+  double admit_s = 0, retire_s = 0, capture_s = 0;   // wall time of admissions, retirements, graph captures
+  int64_t captures = 0;
+  bool changed = true;           // membership changed since the last captured step
   std::string err;
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -81,15 +86,18 @@ int retire(tp_server *s, int r) {
     off += n;
   }
   if (int rc = tp_remove_case(s->ctx, q.cid)) return rc;
+  s->changed = true;
   q.t_done = s->now();
   q.state = 2;
   s->used -= q.bytes;
+  s->active_points -= q.points;
   s->active.erase(std::find(s->active.begin(), s->active.end(), r));
   return 0;
 }
 
-// admit arrived requests in arrival order while they fit; a request that
-// does not fit is passed over for later, smaller ones
+// admit arrived requests in arrival order while they fit and the cases on
+// the device hold fewer than the fill target's points; a request that does
+// not fit the budget is passed over for later, smaller ones
 int admit(tp_server *s, int *nadm) {
   *nadm = 0;
   const double t = s->now();
@@ -102,9 +110,13 @@ int admit(tp_server *s, int *nadm) {
   for (int r : order) {
     Request &q = s->reqs[r];
     if ((int)s->active.size() >= s->max_active) break;
+    if (s->fill > 0 && !s->active.empty() && s->active_points >= s->fill) break;
     if (s->used + q.bytes > s->budget) continue;
+    const double ta = s->now();
     int rc = tp_admit(s->ctx, q.cid);
     if (!rc) rc = tp_initialo(s->ctx, q.cid);
+    s->admit_s += s->now() - ta;
+    s->changed = true;
     if (rc) {
       q.state = 3;
       q.error = tp_last_error();
@@ -114,6 +126,7 @@ int admit(tp_server *s, int *nadm) {
     q.state = 1;
     q.t_adm = s->now();
     s->used += q.bytes;
+    s->active_points += q.points;
     s->peak_used = std::max(s->peak_used, s->used);
     s->active.push_back(r);
     s->max_seen = std::max(s->max_seen, (int)s->active.size());
@@ -132,6 +145,14 @@ int iterate(tp_server *s, int max_steps, int32_t *status) {
     for (int r : s->active) k = std::min(k, s->reqs[r].nsteps - s->reqs[r].steps);
     k = std::max(k, 0);
     if (k > 0) {
+      if (s->graph && s->changed) {
+        // capture the step of the new set of cases (no replay yet)
+        const double tc = s->now();
+        if (int rc = tp_step_graph(s->ctx, 0)) return srv_fail(s, rc, std::string("capture: ") + tp_last_error());
+        s->capture_s += s->now() - tc;
+        s->captures++;
+      }
+      s->changed = false;
       const double t = s->now();
       int rc = s->graph ? tp_step_graph(s->ctx, k) : tp_step(s->ctx, k);
       if (!rc) rc = tp_sync(s->ctx);
@@ -143,8 +164,10 @@ int iterate(tp_server *s, int max_steps, int32_t *status) {
     std::vector<int> fin;
     for (int r : s->active)
       if (s->reqs[r].steps >= s->reqs[r].nsteps) fin.push_back(r);
+    const double tr = s->now();
     for (int r : fin)
       if (int rc = retire(s, r)) return srv_fail(s, rc, std::string("retire: ") + tp_last_error());
+    s->retire_s += s->now() - tr;
   }
   if (status) {
     int queued = 0, waiting = 0, done = 0;
@@ -202,6 +225,12 @@ int32_t tp_server_destroy(tp_server *s) {
 
 tp_context *tp_server_context(tp_server *s) { return s ? s->ctx : nullptr; }
 
+int32_t tp_server_set_fill(tp_server *s, int64_t points) {
+  if (!s || points < 0) return TP_E_ARG;
+  s->fill = points;
+  return 0;
+}
+
 const char *tp_server_error(tp_server *s) { return s ? s->err.c_str() : "null server"; }
 
 double tp_server_clock(tp_server *s) { return s ? s->now() : 0.0; }
@@ -235,6 +264,8 @@ int32_t tp_server_submit(tp_server *s, int32_t nblocks, const int32_t *dims, con
   }
   q.nb = nblocks;
   q.nsteps = nsteps;
+  for (int b = 0; b < nblocks; b++)
+    q.points += (int64_t)(dims[3 * b] + 1) * (dims[3 * b + 1] + 1) * (dims[3 * b + 2] + 1);
   q.t_arr = t_arrival >= 0 ? t_arrival : s->now();
   *req = (int32_t)s->reqs.size();
   s->reqs.push_back(std::move(q));
@@ -293,8 +324,9 @@ int64_t tp_server_result_count(tp_server *s, int32_t req) {
   return (int64_t)s->reqs[req].qa.size();
 }
 
-int32_t tp_server_stats(tp_server *s, double *out12) {
-  if (!s || !out12) return TP_E_ARG;
+int32_t tp_server_stats(tp_server *s, double *out16) {
+  if (!s || !out16) return TP_E_ARG;
+  double *out12 = out16;
   std::vector<double> tts;
   double first = 1e300, last = 0;
   int failed = 0;
@@ -321,6 +353,10 @@ int32_t tp_server_stats(tp_server *s, double *out12) {
   out12[9] = (double)s->max_seen;
   out12[10] = (double)s->admitted_while_busy;
   out12[11] = s->busy_s;
+  out16[12] = s->admit_s;
+  out16[13] = s->retire_s;
+  out16[14] = s->capture_s;
+  out16[15] = (double)s->captures;
   return 0;
 }
 

@@ -251,6 +251,11 @@ module tandem_hip
       import :: c_ptr
       type(c_ptr), value :: srv
     end function
+    integer(c_int32_t) function tp_server_set_fill(srv, points) bind(C, name="tp_server_set_fill")
+      import :: c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: srv
+      integer(c_int64_t), value :: points
+    end function
     type(c_ptr) function tp_server_error(srv) bind(C, name="tp_server_error")
       import :: c_ptr
       type(c_ptr), value :: srv
@@ -295,10 +300,10 @@ module tandem_hip
       type(c_ptr), value :: srv
       integer(c_int32_t), value :: req
     end function
-    integer(c_int32_t) function tp_server_stats(srv, out12) bind(C, name="tp_server_stats")
+    integer(c_int32_t) function tp_server_stats(srv, out16) bind(C, name="tp_server_stats")
       import :: c_ptr, c_int32_t, c_double
       type(c_ptr), value :: srv
-      real(c_double), intent(out) :: out12(12)
+      real(c_double), intent(out) :: out16(16)
     end function
     pure integer(c_int64_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_ptr, c_int64_t

@@ -178,3 +178,25 @@ def test_server_admits_by_hbm_budget():
         tiny.submit(s1[0], s1[2], s1[1], 3, 0.0)
     tiny.close()
     srv.close()
+
+
+def test_server_fill_target_runs_cases_in_turn():
+    """With a fill target below one case's points the loop admits one case at
+    a time, in arrival order, and each result equals the case run alone."""
+    say("request loop with a one-point fill target")
+    srv = api.Server(budget_bytes=0, max_active=8, graph=True, fill_points=1)
+    srv.set_option(0, 1)
+    nsteps = [5, 4, 6]
+    reqs = [srv.submit(lay, p, ss, n, 0.0) for (lay, ss, p), n in zip(CASES[:3], nsteps)]
+    srv.run(max_steps=4)
+    st = srv.stats()
+    assert st["finished"] == 3 and st["peak_cases"] == 1
+    infos = [srv.result(r, with_state=False)[0] for r in reqs]
+    for a, b in zip(infos, infos[1:]):
+        assert b["admitted"] >= a["done"]
+    for r, (lay, ss, p), n in zip(reqs, CASES, nsteps):
+        _, qa = srv.result(r)
+        ref, _ = alone(lay, ss, p, n)
+        for q, rr in zip(qa, ref):
+            np.testing.assert_array_equal(q, rr)
+    srv.close()
