@@ -165,8 +165,9 @@ int32_t tp_set_option(tp_context *ctx, int32_t option, int64_t value);
  * 1..16 chunks) on the context's own blocks, reps launches each, and keep
  * the fastest per direction, kind and line-length class (blocks whose lines
  * have the same default chunk count form a class; the blocks of one chunk
- * count share a launch).  Overwrites the scratch array rr, so call it
- * between steps.  chosen[d*2+kind] (kind 0 deriv, 1 filter) receives the
+ * count share a launch), five fields per launch as a time step runs them.
+ * Overwrites the stage scratch (the flux and gradient work array), so call
+ * it between steps.  chosen[d*2+kind] (kind 0 deriv, 1 filter) receives the
  * chunk count used for the most points (0: sequential kernel), us[d*2+kind]
  * the time of the direction's launches; either may be NULL.  No counterpart
  * in the reference: it replaces the fixed 1-D loop structure of deriv/filte

@@ -26,6 +26,13 @@
  *   stage body          src/Main3D.f90:567-958                     uniform flow preserved on a periodic grid;
  *                                                                  otherwise parity unpinned (see below)
  *   spongeup/spongego   src/SingleTurbulence3D.f90:209-374         parity unpinned
+ *   inflow table        src/SingleTurbulence3D.f90:253-292 (the    tests/test_inflow_cpu.py: periodic over one
+ *     (Fourier modes)     eddy table's layout, filled from modes)  period; one gust mode gives the closed-form
+ *                                                                  travelling sinusoid A cos(theta - 2 pi it/ntz)
+ *   orc_inflow_weights  src/SingleTurbulence3D.f90:336-356         tests/test_inflow_cpu.py: exact for quartics
+ *                                                                  in the node times, wrapped nodes included
+ *   inflow target       src/SingleTurbulence3D.f90:360-372         parity unpinned (isentropic density and
+ *     (spongego)                                                   momentum from the interpolated velocity)
  *   junction            src/SingleTurbulence3D.f90:378-443         parity unpinned
  *   initialo            src/SingleTurbulence3D.f90:179-205         uniform-flow test (trivial state)
  *

@@ -279,7 +279,10 @@ void closure_rows(double fk, CaseCoef &c) {
 //   then per chunk q three registers: {l1 | l2}, {u1 | u2}, {u3 | 0}, row k
 //   of the chunk in lane k (first half) or 32+k (second half).
 // Factor entries the sweeps never use (l1, l2 of rows 0, 1; u2, u3 of rows
-// L-1, L-2) and rows past the chunk are zero.
+// L-1, L-2) are zero.  Rows past the chunk's end (up to the 32 slots) hold
+// l1 = -1, l2 = 0 and u1 = u2 = 0, u3 = -1: with a zero right-hand side such
+// a row only swaps the two carried values, exactly, so the sweeps run over
+// the whole capacity without selects (see fwd_sweep / bwd_sweep).
 std::vector<double> col_tables(int L, int P, const double *l1, const double *l2, const double *u1,
                                const double *u2, const double *u3) {
   const int Cl = (L + P - 1) / P, Pe = (L + Cl - 1) / Cl;
@@ -296,6 +299,11 @@ std::vector<double> col_tables(int L, int P, const double *l1, const double *l2,
       F[k] = L1(i); F[32 + k] = L2(i);
       F[64 + k] = u1[i]; F[96 + k] = U2(i);
       F[128 + k] = U3(i);
+    }
+    for (int k = e - s0; k < 32; k++) {
+      F[k] = -1.0; F[32 + k] = 0.0;
+      F[64 + k] = 0.0; F[96 + k] = 0.0;
+      F[128 + k] = -1.0;
     }
     for (int which = 0; which < 2; which++) {
       double ym2 = which == 0 ? 1 : 0, ym1 = which == 0 ? 0 : 1;
@@ -413,7 +421,7 @@ struct DevList {
 
 // launch tables over a set of blocks
 // one launch of the line solves of a direction and kind: the blocks whose
-// lines use P chunks (P = 0: the sequential kernel), chunk capacity C
+// lines use P chunks (P = 0: the sequential kernel) of capacity C points
 struct ColGroup {
   int P = 0, C = 0;
   long long npts = 0;
@@ -557,35 +565,41 @@ StageArgs base_args(tp_context *c) {
   return a;
 }
 
-void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0) {
+// closure sums of nf fields (columns col0 + f fstep, f < nf) in one launch,
+// into recv slots 0..nf-1
+void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0, int nf = 1, int fstep = 0) {
   if (!c->L.exch.n) return;
   Timed t(c, K_EXCH, 0.0);
   ExchArgs a;
   a.blocks = c->d_blocks; a.work = c->L.exch.d; a.coef = c->d_coef;
-  a.nt = nt; a.nrt = nrt; a.n45 = n45; a.arr = arr; a.col0 = col0;
-  launch_exchange(c->L.exch.n, c->stream, a);
+  a.nt = nt; a.nrt = nrt; a.n45 = n45; a.arr = arr; a.col0 = col0; a.fstep = fstep;
+  launch_exchange(c->L.exch.n, nf, c->stream, a);
 }
 
+// line solves of direction d over nf fields in one launch: field f reads
+// column in_col + f in_step, writes out_col + f out_step, uses recv slot f
+// (filled by an exchange over the same fields) and captures drva column m + f
 void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_arr, int out_col, int m,
-              int acc) {
+              int acc, int nf = 1, int in_step = 0, int out_step = 0) {
   if (!c->L.lines[d].n) return;
   // algorithmic bytes: one field in, one out; an accumulated output is also read
-  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, (acc ? 24.0 : 16.0) * (double)c->npts);
+  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, (acc ? 24.0 : 16.0) * (double)c->npts * nf);
   LineArgs a;
   a.blocks = c->d_blocks; a.work = c->L.lines[d].d; a.coef = c->d_coef;
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
   a.m = m; a.acc = acc; a.nlp = c->nlp[d];
+  a.in_step = in_step; a.out_step = out_step;
   if (c->exact_lines) {
-    if (kind == 0) launch_deriv(d, c->L.lines[d].n, c->lds[d], c->stream, a);
-    else launch_filter(d, c->L.lines[d].n, c->lds[d], c->stream, a);
+    if (kind == 0) launch_deriv(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
+    else launch_filter(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
     return;
   }
   for (const ColGroup &g : c->L.cg[d][kind]) {
     if (!g.work.n) continue;
     a.work = g.work.d;
-    if (g.P) launch_col(d, kind, g.C, g.P, g.work.n, c->stream, a);
-    else if (kind == 0) launch_deriv(d, g.work.n, c->lds[d], c->stream, a);
-    else launch_filter(d, g.work.n, c->lds[d], c->stream, a);
+    if (g.P) launch_col(d, kind, g.C, g.P, g.work.n, nf, c->stream, a);
+    else if (kind == 0) launch_deriv(d, g.work.n, nf, c->lds[d], c->stream, a);
+    else launch_filter(d, g.work.n, nf, c->lds[d], c->stream, a);
   }
 }
 
@@ -623,34 +637,31 @@ void choose_lines(tp_context *c) {
 }
 
 // line-solve launches over the global blocks gs: one per distinct chunk
-// count of each direction and kind
+// count and chunk capacity of each direction and kind
 int make_groups(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
   for (int d = 0; d < 3; d++)
     for (int kind = 0; kind < 2; kind++) {
       std::vector<ColGroup> groups;
       std::vector<std::vector<Work>> works;
-      std::vector<int> cmax;
       for (int g : gs) {
         const Block &b = c->blocks[g];
         const int P = block_chunks(c, g, d, kind);
+        const int C = P ? col_capacity((b.L[d] + P - 1) / P) : 0;
         size_t k = 0;
-        while (k < groups.size() && groups[k].P != P) k++;
+        while (k < groups.size() && (groups[k].P != P || groups[k].C != C)) k++;
         if (k == groups.size()) {
           groups.emplace_back();
           groups.back().P = P;
+          groups.back().C = C;
           works.emplace_back();
-          cmax.push_back(0);
         }
         const int nlines = b.N / b.L[d];
         const int step = P ? 64 : c->nlp[d] - 1;
         for (int q = 0; q < nlines; q += step) works[k].push_back({g, q, std::min(step, nlines - q), d});
-        if (P) cmax[k] = std::max(cmax[k], (b.L[d] + P - 1) / P);
         groups[k].npts += b.N;
       }
-      for (size_t k = 0; k < groups.size(); k++) {
-        groups[k].C = groups[k].P ? col_capacity(cmax[k]) : 0;
+      for (size_t k = 0; k < groups.size(); k++)
         if (int rc = upload_list(m, groups[k].work, works[k])) return rc;
-      }
       L.cg[d][kind] = std::move(groups);
     }
   return 0;
@@ -672,11 +683,11 @@ int regroup(tp_context *c) {
 }
 
 void enqueue_filter_step(tp_context *c) {
-  // src/Main3D.f90:555-561
-  for (int m = 1; m <= 5; m++) {
-    run_exchange(c, 1, 1, 0, F_QA, m - 1);
-    for (int d = 0; d < 3; d++) run_line(c, 1, d, F_QA, m - 1, F_QA, m - 1, 0, 0);
-  }
+  // src/Main3D.f90:555-561: per field m, mpigo then the xi, eta, zeta filters
+  // in place; the five fields are independent, so each of these steps runs
+  // over all five in one launch
+  run_exchange(c, 1, 1, 0, F_QA, 0, 5, 1);
+  for (int d = 0; d < 3; d++) run_line(c, 1, d, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
   run_points(c, PK_COPY5, base_args(c), K_MISC, 80.0);
 }
 
@@ -692,26 +703,22 @@ void enqueue_stage(tp_context *c, int nk) {
   }
   if (c->any_visc) {
     // src/Main3D.f90:627-692: gradients of u, v, w, T (de(:,2:5)) along the
-    // three index directions into fx(:,3f+d), then one pass forming the
-    // stresses and heat fluxes
-    for (int f = 0; f < 4; f++) {
-      run_exchange(c, 0, 1, 0, F_DE, f + 1);
-      run_line(c, 0, 2, F_DE, f + 1, F_FX, 3 * f + 2, 0, 0);
-      run_line(c, 0, 1, F_DE, f + 1, F_FX, 3 * f + 1, 0, 0);
-      run_line(c, 0, 0, F_DE, f + 1, F_FX, 3 * f + 0, 0, 0);
-    }
+    // three index directions into fx(:,3f+d) -- the four fields in one launch
+    // per direction -- then one pass forming the stresses and heat fluxes
+    run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1);
+    run_line(c, 0, 2, F_DE, 1, F_FX, 2, 0, 0, 4, 1, 3);
+    run_line(c, 0, 1, F_DE, 1, F_FX, 1, 0, 0, 4, 1, 3);
+    run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3);
     run_points(c, PK_GRADSTRESS, a, K_VISC, 8.0 * (26 + 10));
   }
   // src/Main3D.f90:710-768: all 15 flux components in one pass, then
   // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
   run_points(c, PK_FLUXALL, a, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 18 + 15));
-  for (int m = 1; m <= 5; m++) {
-    const int f0 = 3 * (m - 1);
-    run_exchange(c, 0, 0, 0, F_FX, f0);
-    run_line(c, 0, 0, F_FX, f0 + 0, F_DE, m - 1, m, 0);
-    run_line(c, 0, 1, F_FX, f0 + 1, F_DE, m - 1, m, 1);
-    run_line(c, 0, 2, F_FX, f0 + 2, F_DE, m - 1, m, 1);
-  }
+  // the five fields m in one launch per step: flux components fx(:,3(m-1)+d)
+  run_exchange(c, 0, 0, 0, F_FX, 0, 5, 3);
+  run_line(c, 0, 0, F_FX, 0, F_DE, 0, 1, 0, 5, 3, 1);
+  run_line(c, 0, 1, F_FX, 1, F_DE, 0, 1, 1, 5, 3, 1);
+  run_line(c, 0, 2, F_FX, 2, F_DE, 0, 1, 1, 5, 3, 1);
   if (c->L.sponge.n) {
     Timed t(c, K_MISC, 0.0);
     StageArgs s = a;
@@ -831,7 +838,7 @@ Arena arena_layout(const Block &b, int nwall) {
   A.tv = take(9 * N); A.fx = take(15 * N);
   for (int d = 0; d < 3; d++) {
     A.f[d][0] = take(10 * (size_t)b.F[d]); A.f[d][1] = take(10 * (size_t)b.F[d]);
-    A.f[d][2] = take(6 * (size_t)b.F[d]); A.f[d][3] = take(6 * (size_t)b.F[d]);
+    A.f[d][2] = take(6 * RCV_SLOTS * (size_t)b.F[d]); A.f[d][3] = take(6 * (size_t)b.F[d]);
   }
   A.lud = take(5 * (size_t)b.nlu); A.luf = take(5 * (size_t)b.nlu); A.rp = take(std::max(nwall, 1));
   A.total = off;
@@ -1768,10 +1775,11 @@ int32_t tp_autotune(tp_context *c, int32_t reps, int32_t *chosen, double *us) {
   int rc = 0;
   // time the line solves of one direction and kind over all blocks
   auto timed = [&](int d, int kind, float *ms) -> int {
-    // rr is scratch between stages: deriv rr(:,1) -> rr(:,2), filter rr(:,3) in place
+    // fx is scratch between steps: five fields at a time, as the stage runs
+    // them: deriv fx(:,1:5) -> fx(:,6:10), filter fx(:,11:15) in place
     auto once = [&]() {
-      if (kind == 0) run_line(c, 0, d, F_RR, 0, F_RR, 1, 0, 0);
-      else run_line(c, 1, d, F_RR, 2, F_RR, 2, 0, 0);
+      if (kind == 0) run_line(c, 0, d, F_FX, 0, F_FX, 5, 0, 0, 5, 1, 1);
+      else run_line(c, 1, d, F_FX, 10, F_FX, 10, 0, 0, 5, 1, 1);
     };
     once(); once();
     if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(TP_E_HIP, "event record");

@@ -64,10 +64,9 @@ struct GLine {
 // boundary rows of the derivative RHS (rows 0..2 and L-3..L-1)
 template <class SA>
 __device__ inline double deriv_bnd_row(const Block &b, const CaseCoef &c, int D, int i, int L, int jk,
-                                       const SA &sa) {
+                                       const SA &sa, const double *rcv) {
   const double aa = 0.6431406736919156, ab = 0.2586011023495066, ac = 0.007140953479797375;
   const int F = b.F[D];
-  const double *rcv = b.recv[D];
   const int ie = L - 1;
   if (i <= 2) {
     if (b.ndf[D][0] == 0) {
@@ -120,9 +119,8 @@ __device__ inline double deriv_bnd_row(const Block &b, const CaseCoef &c, int D,
 // boundary rows of the filter RHS
 template <class SA>
 __device__ inline double filter_bnd_row(const Block &b, const CaseCoef &c, int D, int i, int L, int jk,
-                                        const SA &sa) {
+                                        const SA &sa, const double *rcv) {
   const int F = b.F[D];
-  const double *rcv = b.recv[D];
   const int ie = L - 1;
   const double fa = c.fa, fb = c.fb, fc = c.fc;
   if (i <= 2) {
@@ -202,47 +200,60 @@ __device__ inline double lane_val(double v, int l) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
-// Factors of the chunk's rows, one register per pair of factor kinds (row k
-// in lane k and 32+k); see col_tables on the host.
+// Factors of the chunk's rows, copied to the wave's LDS slice:
+// F[k] = l1, F[32+k] = l2, F[64+k] = u1, F[96+k] = u2, F[128+k] = u3 of row
+// k.  Every lane reads the same address (an LDS broadcast).
 struct ChunkLU {
-  double f0, f1, f2;   // {l1 | l2}, {u1 | u2}, {u3 | 0}
+  const double *F;
 };
 
-// Forward substitution over the chunk rows k < n from the carry (ym2, ym1),
-// in place when store is set; rows k >= n (capacity padding) are left alone.
-// Branch-free: the row condition is wave-uniform and becomes a select.  The
-// opaque lane offset keeps each sweep's factor reads inside the sweep.
+// Forward substitution over the chunk rows from the carry (ym2, ym1), in
+// place when store is set.  Rows k >= n are capacity padding: their factors
+// (l1 = -1, l2 = 0) and zero right-hand side make each of them swap the
+// carried pair exactly (y = 0 + ym2 - 0), so after them the pair is the one
+// of row n-1, swapped once more if their count is odd.  The opaque offset
+// keeps each sweep's factor loads inside the sweep.
 template <int C>
 __device__ inline void fwd_sweep(double (&r)[C], int n, const ChunkLU &f, double &ym2, double &ym1,
                                  bool store) {
-  const int z = opaque_zero();
+  const double *F = f.F + opaque_zero();
 #pragma unroll
   for (int k = 0; k < C; k++) {
-    const double f1 = lane_val(f.f0, k + z);
-    const double f2 = lane_val(f.f0, 32 + k + z);
+    const double f1 = F[k];
+    const double f2 = F[32 + k];
     const double y = r[k] - f1 * ym2 - f2 * ym1;
-    const bool on = k < n;
-    if (store) r[k] = on ? y : r[k];
-    ym2 = on ? ym1 : ym2;
-    ym1 = on ? y : ym1;
+    if (store) r[k] = y;
+    ym2 = ym1;
+    ym1 = y;
+  }
+  if ((C - n) & 1) {
+    const double t = ym2;
+    ym2 = ym1;
+    ym1 = t;
   }
 }
 
-// backward substitution over rows k < n from the carry (xp1, xp2) = (x[e+1], x[e+2])
+// backward substitution from the carry (xp1, xp2) = (x[e+1], x[e+2]): the
+// padding rows k >= n come first, each swapping the pair (u1 = u2 = 0,
+// u3 = -1), so the carry enters them pre-swapped when their count is odd
 template <int C>
 __device__ inline void bwd_sweep(double (&r)[C], int n, const ChunkLU &f, double &xp1, double &xp2,
                                  bool store) {
-  const int z = opaque_zero();
+  const double *F = f.F + opaque_zero();
+  if ((C - n) & 1) {
+    const double t = xp1;
+    xp1 = xp2;
+    xp2 = t;
+  }
 #pragma unroll
   for (int k = C - 1; k >= 0; k--) {
-    const double g1 = lane_val(f.f1, k + z);
-    const double g2 = lane_val(f.f1, 32 + k + z);
-    const double g3 = lane_val(f.f2, k + z);
+    const double g1 = F[64 + k];
+    const double g2 = F[96 + k];
+    const double g3 = F[128 + k];
     const double x = g1 * r[k] - g2 * xp1 - g3 * xp2;
-    const bool on = k < n;
-    if (store) r[k] = on ? x : r[k];
-    xp2 = on ? xp1 : xp2;
-    xp1 = on ? x : xp1;
+    if (store) r[k] = x;
+    xp2 = xp1;
+    xp1 = x;
   }
 }
 
@@ -265,17 +276,23 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   const int stride = D == 0 ? 1 : D == 1 ? b.nx : b.nx * b.ny;
   int base, jk;
   col_line<D>(b, w.a + (act ? lane : 0), base, jk);
-  const double *src = field_ptr(b, a.in_arr, a.in_col);
-  double *dst = field_ptr(b, a.out_arr, a.out_col);
+  const int fi = blockIdx.y;   // field of a merged launch
+  const double *src = field_ptr(b, a.in_arr, a.in_col + fi * a.in_step);
+  double *dst = field_ptr(b, a.out_arr, a.out_col + fi * a.out_step);
+  const double *rcv = b.recv[D] + (size_t)fi * 6 * b.F[D];
+  const int mcap = a.m > 0 ? a.m + fi : 0;
   double2 *pairA = reinterpret_cast<double2 *>(lds);      // [P][64] carry-map inputs
   double2 *pairB = pairA + P * 64;                        // [P][64] pairs handed along a chain
-  double *tr = reinterpret_cast<double *>(pairB + P * 64) + p * (8 * COL_TR);
+  double *fl = reinterpret_cast<double *>(pairB + P * 64) + p * 192;   // [P][192] chunk factors
+  double *tr = reinterpret_cast<double *>(pairB + P * 64) + P * 192 + p * (8 * COL_TR);
   const double *T = b.col[D][KIND];
   const double mapf = T[lane], mapb = T[64 + lane];   // carry maps of all chunks
   ChunkLU f;
   {
     const double *F = T + 128 + 192 * min(p, Pe - 1);
-    f.f0 = F[lane]; f.f1 = F[64 + lane]; f.f2 = F[128 + lane];
+    fl[lane] = F[lane]; fl[64 + lane] = F[64 + lane]; fl[128 + lane] = F[128 + lane];
+    f.F = fl;
+    wave_sync();   // the wave's own LDS writes precede its reads in issue order
   }
 
   // Chains at one-sided closures.  End: the last Rhi chunks take their
@@ -304,8 +321,10 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     }
   }
 
-  // ---- window rows s0-3 .. s0+C+2, clamped to the line ----
+  // ---- window rows s0-3 .. s0+n+2, clamped to the line; the rows of the
+  // capacity padding past them are not read (zero) ----
   double v[C + 6];
+  const int nw = n + 6;   // window rows in use
   if (D == 0) {
     // all global loads of the window first (8 lines x 8 consecutive points per
     // instruction), then the transposes through LDS piece by piece
@@ -317,25 +336,31 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int rl = min((lane >> 3) + 8 * j, nl - 1);
-        tmp[t][j] = at(src, 8u * (unsigned)((w.a + rl) * b.nx + row));
+        tmp[t][j] = 8 * t < nw ? at(src, 8u * (unsigned)((w.a + rl) * b.nx + row)) : 0.0;
       }
     }
 #pragma unroll
     for (int t = 0; t < NPC; t++) {
+      if (8 * t < nw) {
 #pragma unroll
-      for (int j = 0; j < 8; j++) tr[(lane & 7) * COL_TR + (lane >> 3) + 8 * j] = tmp[t][j];
-      wave_sync();
+        for (int j = 0; j < 8; j++) tr[(lane & 7) * COL_TR + (lane >> 3) + 8 * j] = tmp[t][j];
+        wave_sync();
 #pragma unroll
-      for (int rr = 0; rr < 8; rr++)
-        if (8 * t + rr < C + 6) v[8 * t + rr] = tr[rr * COL_TR + lane];
-      wave_sync();
+        for (int rr = 0; rr < 8; rr++)
+          if (8 * t + rr < C + 6) v[8 * t + rr] = tr[rr * COL_TR + lane];
+        wave_sync();
+      } else {
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++)
+          if (8 * t + rr < C + 6) v[8 * t + rr] = 0.0;
+      }
     }
   } else {
     const unsigned b8 = 8u * (unsigned)base, st8 = 8u * (unsigned)stride;
 #pragma unroll
     for (int k = 0; k < C + 6; k++) {
       const int row = min(max(s0 - 3 + k, 0), L - 1);
-      v[k] = at(src, b8 + (unsigned)row * st8);
+      v[k] = k < nw ? at(src, b8 + (unsigned)row * st8) : 0.0;
     }
   }
 
@@ -347,46 +372,59 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     const GLine g{src + base, stride};
     if (n > 0 && s0 <= 2) {
       if (KIND == 0) {
-        rb0 = deriv_bnd_row(b, c, D, 0, L, jk, g);
-        rb1 = deriv_bnd_row(b, c, D, 1, L, jk, g);
-        rb2 = deriv_bnd_row(b, c, D, 2, L, jk, g);
+        rb0 = deriv_bnd_row(b, c, D, 0, L, jk, g, rcv);
+        rb1 = deriv_bnd_row(b, c, D, 1, L, jk, g, rcv);
+        rb2 = deriv_bnd_row(b, c, D, 2, L, jk, g, rcv);
       } else {
-        rb0 = filter_bnd_row(b, c, D, 0, L, jk, g);
-        rb1 = filter_bnd_row(b, c, D, 1, L, jk, g);
-        rb2 = filter_bnd_row(b, c, D, 2, L, jk, g);
+        rb0 = filter_bnd_row(b, c, D, 0, L, jk, g, rcv);
+        rb1 = filter_bnd_row(b, c, D, 1, L, jk, g, rcv);
+        rb2 = filter_bnd_row(b, c, D, 2, L, jk, g, rcv);
       }
     }
     if (n > 0 && s0 + n - 1 >= L - 3) {
       if (KIND == 0) {
-        re2 = deriv_bnd_row(b, c, D, L - 3, L, jk, g);
-        re1 = deriv_bnd_row(b, c, D, L - 2, L, jk, g);
-        re0 = deriv_bnd_row(b, c, D, L - 1, L, jk, g);
+        re2 = deriv_bnd_row(b, c, D, L - 3, L, jk, g, rcv);
+        re1 = deriv_bnd_row(b, c, D, L - 2, L, jk, g, rcv);
+        re0 = deriv_bnd_row(b, c, D, L - 1, L, jk, g, rcv);
       } else {
-        re2 = filter_bnd_row(b, c, D, L - 3, L, jk, g);
-        re1 = filter_bnd_row(b, c, D, L - 2, L, jk, g);
-        re0 = filter_bnd_row(b, c, D, L - 1, L, jk, g);
+        re2 = filter_bnd_row(b, c, D, L - 3, L, jk, g, rcv);
+        re1 = filter_bnd_row(b, c, D, L - 2, L, jk, g, rcv);
+        re0 = filter_bnd_row(b, c, D, L - 1, L, jk, g, rcv);
       }
     }
   }
+  // interior stencil on every row, then the closure rows patched in by the
+  // (at most two) chunks at each end that hold them, then the padding rows
+  // zeroed
   double r[C];
-  const bool edge = n > 0 && (s0 <= 2 || s0 + n - 1 >= L - 3);   // chunk holds closure rows
 #pragma unroll
   for (int k = 0; k < C; k++) {
-    const int i = s0 + k;
-    double rv;
     if (KIND == 0) {
       const double aa = 0.6431406736919156, ab = 0.2586011023495066, ac = 0.007140953479797375;
-      rv = aa * (v[k + 4] - v[k + 2]) + ab * (v[k + 5] - v[k + 1]) + ac * (v[k + 6] - v[k]);
+      r[k] = aa * (v[k + 4] - v[k + 2]) + ab * (v[k + 5] - v[k + 1]) + ac * (v[k + 6] - v[k]);
     } else {
       const double res = 2.0 * v[k + 3];
-      rv = c.fa * (v[k + 2] + v[k + 4] - res) + c.fb * (v[k + 1] + v[k + 5] - res) +
-           c.fc * (v[k] + v[k + 6] - res);
+      r[k] = c.fa * (v[k + 2] + v[k + 4] - res) + c.fb * (v[k + 1] + v[k + 5] - res) +
+             c.fc * (v[k] + v[k + 6] - res);
     }
-    if (edge) {
-      rv = i == 0 ? rb0 : i == 1 ? rb1 : i == 2 ? rb2 : rv;
-      rv = i == L - 1 ? re0 : i == L - 2 ? re1 : i == L - 3 ? re2 : rv;
+  }
+  if (n > 0 && s0 <= 2) {
+#pragma unroll
+    for (int k = 0; k < 3 && k < C; k++) {
+      const int i = s0 + k;
+      r[k] = i == 0 ? rb0 : i == 1 ? rb1 : i == 2 ? rb2 : r[k];
     }
-    r[k] = rv;
+  }
+  if (n > 0 && s0 + n - 1 >= L - 3) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      const int i = s0 + k;
+      r[k] = i == L - 1 ? re0 : i == L - 2 ? re1 : i == L - 3 ? re2 : r[k];
+    }
+  }
+  if (n < C) {
+#pragma unroll
+    for (int k = 0; k < C; k++) r[k] = k < n ? r[k] : 0.0;   // padding rows carry a zero right-hand side
   }
 
   // ---- forward substitution ----
@@ -445,14 +483,14 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   }
 
   // ---- boundary derivatives kept for the characteristic conditions ----
-  if (KIND == 0 && a.m > 0 && act && n > 0) {
+  if (KIND == 0 && mcap > 0 && act && n > 0) {
     const int F = b.F[D];
-    if (p == 0) b.drva[D][(0 * 5 + a.m - 1) * F + jk] = r[0];
+    if (p == 0) b.drva[D][(0 * 5 + mcap - 1) * F + jk] = r[0];
     if (p == Pe - 1) {
       double xe = r[0];
 #pragma unroll
       for (int k = 1; k < C; k++) xe = k == n - 1 ? r[k] : xe;
-      b.drva[D][(1 * 5 + a.m - 1) * F + jk] = xe;
+      b.drva[D][(1 * 5 + mcap - 1) * F + jk] = xe;
     }
   }
 
@@ -462,6 +500,7 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   if (D == 0) {
 #pragma unroll
     for (int t = 0; t < (C + 7) / 8; t++) {
+      if (8 * t >= n) break;   // pieces past the chunk hold no rows of it
 #pragma unroll
       for (int rr = 0; rr < 8; rr++)
         if (8 * t + rr < C) tr[rr * COL_TR + lane] = r[8 * t + rr];
@@ -487,7 +526,7 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     double ov[C];
 #pragma unroll
     for (int k = 0; k < C; k++)
-      ov[k] = add ? at(ob, b8 + (unsigned)min(s0 + k, L - 1) * st8) : 0.0;
+      ov[k] = add && k < n ? at(ob, b8 + (unsigned)min(s0 + k, L - 1) * st8) : 0.0;
     if (act) {
 #pragma unroll
       for (int k = 0; k < C; k++)
@@ -497,14 +536,15 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
 }
 
 size_t col_lds_bytes(int d, int P) {
-  return (size_t)(2 * P * 64) * sizeof(double2) + (d == 0 ? (size_t)P * 8 * COL_TR * sizeof(double) : 0);
+  return (size_t)(2 * P * 64) * sizeof(double2) + (size_t)P * 192 * sizeof(double) +
+         (d == 0 ? (size_t)P * 8 * COL_TR * sizeof(double) : 0);
 }
 
 template <int D, int KIND>
-static void launch_col_c(int C, int P, int nwg, hipStream_t st, const LineArgs &a) {
+static void launch_col_c(int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a) {
   const size_t lds = col_lds_bytes(D, P);
   switch (C) {
-#define CASE_C(CV) case CV: hipLaunchKernelGGL((k_col<D, KIND, CV>), dim3(nwg), dim3(64 * P), lds, st, a); break;
+#define CASE_C(CV) case CV: hipLaunchKernelGGL((k_col<D, KIND, CV>), dim3(nwg, nf), dim3(64 * P), lds, st, a); break;
     CASE_C(8) CASE_C(16) CASE_C(24) CASE_C(32)
 #undef CASE_C
   }
@@ -514,15 +554,15 @@ int col_capacity(int chunk) {
   return chunk <= 8 ? 8 : chunk <= 16 ? 16 : chunk <= 24 ? 24 : chunk <= 32 ? 32 : 0;
 }
 
-void launch_col(int d, int kind, int C, int P, int nwg, hipStream_t st, const LineArgs &a) {
+void launch_col(int d, int kind, int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a) {
   if (kind == 0) {
-    if (d == 0) launch_col_c<0, 0>(C, P, nwg, st, a);
-    else if (d == 1) launch_col_c<1, 0>(C, P, nwg, st, a);
-    else launch_col_c<2, 0>(C, P, nwg, st, a);
+    if (d == 0) launch_col_c<0, 0>(C, P, nwg, nf, st, a);
+    else if (d == 1) launch_col_c<1, 0>(C, P, nwg, nf, st, a);
+    else launch_col_c<2, 0>(C, P, nwg, nf, st, a);
   } else {
-    if (d == 0) launch_col_c<0, 1>(C, P, nwg, st, a);
-    else if (d == 1) launch_col_c<1, 1>(C, P, nwg, st, a);
-    else launch_col_c<2, 1>(C, P, nwg, st, a);
+    if (d == 0) launch_col_c<0, 1>(C, P, nwg, nf, st, a);
+    else if (d == 1) launch_col_c<1, 1>(C, P, nwg, nf, st, a);
+    else launch_col_c<2, 1>(C, P, nwg, nf, st, a);
   }
 }
 

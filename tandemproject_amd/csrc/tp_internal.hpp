@@ -20,6 +20,7 @@ constexpr int LMF = 8;    // filter closure length - 1
 constexpr int LMP = 11;
 constexpr int MBCI = 3;   // GCBC spreading depth
 constexpr int MAX_CASE = 4096;
+constexpr int RCV_SLOTS = 5;  // fields whose closure sums one exchange launch can hold
 
 // ---- per-case scalars on the device (src/MainVar3D.f90:139-142) ----
 enum Scalar {
@@ -59,7 +60,8 @@ struct Block {
   int nsponge;
   double *qa, *qo, *de, *rr, *ss, *xim, *etm, *zem, *p, *yaco, *tv;
   double *fx;                 // [15][N] scratch: gradients of u,v,w,T, then the 15 flux components
-  double *drva[3], *drvb[3], *recv[3], *cm[3];
+  double *drva[3], *drvb[3], *cm[3];
+  double *recv[3];            // [RCV_SLOTS][6][F]: closure sums per field of a merged launch
   double *lud, *luf;          // [5][nlu]: l1, l2, u1, u2, u3 for deriv / filter
   double *rpex;
   int *lcsz;
@@ -119,6 +121,9 @@ struct LineArgs {
   int m;                  // drva capture column 1..5, 0 = none
   int acc;                // deriv: 1 = add into destination
   int nlp;                // LDS row stride (lines per tile + 1)
+  int in_step, out_step;  // column steps per field: field f = blockIdx.y uses
+                          // in_col + f in_step, out_col + f out_step, recv slot f
+                          // and drva column m + f
 };
 
 struct ExchArgs {
@@ -126,7 +131,8 @@ struct ExchArgs {
   const Work *work;        // a = first face point, n = count, d = dir | side << 8
   const CaseCoef *coef;
   int nt, nrt, n45;
-  int arr, col0;           // source field: column col0 + (1-nrt)*dir
+  int arr, col0;           // source field: column col0 + f fstep + (1-nrt)*dir,
+  int fstep;               // for field f = blockIdx.y (recv slot f)
 };
 
 struct StageArgs {
@@ -147,10 +153,10 @@ void launch_inflow_table(const double *modes, int nmodes, const double *pos, int
 // chunked line solves: P waves per 64 lines, chunk capacity C points per lane
 int col_capacity(int chunk);
 size_t col_lds_bytes(int d, int P);
-void launch_col(int d, int kind, int C, int P, int nwg, hipStream_t st, const LineArgs &a);
-void launch_deriv(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a);
-void launch_filter(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a);
-void launch_exchange(int nwg, hipStream_t st, const ExchArgs &a);
+void launch_col(int d, int kind, int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a);
+void launch_deriv(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a);
+void launch_filter(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a);
+void launch_exchange(int nwg, int nf, hipStream_t st, const ExchArgs &a);
 void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a);
 void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a);
 void launch_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st);

@@ -84,8 +84,10 @@ __global__ void __launch_bounds__(64) k_deriv(LineArgs a) {
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const int L = b.L[D], nl = w.n, q0 = w.a, nlp = a.nlp;
-  const double *src = field_ptr(b, a.in_arr, a.in_col);
-  double *dst = field_ptr(b, a.out_arr, a.out_col);
+  const int f = blockIdx.y;
+  const double *src = field_ptr(b, a.in_arr, a.in_col + f * a.in_step);
+  double *dst = field_ptr(b, a.out_arr, a.out_col + f * a.out_step);
+  const int m = a.m > 0 ? a.m + f : 0;
   tile_load<D>(b, src, q0, nl, L, nlp, lds);
   __syncthreads();
   const int r = threadIdx.x;
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(64) k_deriv(LineArgs a) {
     int base, stride, jk;
     line_addr<D>(b, q0 + r, base, stride, jk);
     const int F = b.F[D];
-    const double *rcv = b.recv[D];
+    const double *rcv = b.recv[D] + (size_t)f * 6 * F;
     const double aa = 0.6431406736919156, ab = 0.2586011023495066, ac = 0.007140953479797375;
     const int ie = L - 1;
     double rs0, rs1, rs2, re0, re1, re2;
@@ -169,9 +171,9 @@ __global__ void __launch_bounds__(64) k_deriv(LineArgs a) {
       S(i) = x;
       xp2 = xp1; xp1 = x;
     }
-    if (a.m > 0) {
-      b.drva[D][(0 * 5 + a.m - 1) * F + jk] = S(0);
-      b.drva[D][(1 * 5 + a.m - 1) * F + jk] = S(ie);
+    if (m > 0) {
+      b.drva[D][(0 * 5 + m - 1) * F + jk] = S(0);
+      b.drva[D][(1 * 5 + m - 1) * F + jk] = S(ie);
     }
 #undef S
   }
@@ -188,7 +190,8 @@ __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const int L = b.L[D], nl = w.n, q0 = w.a, nlp = a.nlp;
-  double *io = field_ptr(b, a.in_arr, a.in_col);
+  const int f = blockIdx.y;
+  double *io = field_ptr(b, a.in_arr, a.in_col + f * a.in_step);
   tile_load<D>(b, io, q0, nl, L, nlp, lds);
   __syncthreads();
   const int r = threadIdx.x;
@@ -198,7 +201,7 @@ __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
     int base, stride, jk;
     line_addr<D>(b, q0 + r, base, stride, jk);
     const int F = b.F[D];
-    const double *rcv = b.recv[D];
+    const double *rcv = b.recv[D] + (size_t)f * 6 * F;
     const double fa = c.fa, fb = c.fb, fc = c.fc;
     const int ie = L - 1;
     double rs0, rs1, rs2, re0, re1, re2;
@@ -289,7 +292,8 @@ __global__ void __launch_bounds__(256) k_exchange(ExchArgs a) {
   const Block &s = a.blocks[b.ncd[d][ip]];
   const CaseCoef &c = a.coef[b.cs];
   const int nt = a.nt, mp = nt == 0 ? LMD : LMF;
-  const int col = a.col0 + (1 - a.nrt) * d;
+  const int f = blockIdx.y;
+  const int col = a.col0 + f * a.fstep + (1 - a.nrt) * d;
   const int sp = 1 - ip;
   const int snb = s.nbc[d][sp];
   const int ii = snb == 40 ? 0 : 1;
@@ -317,27 +321,27 @@ __global__ void __launch_bounds__(256) k_exchange(ExchArgs a) {
     r2 = r2 + v;
   }
   const int F = b.F[d];
-  double *rcv = b.recv[d];
+  double *rcv = b.recv[d] + (size_t)f * 6 * F;
   rcv[(ip * 3 + 0) * F + jk] = r0;
   rcv[(ip * 3 + 1) * F + jk] = r1;
   rcv[(ip * 3 + 2) * F + jk] = r2;
 }
 
 // ---- host launchers ----
-void launch_deriv(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a) {
-  if (d == 0) hipLaunchKernelGGL(k_deriv<0>, dim3(nwg), dim3(64), lds, st, a);
-  else if (d == 1) hipLaunchKernelGGL(k_deriv<1>, dim3(nwg), dim3(64), lds, st, a);
-  else hipLaunchKernelGGL(k_deriv<2>, dim3(nwg), dim3(64), lds, st, a);
+void launch_deriv(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a) {
+  if (d == 0) hipLaunchKernelGGL(k_deriv<0>, dim3(nwg, nf), dim3(64), lds, st, a);
+  else if (d == 1) hipLaunchKernelGGL(k_deriv<1>, dim3(nwg, nf), dim3(64), lds, st, a);
+  else hipLaunchKernelGGL(k_deriv<2>, dim3(nwg, nf), dim3(64), lds, st, a);
 }
 
-void launch_filter(int d, int nwg, size_t lds, hipStream_t st, const LineArgs &a) {
-  if (d == 0) hipLaunchKernelGGL(k_filter<0>, dim3(nwg), dim3(64), lds, st, a);
-  else if (d == 1) hipLaunchKernelGGL(k_filter<1>, dim3(nwg), dim3(64), lds, st, a);
-  else hipLaunchKernelGGL(k_filter<2>, dim3(nwg), dim3(64), lds, st, a);
+void launch_filter(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a) {
+  if (d == 0) hipLaunchKernelGGL(k_filter<0>, dim3(nwg, nf), dim3(64), lds, st, a);
+  else if (d == 1) hipLaunchKernelGGL(k_filter<1>, dim3(nwg, nf), dim3(64), lds, st, a);
+  else hipLaunchKernelGGL(k_filter<2>, dim3(nwg, nf), dim3(64), lds, st, a);
 }
 
-void launch_exchange(int nwg, hipStream_t st, const ExchArgs &a) {
-  hipLaunchKernelGGL(k_exchange, dim3(nwg), dim3(256), 0, st, a);
+void launch_exchange(int nwg, int nf, hipStream_t st, const ExchArgs &a) {
+  hipLaunchKernelGGL(k_exchange, dim3(nwg, nf), dim3(256), 0, st, a);
 }
 
 }  // namespace tp
