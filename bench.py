@@ -277,14 +277,15 @@ def cpu_baseline(seconds_target=15.0):
                       f"{steps} steps, {dt:.1f} s"}
 
 
-def measured_traffic():
-    """HBM bytes per derivative launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE
-    passes summarised under profiles/ (counter values scaled by a copy of
-    known size); None when no such summary is present."""
+def measured_traffic_ratio():
+    """HBM bytes moved by the derivative line solves of one time step over
+    their algorithmic bytes, from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+    summarised under profiles/ (counter values scaled by a copy of known
+    size); None when no such summary is present."""
     path = os.path.join(ROOT, "profiles", "line_solve_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)["bytes_per_launch"]
+            return json.load(f)["traffic_over_algorithmic"]
     except (OSError, ValueError, KeyError):
         return None
 
@@ -377,12 +378,15 @@ def main():
     ms, nl, by = prof["deriv"]
     if nl > 0 and ms > 0:
         avg_s = ms / 1e3 / nl
-        # 16 B/pt per launch (field in, derivative out), 24 B/pt for the
+        # 16 B/pt per field (field in, derivative out), 24 B/pt for the
         # launches that accumulate into their output (the eta and zeta flux
-        # derivatives): the library counts these per launch
+        # derivatives), times the fields of the launch (4 gradients, 5
+        # fluxes): the library counts these per launch
         achieved = by / (ms / 1e3) / 1e9
+        ratio = measured_traffic_ratio()
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(ratio * by / nl) if ratio else None,
                 "kernel": "compact derivative line solves (k_col / k_deriv, 3 directions)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl,
                 "bytes_per_launch": round(by / nl)}
