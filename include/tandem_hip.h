@@ -198,6 +198,36 @@ int32_t tp_inflow_modes(int32_t nmodes, uint64_t seed, double tu, double lturb, 
  * reference. */
 int32_t tp_gust_mode(double tu, double kr, const double *uoo3, double *out8, double *slit);
 
+/* ---- surface statistics on the device ----
+ * The wall of a case is the eta face of each sub-block with the wall code
+ * (20 + 5 nviscous), as src/rpt.f90:774-864 (wallArea, walldir) sets it up:
+ * cell areas |dr/dxi x dr/dzeta| halved at the parent block's xi and zeta
+ * ends (faces that are not BC 40 splits), unit normals along the eta metric
+ * into the fluid, points ordered xi fastest.  At the first stage of every
+ * step whose n is a multiple of the interval, a sample takes
+ *   - the force coefficients of src/rpt.f90:1017-1056 (clpost): pressure
+ *     p n area and wall shear rho (tau . n) area summed over the wall, times
+ *     2 / (M^2 span), turned into lift and drag with the angle of attack as
+ *     src/Main3D.f90:700-704 prints them (Cl, Cd);
+ *   - p at every wall point.
+ * Time means are trapezoidal integrals over the sampled times divided by the
+ * window, the running form of the time averages of src/rptPost.f90:870-925
+ * (p3dStats); rms is the square root of mean(p^2) - mean(p)^2.  Sampling
+ * runs inside the captured step, so graph replays sample too. */
+/* start (or restart, clearing the accumulators) sampling every `every`
+ * steps from the next step on; every = 0 stops sampling */
+int32_t tp_stats_start(tp_context *ctx, int32_t case_id, int32_t every);
+/* out8 = samples, first and last sample time, last Cl and Cd, their time
+ * means, window length (means equal the last values for one sample) */
+int32_t tp_stats_forces(tp_context *ctx, int32_t case_id, double *out8);
+/* wall points of a sub-block (0: it has no wall face; -1: bad arguments) */
+int64_t tp_stats_wall_count(tp_context *ctx, int32_t case_id, int32_t block);
+/* per wall point: time mean and rms of p, cell area, unit normal [3][count];
+ * any output may be NULL; count must equal tp_stats_wall_count.  Cp is
+ * (mean p - 1/gamma) 2 / M^2, src/rptPost.f90:693-694. */
+int32_t tp_stats_wall(tp_context *ctx, int32_t case_id, int32_t block, double *mean_p, double *rms_p,
+                      double *area, double *wnor3, int64_t count);
+
 /* ---- cases joining and leaving a running context ----
  * No counterpart in the reference (one case per MPI job there): these serve
  * ensembles and the request loop below.  Cases on the device keep their state

@@ -56,6 +56,10 @@ def lib():
         L.orc_filte.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_stage.argtypes = [P, ctypes.c_int]
         L.orc_step.argtypes = [P, ctypes.c_int]
+        L.orc_stats_start.argtypes = [P, ctypes.c_int]
+        L.orc_stats_forces.argtypes = [P, dp]
+        L.orc_stats_wall.argtypes = [P, ctypes.c_int, dp, dp, dp, dp]
+        L.orc_stats_wall.restype = ctypes.c_int
         L.orc_penta.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_fcint.argtypes = [ctypes.c_double, ctypes.c_double, dp]
         L.orc_fcbcm.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp, dp, dp]
@@ -122,6 +126,25 @@ class Oracle:
         il = np.zeros(5, np.int32); tl = np.zeros(5); w = np.zeros(5)
         on = lib().orc_inflow_weights(self.h, float(timo), float(dtk), _ip(il), _dp(tl), _dp(w))
         return (il, tl, w) if on else None
+
+    def stats_start(self, every):
+        """Sample the surface statistics every `every` steps (0: off)."""
+        lib().orc_stats_start(self.h, int(every))
+
+    def stats_forces(self):
+        o = np.zeros(8)
+        lib().orc_stats_forces(self.h, _dp(o))
+        return dict(samples=int(o[0]), t0=o[1], t1=o[2], cl=o[3], cd=o[4], cl_mean=o[5], cd_mean=o[6],
+                    window=o[7])
+
+    def stats_wall(self, b):
+        """(mean p, rms p, area, unit normal (3, n)) per wall point of block b, or None."""
+        n = lib().orc_stats_wall(self.h, int(b), None, None, None, None)
+        if n <= 0:
+            return None
+        m = np.zeros(n); r = np.zeros(n); a = np.zeros(n); w = np.zeros(3 * n)
+        lib().orc_stats_wall(self.h, int(b), _dp(m), _dp(r), _dp(a), _dp(w))
+        return m, r, a, w.reshape(3, n)
 
     def __del__(self):
         if getattr(self, "h", None):

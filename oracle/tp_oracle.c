@@ -116,6 +116,8 @@ typedef struct {
   int *lctz;        /* their sponge-list positions */
   double *atz, *vito;   /* ramp; table vito(0:ltz, 0:ntz, 3) */
   double xbc_ra0;   /* state of the global ra0 seen by extrabcc */
+  int nsurf, sside; /* wall surface points (eta face), wall side (0 j = 0, 1 j = let) */
+  double *sarea, *snor, *sst;   /* area(0:lcwall), wnor(0:lcwall,3), stats p_prev, S1, S2 */
 } oblk;
 
 struct orc {
@@ -137,6 +139,8 @@ struct orc {
   double sbci[MBCI], cbca_ra0;
   /* scalars */
   double sc[ORC_S_NSCALAR];
+  /* surface statistics: every, t0, t_prev, samples, Cl, Cd, trapezoid sums of Cl, Cd */
+  double st[8];
 };
 
 #define IDX3(B, i, j, k, nn)                                                   \
@@ -725,9 +729,10 @@ void orc_destroy(orc *c) {
     free(B->xu); free(B->xl); free(B->yu); free(B->yl);
     free(B->rpex); free(B->sbcc); free(B->lcsz); free(B->asz); free(B->bsz);
     free(B->lctz); free(B->atz); free(B->vito);
+    free(B->sarea); free(B->snor); free(B->sst);
   }
-  free(c->b); free(c->jct); free(c->jvel); free(c);
   free(c->modes);
+  free(c->b); free(c->jct); free(c->jvel); free(c);
 }
 
 int orc_npoints(const orc *c, int b) { return c->b[b].N; }
@@ -1169,6 +1174,41 @@ void orc_metrics(orc *c) {
           }
       }
     }
+    /* wall surface, src/rpt.f90:774-864 (wallArea, walldir): the eta face of
+     * the block that is a wall; points in the reference's lwall order (xi
+     * fastest), cell areas from the coordinate derivatives still held in
+     * qo, qa, de, halved at the parent block's xi and zeta ends (the faces
+     * that are not splits of the parent, BC 40), unit normals from etm
+     * pointing into the fluid */
+    {
+      const int bcw = 20 + 5 * c->nviscous;
+      int j = -1;
+      double coef = one;
+      if (B->nbc[1][0] == bcw) { j = 0; coef = one; B->sside = 0; }
+      else if (B->nbc[1][1] == bcw) { j = B->let; coef = -one; B->sside = 1; }
+      if (j >= 0) {
+        int nwl = (B->lxi + 1) * (B->lze + 1), ll = -1;
+        free(B->sarea); free(B->snor); free(B->sst);
+        B->nsurf = nwl;
+        B->sarea = dalloc(nwl); B->snor = dalloc(3 * (size_t)nwl); B->sst = dalloc(3 * (size_t)nwl);
+        for (int k = 0; k <= B->lze; k++)
+          for (int i = 0; i <= B->lxi; i++) {
+            int l = IDX3(B, j, k, i, 1);
+            double g11 = qo[l] * qo[l] + qa[l] * qa[l] + de[l] * de[l];
+            double g33 = qo[2 * N + l] * qo[2 * N + l] + qa[2 * N + l] * qa[2 * N + l] + de[2 * N + l] * de[2 * N + l];
+            double g13 = qo[l] * qo[2 * N + l] + qa[l] * qa[2 * N + l] + de[l] * de[2 * N + l];
+            ll = ll + 1;
+            double ar = sqrt(g11 * g33 - g13 * g13);
+            if (i == 0 && B->nbc[0][0] != 40) ar = ar * half;
+            else if (i == B->lxi && B->nbc[0][1] != 40) ar = ar * half;
+            if (k == 0 && B->nbc[2][0] != 40) ar = ar * half;
+            else if (k == B->lze && B->nbc[2][1] != 40) ar = ar * half;
+            B->sarea[ll] = ar;
+            double tmp = coef / sqrt(etm[l] * etm[l] + etm[N + l] * etm[N + l] + etm[2 * N + l] * etm[2 * N + l]);
+            for (int m = 0; m < 3; m++) B->snor[(size_t)m * nwl + ll] = etm[(size_t)m * N + l] * tmp;
+          }
+      }
+    }
     /* GCBC weights and wall flags, src/Main3D.f90:460-480 */
     int nw = 0, ns = 0;
     for (int nn = 0; nn < 3; nn++)
@@ -1366,6 +1406,110 @@ static void viscous_grad(orc *c, int m, int src_de, double **dst) {
   }
 }
 
+/* One sample of the surface statistics, when sampling is on and n is a
+ * multiple of its interval: the force coefficients of src/rpt.f90:1017-1056
+ * (clpost: pressure and wall-shear force over the wall cells, per block
+ * (clp + clv) dynp with dynp = 2/(M^2 span), summed over the blocks as the
+ * reference's MPI_REDUCE does; wall shear tw = rho (tau . n) in the
+ * reference's form), turned into lift and drag with the angle of attack as
+ * src/Main3D.f90:700-704 prints them; and trapezoidal time integrals, over
+ * the sampled times, of Cl, Cd and of p and p^2 at every wall point -- the
+ * running form of the time averages of src/rptPost.f90:870-925 (p3dStats). */
+static void wall_sample(orc *c) {
+  double *st = c->st, *sc = c->sc;
+  if (!(st[0] > 0) || fmod(sc[ORC_S_N], st[0]) != 0.0) return;
+  const double t = sc[ORC_S_TIMO];
+  const double dynp = two / (c->amachoo * c->amachoo * c->span);
+  double cl[2] = {zero, zero};
+  for (int b = 0; b < c->nb; b++) {
+    oblk *B = &c->b[b];
+    if (B->nsurf <= 0) continue;
+    const int N = B->N, nwl = B->nsurf, j = B->sside ? B->let : 0;
+    for (int dir = 0; dir < 2; dir++) {
+      double clp = zero, clv = zero;
+      int ll = -1;
+      for (int k = 0; k <= B->lze; k++)
+        for (int i = 0; i <= B->lxi; i++) {
+          int l = IDX3(B, j, k, i, 1);
+          ll = ll + 1;
+          clp = clp + (B->p[l] * B->snor[(size_t)dir * nwl + ll] * B->sarea[ll]);
+        }
+      if (c->nviscous == 1) {
+        ll = -1;
+        for (int k = 0; k <= B->lze; k++)
+          for (int i = 0; i <= B->lxi; i++) {
+            int l = IDX3(B, j, k, i, 1);
+            ll = ll + 1;
+            const double *tv = B->tv;
+            double n1 = B->snor[ll], n2 = B->snor[(size_t)nwl + ll], n3 = B->snor[2 * (size_t)nwl + ll];
+            double tw = dir == 0 ? B->qa[l] * (tv[l] * n1 + tv[3 * N + l] * n2 + tv[5 * N + l] * n3)
+                                 : B->qa[l] * (tv[3 * N + l] * n1 + tv[N + l] * n2 + tv[4 * N + l] * n3);
+            clv = clv + tw * B->sarea[ll];
+          }
+      }
+      cl[dir] = cl[dir] + (clp + clv) * dynp;
+    }
+    /* p statistics per wall point */
+    int ll = -1;
+    for (int k = 0; k <= B->lze; k++)
+      for (int i = 0; i <= B->lxi; i++) {
+        int l = IDX3(B, j, k, i, 1);
+        ll = ll + 1;
+        double p = B->p[l], *ps = B->sst;
+        if (st[3] > 0) {
+          double dt = t - st[2];
+          ps[(size_t)nwl + ll] = ps[(size_t)nwl + ll] + half * dt * (ps[ll] + p);
+          ps[2 * (size_t)nwl + ll] = ps[2 * (size_t)nwl + ll] + half * dt * (ps[ll] * ps[ll] + p * p);
+        }
+        ps[ll] = p;
+      }
+  }
+  double raoa = sc[ORC_S_AOA] * PI / 180;
+  double clift = cl[1] * cos(raoa) - cl[0] * sin(raoa);
+  double cdrag = cl[1] * sin(raoa) + cl[0] * cos(raoa);
+  if (st[3] > 0) {
+    double dt = t - st[2];
+    st[6] = st[6] + half * dt * (st[4] + clift);
+    st[7] = st[7] + half * dt * (st[5] + cdrag);
+  } else {
+    st[1] = t;
+  }
+  st[4] = clift; st[5] = cdrag; st[2] = t; st[3] = st[3] + 1;
+}
+
+void orc_stats_start(orc *c, int every) {
+  for (int q = 0; q < 8; q++) c->st[q] = zero;
+  c->st[0] = every;
+  for (int b = 0; b < c->nb; b++)
+    if (c->b[b].nsurf > 0) memset(c->b[b].sst, 0, sizeof(double) * 3 * (size_t)c->b[b].nsurf);
+}
+
+void orc_stats_forces(const orc *c, double *out8) {
+  const double *st = c->st;
+  double T = st[2] - st[1];
+  out8[0] = st[3]; out8[1] = st[1]; out8[2] = st[2]; out8[3] = st[4]; out8[4] = st[5];
+  out8[5] = T > 0 ? st[6] / T : st[4];
+  out8[6] = T > 0 ? st[7] / T : st[5];
+  out8[7] = T;
+}
+
+int orc_stats_wall(const orc *c, int b, double *mean_p, double *rms_p, double *area, double *wnor3) {
+  const oblk *B = &c->b[b];
+  const int nwl = B->nsurf;
+  if (nwl <= 0) return 0;
+  double T = c->st[2] - c->st[1];
+  for (int ll = 0; ll < nwl; ll++) {
+    double m = T > 0 ? B->sst[(size_t)nwl + ll] / T : B->sst[ll];
+    double v = T > 0 ? B->sst[2 * (size_t)nwl + ll] / T - m * m : zero;
+    if (mean_p) mean_p[ll] = m;
+    if (rms_p) rms_p[ll] = sqrt(v > zero ? v : zero);
+    if (area) area[ll] = B->sarea[ll];
+    if (wnor3)
+      for (int q = 0; q < 3; q++) wnor3[(size_t)q * nwl + ll] = B->snor[(size_t)q * nwl + ll];
+  }
+  return nwl;
+}
+
 void orc_stage(orc *c, int nk) {
   int nkrk = c->nkrk;
   double *sc = c->sc;
@@ -1504,6 +1648,8 @@ void orc_stage(orc *c, int nk) {
       }
     }
   }
+  /* surface forces and pressure statistics at the first stage, src/Main3D.f90:695-706 */
+  if (nk == 1) wall_sample(c);
   /* fluxes, src/Main3D.f90:710-768 */
   for (int b = 0; b < c->nb; b++) {
     oblk *B = &c->b[b];

@@ -78,6 +78,14 @@ int orc_npoints(const orc *c, int b);
 int orc_count(const orc *c, int b, int which); /* element count of an array */
 double *orc_array(orc *c, int b, int which);
 double *orc_scalars(orc *c);
+/* surface statistics (src/rpt.f90:774-864, 1017-1056; src/rptPost.f90:870-925):
+ * sample every `every` steps at the first stage (0: off), from the next step */
+void orc_stats_start(orc *c, int every);
+/* out8 = samples, first and last sample time, last Cl, Cd, their time means, window */
+void orc_stats_forces(const orc *c, double *out8);
+/* per wall point of block b (xi fastest): time mean and rms of p, cell area,
+ * unit normal [3][n]; returns n (0: no wall face); any pointer may be NULL */
+int orc_stats_wall(const orc *c, int b, double *mean_p, double *rms_p, double *area, double *wnor3);
 void orc_set_threads(int n);
 
 /* setup: coefficient tables (module-global in the reference) */

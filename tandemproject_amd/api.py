@@ -34,7 +34,8 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_remove_case", "tp_case_info", "tp_server_create", "tp_server_destroy",
            "tp_server_context", "tp_server_error", "tp_server_clock", "tp_server_submit",
            "tp_server_iterate", "tp_server_run", "tp_server_result", "tp_server_result_count",
-           "tp_server_stats", "tp_server_set_fill"]
+           "tp_server_stats", "tp_server_set_fill", "tp_stats_start", "tp_stats_forces",
+           "tp_stats_wall_count", "tp_stats_wall"]
 
 _lib = None
 
@@ -84,6 +85,8 @@ def load(path=LIB_PATH):
         "tp_server_iterate": [P, i32, pi32], "tp_server_run": [P, i32],
         "tp_server_result": [P, i32, pd, i64, pd],
         "tp_server_stats": [P, pd], "tp_server_set_fill": [P, i64],
+        "tp_stats_start": [P, i32, i32], "tp_stats_forces": [P, i32, pd],
+        "tp_stats_wall": [P, i32, i32, pd, pd, pd, pd, i64],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -99,6 +102,8 @@ def load(path=LIB_PATH):
     L.tp_server_clock.restype = dbl
     L.tp_server_result_count.argtypes = [P, i32]
     L.tp_server_result_count.restype = i64
+    L.tp_stats_wall_count.argtypes = [P, i32, i32]
+    L.tp_stats_wall_count.restype = i64
     _lib = L
     return L
 
@@ -272,6 +277,29 @@ class Solver:
         for k, v in values.items():
             s[SCALARS.index(k)] = v
         _check(load().tp_set_scalars(self.h, case, _dp(s)))
+
+    # ---- surface statistics ----
+    def stats_start(self, case=0, every=1):
+        """Sample wall forces and pressure every `every` steps (0: stop)."""
+        _check(load().tp_stats_start(self.h, case, int(every)))
+
+    def stats_forces(self, case=0):
+        o = np.zeros(8)
+        _check(load().tp_stats_forces(self.h, case, _dp(o)))
+        return dict(samples=int(o[0]), t0=o[1], t1=o[2], cl=o[3], cd=o[4], cl_mean=o[5], cd_mean=o[6],
+                    window=o[7])
+
+    def stats_wall(self, case, block):
+        """(mean p, rms p, area, unit normal (3, n)) per wall point, or None."""
+        L = load()
+        n = int(L.tp_stats_wall_count(self.h, case, block))
+        if n < 0:
+            raise TPError("bad case/block")
+        if n == 0:
+            return None
+        m = np.zeros(n); r = np.zeros(n); a = np.zeros(n); w = np.zeros(3 * n)
+        _check(L.tp_stats_wall(self.h, case, block, _dp(m), _dp(r), _dp(a), _dp(w), n))
+        return m, r, a, w.reshape(3, n)
 
     # ---- reference routines (all sub-blocks) ----
     def mpigo(self, nt, nrt, n45):

@@ -430,8 +430,10 @@ struct ColGroup {
 
 struct ListSet {
   DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  DevList surf;                     // wall-surface points, 256 per item
   std::vector<ColGroup> cg[3][2];   // line-solve launches per direction and kind
   std::vector<int> wr;   // first point-work item of each linked case, then the end
+  std::vector<int> wwr;  // first wall-surface item of each linked case, then the end
 };
 
 struct ProfRec {
@@ -452,8 +454,8 @@ struct tp_context {
   Mem link;
   Block *d_blocks = nullptr;
   CaseCoef *d_coef = nullptr;
-  double *d_sc = nullptr, *d_part = nullptr;
-  int *d_wr = nullptr, *d_jct = nullptr, *d_jvel = nullptr;
+  double *d_sc = nullptr, *d_part = nullptr, *d_wpart = nullptr;
+  int *d_wr = nullptr, *d_wwr = nullptr, *d_jct = nullptr, *d_jvel = nullptr;
   int njct = 0, maxk = 0;
   ListSet L;
   int nlp[3] = {0, 0, 0};
@@ -562,6 +564,7 @@ StageArgs base_args(tp_context *c) {
   a.coef = c->d_coef;
   a.sc = c->d_sc;
   a.part = c->d_part;
+  a.wpart = c->d_wpart;
   return a;
 }
 
@@ -711,6 +714,12 @@ void enqueue_stage(tp_context *c, int nk) {
     run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3);
     run_points(c, PK_GRADSTRESS, a, K_VISC, 8.0 * (26 + 10));
   }
+  if (nk == 1 && c->L.surf.n) {
+    // surface forces and pressure statistics, src/Main3D.f90:695-706 (the
+    // kernels return at once for cases that do not sample this step)
+    run_faces(c, c->L.surf, FK_SURF_SAMPLE, a, K_MISC);
+    launch_surf_reduce(c->d_coef, c->d_sc, c->d_wpart, c->d_wwr, ncase, c->stream);
+  }
   // src/Main3D.f90:710-768: all 15 flux components in one pass, then
   // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
   run_points(c, PK_FLUXALL, a, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 18 + 15));
@@ -825,7 +834,7 @@ namespace {
 
 // sub-array offsets of one block's arena (256-byte aligned)
 struct Arena {
-  size_t qa, qo, de, rr, ss, xi, et, ze, p, ya, tv, fx, f[3][4], lud, luf, rp, total;
+  size_t qa, qo, de, rr, ss, xi, et, ze, p, ya, tv, fx, f[3][4], lud, luf, rp, sa, sn, ss2, total;
 };
 
 Arena arena_layout(const Block &b, int nwall) {
@@ -841,6 +850,7 @@ Arena arena_layout(const Block &b, int nwall) {
     A.f[d][2] = take(6 * RCV_SLOTS * (size_t)b.F[d]); A.f[d][3] = take(6 * (size_t)b.F[d]);
   }
   A.lud = take(5 * (size_t)b.nlu); A.luf = take(5 * (size_t)b.nlu); A.rp = take(std::max(nwall, 1));
+  A.sa = take(b.nsurf); A.sn = take(3 * (size_t)b.nsurf); A.ss2 = take(3 * (size_t)b.nsurf);
   A.total = off;
   return A;
 }
@@ -871,6 +881,12 @@ Block block_shape(const HostCase &h, int lb, int *nwall) {
       else b.wall_off[d][s] = -1;
     }
   *nwall = nw;
+  // the wall surface of src/rpt.f90:774-830: the eta face with the wall code
+  const int bcw = 20 + 5 * h.ip[TP_I_NVISCOUS];
+  b.nsurf = 0;
+  b.sside = 0;
+  if (b.nbc[1][0] == bcw) { b.nsurf = b.nx * b.nz; b.sside = 0; }
+  else if (b.nbc[1][1] == bcw) { b.nsurf = b.nx * b.nz; b.sside = 1; }
   return b;
 }
 
@@ -1012,6 +1028,7 @@ int build_case(tp_context *c, HostCase &h) {
       b.drva[d] = P(A.f[d][0]); b.drvb[d] = P(A.f[d][1]); b.recv[d] = P(A.f[d][2]); b.cm[d] = P(A.f[d][3]);
     }
     b.lud = P(A.lud); b.luf = P(A.luf); b.rpex = P(A.rp);
+    b.sarea = P(A.sa); b.snor = P(A.sn); b.sst = P(A.ss2);
     std::vector<double> lud(5 * b.nlu), luf(5 * b.nlu);
     const int nl = b.nlu;
     for (int d = 0; d < 3; d++) {
@@ -1050,12 +1067,18 @@ int build_case(tp_context *c, HostCase &h) {
 
 // launch tables over the global blocks gs (sorted by case)
 int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
-  std::vector<Work> pts, sp, lw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm;
+  std::vector<Work> pts, sp, lw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm, sf;
   L.wr.assign(c->linked.size() + 1, 0);
+  L.wwr.assign(c->linked.size() + 1, 0);
   int slot = -1;
   for (int g : gs) {
     const Block &b = c->blocks[g];
-    while (slot < b.cs) L.wr[++slot] = (int)pts.size();
+    while (slot < b.cs) {
+      ++slot;
+      L.wr[slot] = (int)pts.size();
+      L.wwr[slot] = (int)sf.size();
+    }
+    for (int a0 = 0; a0 < b.nsurf; a0 += 256) sf.push_back({g, a0, std::min(256, b.nsurf - a0), 1});
     const int CH = 1024;
     for (int a0 = 0; a0 < b.N; a0 += CH) pts.push_back({g, a0, std::min(CH, b.N - a0), 0});
     for (int a0 = 0; a0 < b.nsponge; a0 += 1024) sp.push_back({g, a0, std::min(1024, b.nsponge - a0), 0});
@@ -1081,8 +1104,13 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
         add(nm);
       }
   }
-  while (slot < (int)c->linked.size()) L.wr[++slot] = (int)pts.size();
+  while (slot < (int)c->linked.size()) {
+    ++slot;
+    L.wr[slot] = (int)pts.size();
+    L.wwr[slot] = (int)sf.size();
+  }
   int rc;
+  if ((rc = upload_list(m, L.surf, sf))) return rc;
   if ((rc = upload_list(m, L.points, pts))) return rc;
   if ((rc = upload_list(m, L.sponge, sp))) return rc;
   if ((rc = upload_list(m, L.exch, ex))) return rc;
@@ -1117,8 +1145,8 @@ int link(tp_context *c) {
   drop_graph(c);
   c->link.release();
   c->lmem.release();
-  c->d_blocks = nullptr; c->d_coef = nullptr; c->d_sc = nullptr; c->d_part = nullptr;
-  c->d_wr = nullptr; c->d_jct = nullptr; c->d_jvel = nullptr;
+  c->d_blocks = nullptr; c->d_coef = nullptr; c->d_sc = nullptr; c->d_part = nullptr; c->d_wpart = nullptr;
+  c->d_wr = nullptr; c->d_wwr = nullptr; c->d_jct = nullptr; c->d_jvel = nullptr;
   c->L = ListSet();
   c->blocks.clear(); c->bsrc.clear(); c->linked.clear();
   c->npts = 0; c->any_visc = false; c->njct = 0; c->maxk = 0;
@@ -1176,6 +1204,9 @@ int link(tp_context *c) {
   std::vector<double> part(2 * std::max(c->L.points.n, 1), 0.0);
   if ((rc = upload_vec(c->link, &c->d_part, part))) return rc;
   if ((rc = upload_vec(c->link, &c->d_wr, c->L.wr))) return rc;
+  if ((rc = upload_vec(c->link, &c->d_wwr, c->L.wwr))) return rc;
+  std::vector<double> wpart(4 * std::max(c->L.surf.n, 1), 0.0);
+  if ((rc = upload_vec(c->link, &c->d_wpart, wpart))) return rc;
   // junctions
   std::vector<int> jct, jvel;
   for (const HostCase &h : c->cases) {
@@ -1406,6 +1437,7 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
   k.uoo[0] = m1 * aoo; k.uoo[1] = m2 * aoo; k.uoo[2] = m3 * aoo;
   k.nkrk = iparams[TP_I_NKRK]; k.nviscous = iparams[TP_I_NVISCOUS]; k.nsmf = iparams[TP_I_NSMF];
   k.les = iparams[TP_I_LES]; k.nextgcic = iparams[TP_I_NEXTGCIC];
+  k.span = dparams[TP_D_SPAN];
   k.inflow = iparams[TP_I_INFLOW];
   if (k.inflow < 0 || k.inflow > 2) return fail(TP_E_ARG, "TP_I_INFLOW must be 0, 1 or 2");
   if (k.inflow) {
@@ -1495,6 +1527,7 @@ int32_t tp_setup_geometry(tp_context *c) {
     }
     run_points(c, PK_METRICS, base_args(c), K_MISC, 0.0);
     run_faces(c, c->L.normals, FK_NORMALS, base_args(c), K_MISC);
+    run_faces(c, c->L.surf, FK_SURF_SETUP, base_args(c), K_MISC);
     std::swap(c->L, sub);
     const hipError_t e = hipStreamSynchronize(c->stream);
     tmp.release();
@@ -1952,6 +1985,74 @@ int32_t tp_case_info(tp_context *c, int32_t cs, int64_t *out4) {
   out4[1] = np;
   out4[2] = h.nb;
   out4[3] = (int64_t)h.mem.bytes;
+  return 0;
+}
+
+int32_t tp_stats_start(tp_context *c, int32_t cs, int32_t every) {
+  if (int rc = check_ready(c)) return rc;
+  HostCase *h = linked_case(c, cs);
+  if (!h || every < 0) return fail(TP_E_ARG, "tp_stats_start: bad case or interval");
+  double s[S_STRIDE];
+  HIPCHK(hipMemcpyAsync(s, c->d_sc + h->slot * S_STRIDE, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  s[S_ST_EVERY] = every;
+  s[S_ST_T0] = s[S_ST_TP] = s[S_ST_NS] = 0.0;
+  s[S_CL] = s[S_CD] = s[S_CLS] = s[S_CDS] = 0.0;
+  HIPCHK(hipMemcpyAsync(c->d_sc + h->slot * S_STRIDE, s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  for (const Block &b : h->blk)
+    if (b.nsurf > 0) HIPCHK(hipMemsetAsync(b.sst, 0, sizeof(double) * 3 * (size_t)b.nsurf, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int32_t tp_stats_forces(tp_context *c, int32_t cs, double *out8) {
+  if (int rc = check_ready(c)) return rc;
+  const HostCase *h = linked_case(c, cs);
+  if (!h || !out8) return fail(TP_E_ARG, "tp_stats_forces: bad case");
+  double s[S_STRIDE];
+  HIPCHK(hipMemcpyAsync(s, c->d_sc + h->slot * S_STRIDE, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double T = s[S_ST_TP] - s[S_ST_T0];
+  out8[0] = s[S_ST_NS]; out8[1] = s[S_ST_T0]; out8[2] = s[S_ST_TP];
+  out8[3] = s[S_CL]; out8[4] = s[S_CD];
+  out8[5] = T > 0 ? s[S_CLS] / T : s[S_CL];
+  out8[6] = T > 0 ? s[S_CDS] / T : s[S_CD];
+  out8[7] = T;
+  return 0;
+}
+
+int64_t tp_stats_wall_count(tp_context *c, int32_t cs, int32_t lb) {
+  if (check_ready(c)) return -1;
+  const int g = block_index(c, cs, lb);
+  return g < 0 ? -1 : c->blocks[g].nsurf;
+}
+
+int32_t tp_stats_wall(tp_context *c, int32_t cs, int32_t lb, double *mean_p, double *rms_p, double *area,
+                      double *wnor3, int64_t count) {
+  if (int rc = check_ready(c)) return rc;
+  const int g = block_index(c, cs, lb);
+  if (g < 0) return fail(TP_E_ARG, "tp_stats_wall: bad case/block");
+  const Block &b = c->blocks[g];
+  const int nw = b.nsurf;
+  if (count != nw) return fail(TP_E_SIZE, "tp_stats_wall: the block has " + std::to_string(nw) + " wall points");
+  if (nw == 0) return 0;
+  const HostCase *h = linked_case(c, cs);
+  double s[S_STRIDE];
+  std::vector<double> st(3 * (size_t)nw), ar(nw), nr(3 * (size_t)nw);
+  HIPCHK(hipMemcpyAsync(s, c->d_sc + h->slot * S_STRIDE, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(st.data(), b.sst, sizeof(double) * st.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(ar.data(), b.sarea, sizeof(double) * ar.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(nr.data(), b.snor, sizeof(double) * nr.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double T = s[S_ST_TP] - s[S_ST_T0];
+  for (int ll = 0; ll < nw; ll++) {
+    const double m = T > 0 ? st[nw + ll] / T : st[ll];
+    const double v = T > 0 ? st[2 * (size_t)nw + ll] / T - m * m : 0.0;
+    if (mean_p) mean_p[ll] = m;
+    if (rms_p) rms_p[ll] = std::sqrt(v > 0.0 ? v : 0.0);
+    if (area) area[ll] = ar[ll];
+  }
+  if (wnor3) std::memcpy(wnor3, nr.data(), sizeof(double) * nr.size());
   return 0;
 }
 

@@ -27,7 +27,12 @@ enum Scalar {
   S_N = 0, S_NDT, S_DT, S_DTS, S_DTE, S_TIMO, S_UMF1, S_UMF2, S_UMF3,
   S_DUDT1, S_DUDT2, S_DUDT3, S_DTK, S_DTKO, S_AOA, S_NSCALAR = 16,
   // internal: weights and table rows of the inflow's time interpolation for the current stage
-  S_LAGW = 16, S_LAGI = 21, S_LAGON = 26, S_STRIDE = 32
+  S_LAGW = 16, S_LAGI = 21, S_LAGON = 26,
+  // surface statistics: sampling interval (0 off), first and previous sample
+  // time, samples taken, last Cl, Cd, trapezoid integrals of Cl, Cd
+  S_ST_EVERY = 27, S_ST_T0 = 28, S_ST_TP = 29, S_ST_NS = 30,
+  S_CL = 32, S_CD = 33, S_CLS = 34, S_CDS = 35,
+  S_STRIDE = 48
 };
 
 // ---- per-case constants (scheme coefficients and physics) ----
@@ -45,6 +50,7 @@ struct CaseCoef {
   // turbulent inflow through the sponge target (SingleTurbulence3D.f90:332-372)
   int inflow, ntz;
   double tgusto, period, fctr;   // table period slit/uoo(1) and spacing slit/(ntz uoo(1))
+  double span;                   // spanwise extent (force coefficients' reference area per chord)
 };
 
 // ---- device view of one sub-block ----
@@ -70,6 +76,11 @@ struct Block {
   int *gix;                   // per sponge point: its inflow index, -1 if none
   double *atz;                // per inflow point: ramp sin^2 of the reference
   double *vito;               // [3][ntz+1][ngust] inflow velocity table
+  // wall surface (the eta face that is a wall): points ll = k*nx + i
+  int nsurf, sside;           // wall points (0: none), side (0: j = 0, 1: j = ny-1)
+  double *sarea;              // [nsurf] cell areas (src/rpt.f90:774-830, wallArea)
+  double *snor;               // [3][nsurf] unit normals into the fluid (walldir)
+  double *sst;                // [3][nsurf] previous sampled p, trapezoid integrals of p and p^2
   const double *col[3][2];    // chunked-solver carry maps per direction, kind (deriv, filter)
 };
 
@@ -141,6 +152,7 @@ struct StageArgs {
   const CaseCoef *coef;
   double *sc;        // [ncase][S_STRIDE]
   double *part;      // reduction partials [nwork][2]
+  double *wpart;     // wall-force partials [nwallwork][4]
   int nk, m, d, opt; // stage number, field, direction, option
   int arr, col;      // generic field selector
   int o0, o1, o2;    // tv slots for gradient combination (-1..-3 = ss columns)
@@ -166,9 +178,12 @@ void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream
 void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st);
 void launch_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk,
                      hipStream_t st);
+void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, const int *wwr, int ncase,
+                        hipStream_t st);
 
 enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS,
                    PK_GRADSTRESS = 13, PK_FLUXALL };
-enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS };
+enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,
+                  FK_SURF_SETUP, FK_SURF_SAMPLE };
 
 }  // namespace tp

@@ -827,6 +827,132 @@ __global__ void __launch_bounds__(256) k_normals(StageArgs a) {
 }
 
 // ---- host launchers ----
+// ---- wall surface, src/rpt.f90:774-864 (wallArea, walldir) ----
+// One thread per point ll = k*nx + i of the eta face that is a wall (the
+// reference's lwall order, xi fastest).  The cell area comes from the
+// coordinate derivatives the metrics set-up leaves in qo, qa, de (x, y, z
+// along xi and zeta), halved at the parent block's xi and zeta ends: the
+// faces that are not splits of the parent (BC 40).  The unit normal is the
+// eta metric, pointing into the fluid.
+__global__ void __launch_bounds__(256) k_surf_setup(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  const int t = threadIdx.x;
+  if (t >= w.n) return;
+  const Block &b = a.blocks[w.blk];
+  const int N = b.N, nw = b.nsurf, ll = w.a + t;
+  const int i = ll % b.nx, k = ll / b.nx, j = b.sside ? b.ny - 1 : 0;
+  const int l = (k * b.ny + j) * b.nx + i;
+  const double *qo = b.qo, *qa = b.qa, *de = b.de, *et = b.etm;
+  const double g11 = qo[l] * qo[l] + qa[l] * qa[l] + de[l] * de[l];
+  const double g33 = qo[2 * N + l] * qo[2 * N + l] + qa[2 * N + l] * qa[2 * N + l] + de[2 * N + l] * de[2 * N + l];
+  const double g13 = qo[l] * qo[2 * N + l] + qa[l] * qa[2 * N + l] + de[l] * de[2 * N + l];
+  double ar = sqrt(g11 * g33 - g13 * g13);
+  if (i == 0 && b.nbc[0][0] != 40) ar = ar * 0.5;
+  else if (i == b.nx - 1 && b.nbc[0][1] != 40) ar = ar * 0.5;
+  if (k == 0 && b.nbc[2][0] != 40) ar = ar * 0.5;
+  else if (k == b.nz - 1 && b.nbc[2][1] != 40) ar = ar * 0.5;
+  b.sarea[ll] = ar;
+  const double coef = b.sside ? -1.0 : 1.0;
+  const double tmp = coef / sqrt(et[l] * et[l] + et[N + l] * et[N + l] + et[2 * N + l] * et[2 * N + l]);
+  b.snor[ll] = et[l] * tmp;
+  b.snor[nw + ll] = et[N + l] * tmp;
+  b.snor[2 * nw + ll] = et[2 * N + l] * tmp;
+}
+
+// ---- one sample of the surface statistics at the first stage ----
+// When the case samples at this step (n a multiple of its interval): the
+// pressure and wall-shear force of each wall cell, p n area and
+// rho (tau . n) area as src/rpt.f90:1017-1056 (clpost) forms them, summed per
+// workgroup in a fixed order into four partials; and the trapezoidal time
+// integrals of p and p^2 at each wall point over the sampled times (the
+// running form of the time averages of src/rptPost.f90:870-925).
+__global__ void __launch_bounds__(256) k_surf_sample(StageArgs a) {
+  const Work w = a.work[blockIdx.x];
+  const Block &b = a.blocks[w.blk];
+  const CaseCoef &c = a.coef[b.cs];
+  const double *s = a.sc + b.cs * S_STRIDE;
+  const double every = s[S_ST_EVERY];
+  const bool on = every > 0 && fmod(s[S_N], every) == 0.0;
+  __shared__ double red[4][256];
+  double f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+  const int t = threadIdx.x;
+  if (on && t < w.n) {
+    const int N = b.N, nw = b.nsurf, ll = w.a + t;
+    const int i = ll % b.nx, k = ll / b.nx, j = b.sside ? b.ny - 1 : 0;
+    const int l = (k * b.ny + j) * b.nx + i;
+    const double p = b.p[l], ar = b.sarea[ll];
+    const double n1 = b.snor[ll], n2 = b.snor[nw + ll], n3 = b.snor[2 * nw + ll];
+    f0 = p * n1 * ar;
+    f1 = p * n2 * ar;
+    if (c.nviscous == 1) {
+      const double *tv = b.tv, rho = b.qa[l];
+      const double tw1 = rho * (tv[l] * n1 + tv[3 * N + l] * n2 + tv[5 * N + l] * n3);
+      const double tw2 = rho * (tv[3 * N + l] * n1 + tv[N + l] * n2 + tv[4 * N + l] * n3);
+      f2 = tw1 * ar;
+      f3 = tw2 * ar;
+    }
+    double *ps = b.sst;
+    if (s[S_ST_NS] > 0) {
+      const double dt = s[S_TIMO] - s[S_ST_TP];
+      ps[nw + ll] = ps[nw + ll] + 0.5 * dt * (ps[ll] + p);
+      ps[2 * nw + ll] = ps[2 * nw + ll] + 0.5 * dt * (ps[ll] * ps[ll] + p * p);
+    }
+    ps[ll] = p;
+  }
+  red[0][t] = f0; red[1][t] = f1; red[2][t] = f2; red[3][t] = f3;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h)
+      for (int q = 0; q < 4; q++) red[q][t] = red[q][t] + red[q][t + h];
+    __syncthreads();
+  }
+  if (t < 4) a.wpart[4 * blockIdx.x + t] = red[t][0];
+}
+
+// per case: the force coefficients from the partials of its wall work items,
+// lift and drag with the angle of attack (src/Main3D.f90:700-704), and their
+// trapezoidal time integrals
+__global__ void k_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, const int *wwr,
+                              int ncase) {
+  const int cs = blockIdx.x;
+  if (cs >= ncase) return;
+  double *s = sc + cs * S_STRIDE;
+  const double every = s[S_ST_EVERY];
+  if (!(every > 0 && fmod(s[S_N], every) == 0.0)) return;
+  __shared__ double red[4][256];
+  const int t = threadIdx.x;
+  double v[4] = {0, 0, 0, 0};
+  for (int q = wwr[cs] + t; q < wwr[cs + 1]; q += blockDim.x)
+    for (int r = 0; r < 4; r++) v[r] = v[r] + wpart[4 * q + r];
+  for (int r = 0; r < 4; r++) red[r][t] = v[r];
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int r = 0; r < 4; r++) red[r][t] = red[r][t] + red[r][t + h];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const CaseCoef &c = coef[cs];
+    const double dynp = 2.0 / (c.amachoo * c.amachoo * c.span);
+    const double cl1 = (red[0][0] + red[2][0]) * dynp, cl2 = (red[1][0] + red[3][0]) * dynp;
+    const double raoa = s[S_AOA] * PI_D / 180;
+    const double clift = cl2 * cos(raoa) - cl1 * sin(raoa);
+    const double cdrag = cl2 * sin(raoa) + cl1 * cos(raoa);
+    const double tm = s[S_TIMO];
+    if (s[S_ST_NS] > 0) {
+      const double dt = tm - s[S_ST_TP];
+      s[S_CLS] = s[S_CLS] + 0.5 * dt * (s[S_CL] + clift);
+      s[S_CDS] = s[S_CDS] + 0.5 * dt * (s[S_CD] + cdrag);
+    } else {
+      s[S_ST_T0] = tm;
+    }
+    s[S_CL] = clift;
+    s[S_CD] = cdrag;
+    s[S_ST_TP] = tm;
+    s[S_ST_NS] = s[S_ST_NS] + 1;
+  }
+}
+
 #define LAUNCH(K, NWG, BS, ST, A) hipLaunchKernelGGL(K, dim3(NWG), dim3(BS), 0, ST, A)
 void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
   switch (which) {
@@ -851,7 +977,13 @@ void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a) {
     case 4: LAUNCH(k_avg_capture, nwg, 256, st, a); break;
     case 5: LAUNCH(k_avg_apply, nwg, 256, st, a); break;
     case 6: LAUNCH(k_normals, nwg, 256, st, a); break;
+    case 7: LAUNCH(k_surf_setup, nwg, 256, st, a); break;
+    case 8: LAUNCH(k_surf_sample, nwg, 256, st, a); break;
   }
+}
+void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, const int *wwr, int ncase,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_surf_reduce, dim3(ncase), dim3(256), 0, st, coef, sc, wpart, wwr, ncase);
 }
 void launch_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st) {
   hipLaunchKernelGGL(k_sc_stage, dim3((ncase + 63) / 64), dim3(64), 0, st, coef, sc, ncase, nk);

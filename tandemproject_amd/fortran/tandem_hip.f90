@@ -212,6 +212,31 @@ module tandem_hip
       real(c_double), intent(in) :: uoo3(3)
       real(c_double), intent(out) :: out8(8), slit
     end function
+    ! surface statistics
+    integer(c_int32_t) function tp_stats_start(ctx, case_id, every) bind(C, name="tp_stats_start")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id, every
+    end function
+    integer(c_int32_t) function tp_stats_forces(ctx, case_id, out8) bind(C, name="tp_stats_forces")
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      real(c_double), intent(out) :: out8(8)
+    end function
+    integer(c_int64_t) function tp_stats_wall_count(ctx, case_id, block) bind(C, name="tp_stats_wall_count")
+      import :: c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id, block
+    end function
+    integer(c_int32_t) function tp_stats_wall(ctx, case_id, block, mean_p, rms_p, area, wnor3, count) &
+        bind(C, name="tp_stats_wall")
+      import :: c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id, block
+      real(c_double), intent(out) :: mean_p(*), rms_p(*), area(*), wnor3(*)
+      integer(c_int64_t), value :: count
+    end function
     ! cases joining and leaving a running context
     integer(c_int32_t) function tp_case_bytes(ctx, case_id, bytes) bind(C, name="tp_case_bytes")
       import :: c_ptr, c_int32_t, c_int64_t

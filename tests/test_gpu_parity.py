@@ -341,3 +341,44 @@ def test_inflow_gust_steps():
     o.step(40); s.step(40)
     s.sync()
     compare_state(o, s, lay, 1e-10)
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_surface_statistics(graph):
+    """Wall cell areas and normals (src/rpt.f90:774-864), the force
+    coefficients of clpost (:1017-1056) and the wall-pressure time averages,
+    sampled every 2 steps over 20 steps (inside the captured step for graph
+    replays), against the oracle's restatement."""
+    say("surface statistics, 20 steps sampled every 2,", "graph" if graph else "eager")
+    lay, ss, p = aerofoil_case()
+    o, s = geometry_pair(lay, ss, p, exact=True)
+    walls = 0
+    for b in range(lay.nb):
+        ro, rs = o.stats_wall(b), s.stats_wall(0, b)
+        assert (ro is None) == (rs is None), b
+        if ro is None:
+            continue
+        walls += 1
+        assert rel(rs[2], ro[2]) <= 1e-14 and rel(rs[3], ro[3]) <= 1e-14
+    assert walls == 2
+    o.initialo(); s.initialo(0)
+    o.step(3); s.step(3)
+    o.stats_start(2); s.stats_start(0, 2)
+    o.step(20); s.step(20, graph=graph)
+    s.sync()
+    fo, fs = o.stats_forces(), s.stats_forces(0)
+    say("  oracle", fo)
+    say("  device", fs)
+    assert fo["samples"] == fs["samples"] == 10
+    for k in ("t0", "t1", "window"):
+        assert abs(fo[k] - fs[k]) <= 1e-14 * abs(fo[k]), k
+    for k in ("cl", "cd", "cl_mean", "cd_mean"):
+        assert abs(fo[k] - fs[k]) <= 1e-12 * max(abs(fo["cl"]), abs(fo["cd"])), (k, fo[k], fs[k])
+    for b in range(lay.nb):
+        ro = o.stats_wall(b)
+        if ro is None:
+            continue
+        rs = s.stats_wall(0, b)
+        assert rel(rs[0], ro[0]) <= 1e-13
+        assert np.abs(rs[1] ** 2 - ro[1] ** 2).max() <= 1e-12 * np.abs(ro[0]).max() ** 2
+        assert ro[1].max() > 0
