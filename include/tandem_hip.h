@@ -198,6 +198,46 @@ int32_t tp_inflow_modes(int32_t nmodes, uint64_t seed, double tu, double lturb, 
  * reference. */
 int32_t tp_gust_mode(double tu, double kr, const double *uoo3, double *out8, double *slit);
 
+/* ---- the reference's file formats ----
+ * The reference writes one restart and one raw grid file per parent block
+ * (a block of its bkx x bky x bkz layout, cut into sub-blocks for MPI) and
+ * one PLOT3D file for all parent blocks.  tp_set_parents tells which parent
+ * each sub-block belongs to: parent[b], the sub-block's first point in it
+ * start3[b*3+d], and the parents' last indices pdims3[mb*3+d] (lxi, let, lze
+ * of the parent).  Without it every sub-block is its own parent. */
+int32_t tp_set_parents(tp_context *ctx, int32_t case_id, int32_t nparent, const int32_t *parent,
+                       const int32_t *start3, const int32_t *pdims3);
+/* the parent layout in use (outputs may be NULL; parent and start3 hold one
+ * entry per sub-block, pdims3 one per parent) */
+int32_t tp_get_parents(tp_context *ctx, int32_t case_id, int32_t *nparent, int32_t *parent,
+                       int32_t *start3, int32_t *pdims3);
+/* the grid of a sub-block as tp_set_grid gave it (count 3 * points) */
+int32_t tp_get_grid(tp_context *ctx, int32_t case_id, int32_t block, double *xyz, int64_t count);
+/* lxi, let, lze of every sub-block, and the case's iparams / dparams as added */
+int32_t tp_case_dims(tp_context *ctx, int32_t case_id, int32_t *dims3);
+int32_t tp_case_params(tp_context *ctx, int32_t case_id, int32_t *iparams, double *dparams);
+/* <rundir>/rsta/restartNNN.dat per parent block NNN (src/rpt.f90:555-623,
+ * wrRsta): n and ndt as INTEGER4 at bytes 0 and 8, dt, dts, dte, timo as
+ * REAL8 at 16..40, then qa(0:lmx,1:5) of the parent block (REAL8, Fortran
+ * order) from byte 48.  Directories are created when missing. */
+int32_t tp_write_restart(tp_context *ctx, int32_t case_id, const char *rundir);
+/* read them back into qa and n, ndt, dt, dts, dte, timo (rdRsta,
+ * src/rpt.f90:628-661); the case must be on the device */
+int32_t tp_read_restart(tp_context *ctx, int32_t case_id, const char *rundir);
+/* <rundir>/misc/gridNNN.dat per parent block: the grid ss(0:lmx,1:3) as set
+ * by tp_set_grid, REAL8, no header (written by src/Grid.f90:452-459, read by
+ * rdGrid, src/rpt.f90:666-696) */
+int32_t tp_write_grid(tp_context *ctx, int32_t case_id, const char *rundir);
+/* read them and set every sub-block's grid (tp_set_grid) */
+int32_t tp_read_grid(tp_context *ctx, int32_t case_id, const char *rundir);
+/* PLOT3D multiblock files in <rundir>/out (src/rpt.f90:115-330): what & 1
+ * grid.xyz (INTEGER4 block count and IMax, JMax, KMax per block, then REAL4
+ * x, y, z per block); what & 2 solT<timo as f8.4, blanks as 0>.q (same
+ * header, then per block REAL4 Mach, AoA, Re, time and REAL4 rho, u + umf,
+ * v + umf, w + umf, p -- p being the pressure of the last stage, as the
+ * reference writes it) */
+int32_t tp_write_plot3d(tp_context *ctx, int32_t case_id, const char *rundir, int32_t what);
+
 /* ---- surface statistics on the device ----
  * The wall of a case is the eta face of each sub-block with the wall code
  * (20 + 5 nviscous), as src/rpt.f90:774-864 (wallArea, walldir) sets it up:

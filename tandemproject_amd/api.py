@@ -35,7 +35,9 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_server_context", "tp_server_error", "tp_server_clock", "tp_server_submit",
            "tp_server_iterate", "tp_server_run", "tp_server_result", "tp_server_result_count",
            "tp_server_stats", "tp_server_set_fill", "tp_stats_start", "tp_stats_forces",
-           "tp_stats_wall_count", "tp_stats_wall"]
+           "tp_stats_wall_count", "tp_stats_wall", "tp_set_parents", "tp_get_parents",
+           "tp_case_dims", "tp_case_params", "tp_write_restart", "tp_read_restart", "tp_write_grid",
+           "tp_read_grid", "tp_write_plot3d", "tp_get_grid"]
 
 _lib = None
 
@@ -87,6 +89,13 @@ def load(path=LIB_PATH):
         "tp_server_stats": [P, pd], "tp_server_set_fill": [P, i64],
         "tp_stats_start": [P, i32, i32], "tp_stats_forces": [P, i32, pd],
         "tp_stats_wall": [P, i32, i32, pd, pd, pd, pd, i64],
+        "tp_set_parents": [P, i32, i32, pi32, pi32, pi32],
+        "tp_get_parents": [P, i32, pi32, pi32, pi32, pi32],
+        "tp_case_dims": [P, i32, pi32], "tp_case_params": [P, i32, pi32, pd],
+        "tp_write_restart": [P, i32, ctypes.c_char_p], "tp_read_restart": [P, i32, ctypes.c_char_p],
+        "tp_write_grid": [P, i32, ctypes.c_char_p], "tp_read_grid": [P, i32, ctypes.c_char_p],
+        "tp_write_plot3d": [P, i32, ctypes.c_char_p, i32],
+        "tp_get_grid": [P, i32, i32, pd, i64],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -186,6 +195,29 @@ class Solver:
                                   _ip(j), _ip(v), _ip(ipar), _dp(dpar), ctypes.byref(cid)))
         self.layouts[cid.value] = layout
         return cid.value
+
+    def set_parents(self, case, layout):
+        """Parent blocks of the reference's file formats from a layout with
+        parent/start/blockdims (layout.h_layout fills them)."""
+        par = np.ascontiguousarray(layout.parent, np.int32)
+        st = np.ascontiguousarray(layout.start, np.int32).ravel()
+        pd_ = np.ascontiguousarray(layout.blockdims, np.int32).ravel()
+        _check(load().tp_set_parents(self.h, case, int(len(layout.blockdims)), _ip(par), _ip(st), _ip(pd_)))
+
+    def write_restart(self, case, rundir):
+        _check(load().tp_write_restart(self.h, case, str(rundir).encode()))
+
+    def read_restart(self, case, rundir):
+        _check(load().tp_read_restart(self.h, case, str(rundir).encode()))
+
+    def write_grid(self, case, rundir):
+        _check(load().tp_write_grid(self.h, case, str(rundir).encode()))
+
+    def read_grid(self, case, rundir):
+        _check(load().tp_read_grid(self.h, case, str(rundir).encode()))
+
+    def write_plot3d(self, case, rundir, grid=True, solution=True):
+        _check(load().tp_write_plot3d(self.h, case, str(rundir).encode(), (1 if grid else 0) | (2 if solution else 0)))
 
     def case_bytes(self, case):
         """Device bytes the case holds once admitted (needs its grids)."""

@@ -11,7 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["tp_line.hip", "tp_col.hip", "tp_stage.hip", "tp_capi.hip", "tp_serve.cpp"]
+SOURCES = ["tp_line.hip", "tp_col.hip", "tp_stage.hip", "tp_capi.hip", "tp_serve.cpp", "tp_io.cpp"]
 LIB = os.path.join(HERE, "libtandem_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TP_OFFLOAD_ARCH", "gfx950")

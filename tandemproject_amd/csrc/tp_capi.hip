@@ -39,6 +39,11 @@ static int fail(int code, const std::string &msg) {
       return fail(TP_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));               \
   } while (0)
 
+namespace tp {
+// the message and code of a failure in another host file of the library
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace tp
+
 // ----------------------------------------------------------------------------
 // scheme coefficients (host, once per case)
 // ----------------------------------------------------------------------------
@@ -412,6 +417,10 @@ struct HostCase {
   std::vector<std::vector<double>> grid;    // host copy of ss per block
   double sc[S_STRIDE];                      // scalars kept across a relink
   bool sc_saved = false;
+  // parent blocks of the reference's file formats (tp_set_parents): parent of
+  // each sub-block, its first point in the parent, the parents' last indices
+  int npar = 0;
+  std::vector<int> parent, pstart, pdims;
 };
 
 struct DevList {
@@ -2053,6 +2062,75 @@ int32_t tp_stats_wall(tp_context *c, int32_t cs, int32_t lb, double *mean_p, dou
     if (area) area[ll] = ar[ll];
   }
   if (wnor3) std::memcpy(wnor3, nr.data(), sizeof(double) * nr.size());
+  return 0;
+}
+
+int32_t tp_set_parents(tp_context *c, int32_t cs, int32_t nparent, const int32_t *parent, const int32_t *start3,
+                       const int32_t *pdims3) {
+  if (!c || !parent || !start3 || !pdims3 || nparent < 1) return fail(TP_E_ARG, "tp_set_parents: bad arguments");
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  HostCase &h = c->cases[ci];
+  for (int b = 0; b < h.nb; b++) {
+    const int mb = parent[b];
+    if (mb < 0 || mb >= nparent) return fail(TP_E_LAYOUT, "sub-block " + std::to_string(b) + ": bad parent");
+    for (int d = 0; d < 3; d++)
+      if (start3[3 * b + d] < 0 || start3[3 * b + d] + h.dims[3 * b + d] > pdims3[3 * mb + d])
+        return fail(TP_E_LAYOUT, "sub-block " + std::to_string(b) + " does not fit in its parent block");
+  }
+  h.npar = nparent;
+  h.parent.assign(parent, parent + h.nb);
+  h.pstart.assign(start3, start3 + 3 * h.nb);
+  h.pdims.assign(pdims3, pdims3 + 3 * nparent);
+  return 0;
+}
+
+int32_t tp_get_parents(tp_context *c, int32_t cs, int32_t *nparent, int32_t *parent, int32_t *start3,
+                       int32_t *pdims3) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  const HostCase &h = c->cases[ci];
+  const bool set = h.npar > 0;
+  if (nparent) *nparent = set ? h.npar : h.nb;
+  for (int b = 0; b < h.nb; b++) {
+    if (parent) parent[b] = set ? h.parent[b] : b;
+    for (int d = 0; d < 3; d++) {
+      if (start3) start3[3 * b + d] = set ? h.pstart[3 * b + d] : 0;
+      if (pdims3 && !set) pdims3[3 * b + d] = h.dims[3 * b + d];
+    }
+  }
+  if (pdims3 && set)
+    for (int q = 0; q < 3 * h.npar; q++) pdims3[q] = h.pdims[q];
+  return 0;
+}
+
+int32_t tp_get_grid(tp_context *c, int32_t cs, int32_t lb, double *xyz, int64_t count) {
+  if (!c || !xyz) return fail(TP_E_ARG, "null argument");
+  const int ci = case_index(c, cs);
+  if (ci < 0 || lb < 0 || lb >= c->cases[ci].nb) return fail(TP_E_ARG, "tp_get_grid: bad case/block");
+  const HostCase &h = c->cases[ci];
+  if ((int)h.grid.size() <= lb || h.grid[lb].empty()) return fail(TP_E_STATE, "grid not set");
+  if (count != (int64_t)h.grid[lb].size())
+    return fail(TP_E_SIZE, "tp_get_grid: expected " + std::to_string(h.grid[lb].size()) + " values");
+  std::memcpy(xyz, h.grid[lb].data(), sizeof(double) * h.grid[lb].size());
+  return 0;
+}
+
+int32_t tp_case_dims(tp_context *c, int32_t cs, int32_t *dims3) {
+  if (!c || !dims3) return fail(TP_E_ARG, "null argument");
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  std::memcpy(dims3, c->cases[ci].dims.data(), sizeof(int32_t) * c->cases[ci].dims.size());
+  return 0;
+}
+
+int32_t tp_case_params(tp_context *c, int32_t cs, int32_t *iparams, double *dparams) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  const int ci = case_index(c, cs);
+  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  if (iparams) std::memcpy(iparams, c->cases[ci].ip, sizeof(c->cases[ci].ip));
+  if (dparams) std::memcpy(dparams, c->cases[ci].dp, sizeof(c->cases[ci].dp));
   return 0;
 }
 

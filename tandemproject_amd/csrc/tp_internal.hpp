@@ -181,6 +181,9 @@ void launch_junction(const Block *blocks, const int *jct, const int *jvel, int n
 void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, const int *wwr, int ncase,
                         hipStream_t st);
 
+// host: record the message of a failure (tp_last_error) and return its code
+int set_error(int code, const std::string &msg);
+
 enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS,
                    PK_GRADSTRESS = 13, PK_FLUXALL };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,

@@ -212,6 +212,71 @@ module tandem_hip
       real(c_double), intent(in) :: uoo3(3)
       real(c_double), intent(out) :: out8(8), slit
     end function
+    ! the reference's file formats
+    integer(c_int32_t) function tp_set_parents(ctx, case_id, nparent, parent, start3, pdims3) &
+        bind(C, name="tp_set_parents")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id, nparent
+      integer(c_int32_t), intent(in) :: parent(*), start3(*), pdims3(*)
+    end function
+    integer(c_int32_t) function tp_get_parents(ctx, case_id, nparent, parent, start3, pdims3) &
+        bind(C, name="tp_get_parents")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      integer(c_int32_t), intent(out) :: nparent, parent(*), start3(*), pdims3(*)
+    end function
+    integer(c_int32_t) function tp_get_grid(ctx, case_id, block, xyz, count) bind(C, name="tp_get_grid")
+      import :: c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id, block
+      real(c_double), intent(out) :: xyz(*)
+      integer(c_int64_t), value :: count
+    end function
+    integer(c_int32_t) function tp_case_dims(ctx, case_id, dims3) bind(C, name="tp_case_dims")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      integer(c_int32_t), intent(out) :: dims3(*)
+    end function
+    integer(c_int32_t) function tp_case_params(ctx, case_id, iparams, dparams) bind(C, name="tp_case_params")
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      integer(c_int32_t), intent(out) :: iparams(16)
+      real(c_double), intent(out) :: dparams(32)
+    end function
+    integer(c_int32_t) function tp_write_restart(ctx, case_id, rundir) bind(C, name="tp_write_restart")
+      import :: c_ptr, c_int32_t, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      character(kind=c_char), intent(in) :: rundir(*)
+    end function
+    integer(c_int32_t) function tp_read_restart(ctx, case_id, rundir) bind(C, name="tp_read_restart")
+      import :: c_ptr, c_int32_t, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      character(kind=c_char), intent(in) :: rundir(*)
+    end function
+    integer(c_int32_t) function tp_write_grid(ctx, case_id, rundir) bind(C, name="tp_write_grid")
+      import :: c_ptr, c_int32_t, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      character(kind=c_char), intent(in) :: rundir(*)
+    end function
+    integer(c_int32_t) function tp_read_grid(ctx, case_id, rundir) bind(C, name="tp_read_grid")
+      import :: c_ptr, c_int32_t, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id
+      character(kind=c_char), intent(in) :: rundir(*)
+    end function
+    integer(c_int32_t) function tp_write_plot3d(ctx, case_id, rundir, what) bind(C, name="tp_write_plot3d")
+      import :: c_ptr, c_int32_t, c_char
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id, what
+      character(kind=c_char), intent(in) :: rundir(*)
+    end function
     ! surface statistics
     integer(c_int32_t) function tp_stats_start(ctx, case_id, every) bind(C, name="tp_stats_start")
       import :: c_ptr, c_int32_t
