@@ -333,6 +333,18 @@ class Solver:
         _check(L.tp_stats_wall(self.h, case, block, _dp(m), _dp(r), _dp(a), _dp(w), n))
         return m, r, a, w.reshape(3, n)
 
+    def stats_cp(self, case, block):
+        """(mean Cp, rms Cp) per wall point, or None: Cp = (p - poo) 2 / M^2
+        with poo = 1/gamma and M the free-stream Mach number
+        (src/rptPost.f90:693-694, src/Main3D.f90:61)."""
+        w = self.stats_wall(case, block)
+        if w is None:
+            return None
+        dp = np.zeros(len(D_NAMES))
+        _check(load().tp_case_params(self.h, case, None, _dp(dp)))
+        m2 = sum(dp[D_NAMES.index(k)] ** 2 for k in ("amach1", "amach2", "amach3"))
+        return (w[0] - 1.0 / 1.4) * 2.0 / m2, w[1] * 2.0 / m2
+
     # ---- reference routines (all sub-blocks) ----
     def mpigo(self, nt, nrt, n45):
         _check(load().tp_mpigo(self.h, nt, nrt, n45))

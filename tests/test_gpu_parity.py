@@ -382,3 +382,54 @@ def test_surface_statistics(graph):
         assert rel(rs[0], ro[0]) <= 1e-13
         assert np.abs(rs[1] ** 2 - ro[1] ** 2).max() <= 1e-12 * np.abs(ro[0]).max() ** 2
         assert ro[1].max() > 0
+
+
+def test_surface_statistics_long_run():
+    """Mean and rms wall pressure and the mean force coefficients over a long
+    gust-response run on the production path (chunked line solves, replayed
+    step graph): 600 steps, sampled every 4 steps after 40 steps of start-up,
+    against the oracle within 1 % (the bar for long runs; the agreement
+    reached is printed and is far tighter)."""
+    say("surface statistics, gust response, 600 steps sampled every 4, chunked + graph")
+    lay, ss, p = aerofoil_case()
+    a = np.radians(4.0)
+    p = dict(p)
+    p.update(amach1=0.4 * np.cos(a), amach2=0.4 * np.sin(a), inflow=2, ntz=32, tgusto=0.0, tu=0.04,
+             gustk=2.0, gustw=1.5)
+    uoo = np.array([p["amach1"], p["amach2"], 0.0])
+    mode, slit = api.gust_mode(p["tu"], p["gustk"], uoo)
+    o = po.Oracle(lay, dict(p, slit=slit, nmodes=1))
+    o.set_inflow_modes(mode)
+    s = api.Solver([(lay, p)])
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b]); s.set_grid(0, b, ss[b])
+    o.metrics(); s.setup_geometry()
+    o.initialo(); s.initialo(0)
+    o.step(40); s.step(40)
+    o.stats_start(4); s.stats_start(0, 4)
+    o.step(600); s.step(600, graph=True)
+    s.sync()
+    fo, fs = o.stats_forces(), s.stats_forces(0)
+    say("  oracle", fo)
+    say("  device", fs)
+    assert fo["samples"] == fs["samples"] == 150
+    worst = 0.0
+    for k in ("cl_mean", "cd_mean"):
+        e = abs(fs[k] - fo[k]) / max(abs(fo["cl_mean"]), abs(fo["cd_mean"]))
+        worst = max(worst, e)
+        assert e <= 0.01, (k, fo[k], fs[k])
+    for b in range(lay.nb):
+        ro = o.stats_wall(b)
+        if ro is None:
+            continue
+        rs = s.stats_wall(0, b)
+        em = np.abs(rs[0] - ro[0]).max() / np.abs(ro[0]).max()
+        er = np.abs(rs[1] - ro[1]).max() / ro[1].max()
+        worst = max(worst, em, er)
+        assert em <= 0.01 and er <= 0.01, (b, em, er)
+        assert ro[1].max() > 1e-4 * np.abs(ro[0]).max()
+        cm, cr = s.stats_cp(0, b)
+        m2 = p["amach1"] ** 2 + p["amach2"] ** 2
+        np.testing.assert_allclose(cm, (ro[0] - 1 / 1.4) * 2 / m2, rtol=0, atol=0.01 * np.abs(cm).max())
+        np.testing.assert_allclose(cr, ro[1] * 2 / m2, rtol=0, atol=0.01 * cr.max())
+    say("  worst relative difference of the statistics", worst)
