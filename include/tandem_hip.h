@@ -72,7 +72,9 @@ enum {
   TP_S_N = 0, TP_S_NDT, TP_S_DT, TP_S_DTS, TP_S_DTE, TP_S_TIMO, TP_S_UMF1, TP_S_UMF2, TP_S_UMF3,
   TP_S_DUDT1, TP_S_DUDT2, TP_S_DUDT3, TP_S_DTK, TP_S_DTKO, TP_S_AOA, TP_S_NSCALAR = 16
 };
-/* kernel classes reported by tp_profile_read */
+/* kernel classes reported by tp_profile_read (TP_K_FLUX: the pass forming the
+ * viscous stresses and all fluxes; TP_K_VISC: none since the stresses are
+ * formed inside that pass, kept so that the numbering stays) */
 enum {
   TP_K_DERIV = 0, TP_K_FILTER, TP_K_EXCH, TP_K_POINT, TP_K_FLUX, TP_K_VISC, TP_K_GCBC, TP_K_RK,
   TP_K_MISC, TP_K_NCLASS
@@ -157,8 +159,15 @@ int32_t tp_sync(tp_context *ctx);
  *   and direction.  The chunk count of a block depends only on its own line
  *   lengths (and on the autotuner's choice for its line-length class), so a
  *   case computes bit-identical results alone or batched with others when
- *   the chunk counts are fixed or left at their defaults. */
-enum { TP_OPT_EXACT_LINES = 0, TP_OPT_CHUNKS = 1 };
+ *   the chunk counts are fixed or left at their defaults.
+ * TP_OPT_STRESSES = 1: every stage leaves the viscous stresses and heat
+ *   fluxes (the reference's txx, tyy, tzz, txy, tyz, tzx, hxx, hyy, hzz) in
+ *   TP_TV at every point; 0 (default): the stresses stay inside the flux
+ *   pass and TP_TV holds them only at the wall rows of blocks with a wall
+ *   surface, after the first stage of a step (what the surface forces read).
+ *   Either way the stage's results are the same.  May be changed between
+ *   steps. */
+enum { TP_OPT_EXACT_LINES = 0, TP_OPT_CHUNKS = 1, TP_OPT_STRESSES = 2 };
 int32_t tp_set_option(tp_context *ctx, int32_t option, int64_t value);
 
 /* Time every usable line-solver configuration (sequential kernel and P =

@@ -255,6 +255,11 @@ class Solver:
         order) instead of the chunked sweeps."""
         _check(load().tp_set_option(self.h, 0, 1 if exact else 0))
 
+    def set_keep_stresses(self, keep=True):
+        """Leave the viscous stresses in tv at every point after every stage
+        (default: only at the wall rows the surface forces read)."""
+        _check(load().tp_set_option(self.h, 2, 1 if keep else 0))
+
     def set_chunks(self, chunks):
         """Chunks per line of the chunked sweeps (0: the library's choice)."""
         _check(load().tp_set_option(self.h, 1, int(chunks)))

@@ -470,6 +470,7 @@ struct tp_context {
   int nlp[3] = {0, 0, 0};
   size_t lds[3] = {0, 0, 0};
   bool exact_lines = false;        // sequential one-lane-per-line sweeps
+  bool keep_stress = false;        // tv written at every point of every stage
   int col_fixed = 0;               // requested chunks per line (0: chosen per direction)
   // autotuned chunks per line for each direction, kind and line-length class (-1: none)
   int tuned[3][2][kNClass];
@@ -714,14 +715,21 @@ void enqueue_stage(tp_context *c, int nk) {
     launch_sc_dt(c->d_coef, c->d_sc, c->d_part, c->d_wr, ncase, 1, c->stream);
   }
   if (c->any_visc) {
-    // src/Main3D.f90:627-692: gradients of u, v, w, T (de(:,2:5)) along the
+    // src/Main3D.f90:627-651: gradients of u, v, w, T (de(:,2:5)) along the
     // three index directions into fx(:,3f+d) -- the four fields in one launch
-    // per direction -- then one pass forming the stresses and heat fluxes
+    // per direction
     run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1);
     run_line(c, 0, 2, F_DE, 1, F_FX, 2, 0, 0, 4, 1, 3);
     run_line(c, 0, 1, F_DE, 1, F_FX, 1, 0, 0, 4, 1, 3);
     run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3);
-    run_points(c, PK_GRADSTRESS, a, K_VISC, 8.0 * (26 + 10));
+  }
+  // src/Main3D.f90:653-768: stresses and heat fluxes, then all 15 flux
+  // components, in one pass over the gradients; then
+  // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
+  {
+    StageArgs f = a;
+    f.opt = c->keep_stress ? 1 : 0;
+    run_points(c, PK_FLUXALL, f, K_FLUX, 8.0 * (c->any_visc ? 32 + 15 : 18 + 15));
   }
   if (nk == 1 && c->L.surf.n) {
     // surface forces and pressure statistics, src/Main3D.f90:695-706 (the
@@ -729,9 +737,6 @@ void enqueue_stage(tp_context *c, int nk) {
     run_faces(c, c->L.surf, FK_SURF_SAMPLE, a, K_MISC);
     launch_surf_reduce(c->d_coef, c->d_sc, c->d_wpart, c->d_wwr, ncase, c->stream);
   }
-  // src/Main3D.f90:710-768: all 15 flux components in one pass, then
-  // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
-  run_points(c, PK_FLUXALL, a, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 18 + 15));
   // the five fields m in one launch per step: flux components fx(:,3(m-1)+d)
   run_exchange(c, 0, 0, 0, F_FX, 0, 5, 3);
   run_line(c, 0, 0, F_FX, 0, F_DE, 0, 1, 0, 5, 3, 1);
@@ -1100,8 +1105,15 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
     for (int d = 0; d < 3; d++)
       for (int s = 0; s < 2; s++) {
         const int np = b.nbc[d][s], F = b.F[d];
+        // eta faces in tiles of FACE_TX xi x FACE_TZ zeta (a = first xi,
+        // n = first zeta), the others in runs of 256 face indices
         auto add = [&](std::vector<Work> &v) {
-          for (int a0 = 0; a0 < F; a0 += 256) v.push_back({g, a0, std::min(256, F - a0), d | (s << 8)});
+          if (d == 1) {
+            for (int x0 = 0; x0 < b.nx; x0 += FACE_TX)
+              for (int z0 = 0; z0 < b.nz; z0 += FACE_TZ) v.push_back({g, x0, z0, d | (s << 8)});
+          } else {
+            for (int a0 = 0; a0 < F; a0 += 256) v.push_back({g, a0, std::min(256, F - a0), d | (s << 8)});
+          }
         };
         const bool gc_face = np == 10 || np == 20 || np == 25 || np == 30;
         const bool wall = np == 20 || np == 25;
@@ -1381,6 +1393,10 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
     for (int d = 0; d < 3; d++) {
       const int L = dims[3 * b + d] + 1;
       int need = 7;
+      // both ends characteristic: their bands of MBCI + 1 points, updated by
+      // one launch, must not share a point
+      const auto gc = [](int np) { return np == 10 || np == 20 || np == 25 || np == 30; };
+      if (gc(nbc[6 * b + 2 * d]) && gc(nbc[6 * b + 2 * d + 1])) need = 2 * (MBCI + 1);
       for (int s = 0; s < 2; s++) {
         const int np = nbc[6 * b + 2 * d + s];
         if (np != 10 && np != 20 && np != 25 && np != 30 && np != 35 && np != 40 && np != 45)
@@ -1788,6 +1804,11 @@ int32_t tp_set_option(tp_context *c, int32_t option, int64_t value) {
   if (option == TP_OPT_EXACT_LINES) {
     c->exact_lines = value != 0;
     drop_graph(c);   // a captured step keeps its kernels: capture again
+    return 0;
+  }
+  if (option == TP_OPT_STRESSES) {
+    c->keep_stress = value != 0;
+    drop_graph(c);
     return 0;
   }
   if (option == TP_OPT_CHUNKS) {

@@ -116,6 +116,25 @@ struct Work {
   int d;      // direction (line and face launches), side in bit 8
 };
 
+// Face launches: the face point jk of this thread (256 threads per work item).
+// A xi or zeta face takes 256 consecutive face indices: on a zeta face they
+// are consecutive in memory, on a xi face each is its own row.  An eta face
+// (face index xi*nz + zeta, memory xi-fastest) is taken in tiles of 32 xi x 8
+// zeta, work item a = first xi, n = first zeta: a wave's 32-lane rows read
+// consecutive doubles of the volume arrays, and the tile's 8 rows share the
+// cache lines of the face arrays.
+constexpr int FACE_TX = 32, FACE_TZ = 8;
+__device__ inline bool face_point(const Block &b, const Work &w, int d, int &jk) {
+  const int t = threadIdx.x;
+  if (d == 1) {
+    const int xi = w.a + (t & (FACE_TX - 1)), ze = w.n + t / FACE_TX;
+    jk = xi * b.nz + ze;
+    return xi < b.nx && ze < b.nz;
+  }
+  jk = w.a + t;
+  return t < w.n;
+}
+
 // kernel classes timed by the profiler hooks
 enum KClass {
   K_DERIV = 0, K_FILTER, K_EXCH, K_POINT, K_FLUX, K_VISC, K_GCBC, K_RK, K_MISC, K_NCLASS
@@ -139,7 +158,7 @@ struct LineArgs {
 
 struct ExchArgs {
   const Block *blocks;
-  const Work *work;        // a = first face point, n = count, d = dir | side << 8
+  const Work *work;        // face work items (face_point), d = dir | side << 8
   const CaseCoef *coef;
   int nt, nrt, n45;
   int arr, col0;           // source field: column col0 + f fstep + (1-nrt)*dir,
@@ -153,7 +172,7 @@ struct StageArgs {
   double *sc;        // [ncase][S_STRIDE]
   double *part;      // reduction partials [nwork][2]
   double *wpart;     // wall-force partials [nwallwork][4]
-  int nk, m, d, opt; // stage number, field, direction, option
+  int nk, m, d, opt; // stage number, field, direction, option (k_flux_all: write tv everywhere)
   int arr, col;      // generic field selector
   int o0, o1, o2;    // tv slots for gradient combination (-1..-3 = ss columns)
 };
@@ -185,7 +204,7 @@ void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, c
 int set_error(int code, const std::string &msg);
 
 enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS,
-                   PK_GRADSTRESS = 13, PK_FLUXALL };
+                   PK_FLUXALL = 14 };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,
                   FK_SURF_SETUP, FK_SURF_SAMPLE };
 

@@ -280,18 +280,18 @@ __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
 
 // ---- closure sums received from the neighbour, Subroutines3D.f90:16-80 ----
 // One thread per receiving face point; the sender's interior points are read
-// straight from its field (no send buffer, no host round trip).
-
-
+// straight from its field (no send buffer, no host round trip).  NT: 0 the
+// derivative's closure (LMD + 1 points), 1 the filter's (LMF + 1).
+template <int NT>
 __global__ void __launch_bounds__(256) k_exchange(ExchArgs a) {
+  constexpr int MP = NT == 0 ? LMD : LMF;
   const Work w = a.work[blockIdx.x];
-  const int jk = w.a + threadIdx.x;
-  if (threadIdx.x >= w.n) return;
   const int d = w.d & 0xff, ip = w.d >> 8;
   const Block &b = a.blocks[w.blk];
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   const Block &s = a.blocks[b.ncd[d][ip]];
   const CaseCoef &c = a.coef[b.cs];
-  const int nt = a.nt, mp = nt == 0 ? LMD : LMF;
   const int f = blockIdx.y;
   const int col = a.col0 + f * a.fstep + (1 - a.nrt) * d;
   const int sp = 1 - ip;
@@ -302,23 +302,28 @@ __global__ void __launch_bounds__(256) k_exchange(ExchArgs a) {
   const int j = jk % fd, k = jk / fd;
   const double *src = field_ptr(s, a.arr, col);
   const int is = sp * (s.L[d] - 1), step = 1 - 2 * sp;
-  const double res = ra0 * src[indx3(s, is, j, k, d)];
-  double t0 = 0, t1 = 0, s0 = 0;
-  for (int i = 0; i <= mp; i++) {
-    double v = src[indx3(s, is + step * (i + ii), j, k, d)];
-    if (i == 0) s0 = v;
-    t0 = t0 + c.pbco[nt][0][i] * v;
-    t1 = t1 + c.pbco[nt][1][i] * v;
+  // every load first, then the sums in the reference's order
+  double v[MP + 1];
+  const double e0 = src[indx3(s, is, j, k, d)];
+#pragma unroll
+  for (int i = 0; i <= MP; i++) v[i] = src[indx3(s, is + step * (i + ii), j, k, d)];
+  double own = 0.0;
+  const bool self45 = a.n45 == 1 && b.nbc[d][ip] == 45;
+  if (self45) own = field_ptr(b, a.arr, col)[indx3(b, ip * (b.L[d] - 1), j, k, d)];
+  const double res = ra0 * e0;
+  double t0 = 0, t1 = 0;
+#pragma unroll
+  for (int i = 0; i <= MP; i++) {
+    t0 = t0 + c.pbco[NT][0][i] * v[i];
+    t1 = t1 + c.pbco[NT][1][i] * v[i];
   }
-  double r0 = t0 - res * c.pbcot[nt][0];
-  double r1 = t1 - res * c.pbcot[nt][1];
-  double r2 = s0 - res;
-  if (a.n45 == 1 && b.nbc[d][ip] == 45) {
-    const double *own = field_ptr(b, a.arr, col);
-    double v = own[indx3(b, ip * (b.L[d] - 1), j, k, d)];
-    r0 = r0 + v * c.pbcot[nt][0];
-    r1 = r1 + v * c.pbcot[nt][1];
-    r2 = r2 + v;
+  double r0 = t0 - res * c.pbcot[NT][0];
+  double r1 = t1 - res * c.pbcot[NT][1];
+  double r2 = v[0] - res;
+  if (self45) {
+    r0 = r0 + own * c.pbcot[NT][0];
+    r1 = r1 + own * c.pbcot[NT][1];
+    r2 = r2 + own;
   }
   const int F = b.F[d];
   double *rcv = b.recv[d] + (size_t)f * 6 * F;
@@ -341,7 +346,8 @@ void launch_filter(int d, int nwg, int nf, size_t lds, hipStream_t st, const Lin
 }
 
 void launch_exchange(int nwg, int nf, hipStream_t st, const ExchArgs &a) {
-  hipLaunchKernelGGL(k_exchange, dim3(nwg, nf), dim3(256), 0, st, a);
+  if (a.nt == 0) hipLaunchKernelGGL(k_exchange<0>, dim3(nwg, nf), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_exchange<1>, dim3(nwg, nf), dim3(256), 0, st, a);
 }
 
 }  // namespace tp

@@ -255,107 +255,99 @@ __global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
   if (threadIdx.x == 0) { a.part[2 * blockIdx.x] = mi[0]; a.part[2 * blockIdx.x + 1] = mv[0]; }
 }
 
-// ---- viscous stresses and heat fluxes, Main3D.f90:627-692, one pass ----
-// fx(:,3f+d) holds d(field f)/d(index d) for f = u, v, w, T (de(:,2:5)) and
-// d = xi, eta, zeta.  Each gradient is combined with the metric cofactors
-// exactly as the reference does into its work arrays (Main3D.f90:631-651:
-// u -> txx, hzz, tzx; v -> txy, tyy, hxx; w -> hyy, tyz, tzz; T -> ss(:,1:3))
-// and the stresses and heat fluxes follow in the reference's operation order
-// (Main3D.f90:653-692); the viscosity is ss(:,1) (the reference copies it
-// to de(:,1) first, :627, because its T gradient overwrites ss).  Reads the
-// 12 gradients and the metrics once instead of once per field.
-__global__ void __launch_bounds__(256) k_grad_stress(StageArgs a) {
-  WORK_POINTS
-  const CaseCoef &c = a.coef[b.cs];
-  const Phys ph = phys();
-  const double *g = b.fx, *xi = b.xim, *et = b.etm, *ze = b.zem;
-  double *tv = b.tv, *de = b.de;
-  const double *ss = b.ss, *ya = b.yaco, *qa = b.qa;
-  for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    const double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
-    const double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
-    const double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
-    double cb[4][3];
-#pragma unroll
-    for (int f = 0; f < 4; f++) {
-      const double g1 = g[(size_t)(3 * f) * N + l], g2 = g[(size_t)(3 * f + 1) * N + l],
-                   g3 = g[(size_t)(3 * f + 2) * N + l];
-      cb[f][0] = x1 * g1 + e1 * g2 + z1 * g3;
-      cb[f][1] = x2 * g1 + e2 * g2 + z2 * g3;
-      cb[f][2] = x3 * g1 + e3 * g2 + z3 * g3;
-    }
-    double txx = cb[0][0], hzz = cb[0][1], tzx = cb[0][2];
-    double txy = cb[1][0], tyy = cb[1][1], hxx = cb[1][2];
-    double hyy = cb[2][0], tyz = cb[2][1], tzz = cb[2][2];
-    const double sx = cb[3][0], sy = cb[3][1], sz = cb[3][2];
-    double r1 = ss[l] * ya[l];
-    double r2 = ph.gamm1prndtli * r1, d5;
-    if (c.les == 1) {
-      double d1 = txx * txx + tyy * tyy + tzz * tzz + (hzz + txy) * (hzz + txy) + (hyy + tzx) * (hyy + tzx) +
-                  (hxx + tyz) * (hxx + tyz);
-      double varr = (double)(float)pow(-1 / ya[l], 1.5);   // single-precision work array
-      double r3 = qa[l] * (c.smago1 * c.smago1) * varr * sqrt(2 * d1);
-      r1 = r1 + r3 * ya[l];
-      r2 = r2 + ph.tgamm1prndtli * r3 * ya[l];
-      r3 = ph.two_three * (qa[l] * c.smago2 * varr * d1);
-      d5 = ph.two_three * (txx + tyy + tzz);
-      double ntxx = r1 * (2 * txx - d5) - r3, ntyy = r1 * (2 * tyy - d5) - r3, ntzz = r1 * (2 * tzz - d5) - r3;
-      double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);
-      txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;
-      de[l] = d1;
-    } else {
-      d5 = ph.two_three * (txx + tyy + tzz);
-      double ntxx = r1 * (2 * txx - d5), ntyy = r1 * (2 * tyy - d5), ntzz = r1 * (2 * tzz - d5);
-      double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);
-      txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;
-    }
-    de[4 * N + l] = d5;
-    const double u = de[N + l], v = de[2 * N + l], ww = de[3 * N + l];
-    hxx = r2 * sx + u * txx + v * txy + ww * tzx;
-    hyy = r2 * sy + u * txy + v * tyy + ww * tyz;
-    hzz = r2 * sz + u * tzx + v * tyz + ww * tzz;
-    tv[l] = txx; tv[N + l] = tyy; tv[2 * N + l] = tzz; tv[3 * N + l] = txy; tv[4 * N + l] = tyz;
-    tv[5 * N + l] = tzx; tv[6 * N + l] = hxx; tv[7 * N + l] = hyy; tv[8 * N + l] = hzz;
-  }
+// ---- viscous stresses and heat fluxes (Main3D.f90:627-692), contravariant
+// velocities and the metric-weighted Euler + viscous fluxes of all five
+// fields (Main3D.f90:710-766), one pass ----
+// On entry fx(:,3f+d) holds d(field f)/d(index d) for f = u, v, w, T
+// (de(:,2:5)) and d = xi, eta, zeta.  Each gradient is combined with the
+// metric cofactors exactly as the reference does into its work arrays
+// (Main3D.f90:631-651: u -> txx, hzz, tzx; v -> txy, tyy, hxx; w -> hyy, tyz,
+// tzz; T -> ss(:,1:3)) and the stresses and heat fluxes follow in the
+// reference's operation order (Main3D.f90:653-692); the viscosity is ss(:,1)
+// (the reference copies it to de(:,1) first, :627, because its T gradient
+// overwrites ss).  The fluxes then overwrite the gradients of the same point:
+// fx(:,3(m-1)+d) = flux of field m along direction d, with contravariant
+// U_d = m_d . (u + umf) (:710-715), per field as :718-766.
+// The reference keeps the stresses in module arrays (tv here) and passes
+// de(:,1) and de(:,5) through as temporaries that the flux derivatives
+// overwrite; here the stresses stay in registers.  tv is written where it is
+// read: at the wall rows of blocks with a wall surface on the first stage
+// (the surface forces), everywhere when a.opt is set.
+__device__ inline void flux_put(double *fx, int N, int l, int m, double r1, double r2, double r3) {
+  fx[(size_t)(3 * (m - 1)) * N + l] = r1;
+  fx[(size_t)(3 * (m - 1) + 1) * N + l] = r2;
+  fx[(size_t)(3 * (m - 1) + 2) * N + l] = r3;
 }
 
-// ---- contravariant velocities and the metric-weighted Euler + viscous
-// fluxes of all five fields, Main3D.f90:710-766, one pass ----
-// fx(:,3(m-1)+d) = flux of field m along direction d, computed with the
-// reference's operations: contravariant U_d = m_d . (u + umf) (:710-715),
-// then per field (:718-766).  The reference evaluates one field at a time
-// into rr(:,1:3) (and puts E + p into de(:,5), which its m = 5 derivative
-// then overwrites); here the shared inputs are read once for all 15 fluxes.
 __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
+  const Phys ph = phys();
   const double *sc = a.sc + b.cs * S_STRIDE;
   const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
   const int visc = c.nviscous == 1;
-  const double *qa = b.qa, *xi = b.xim, *et = b.etm, *ze = b.zem, *p = b.p, *tv = b.tv, *de = b.de;
-  double *fx = b.fx;
+  // rows of this block whose stresses the surface forces read
+  const int wall_row = a.nk == 1 && b.nsurf > 0 ? (b.sside ? b.ny - 1 : 0) : -1;
+  const double *qa = b.qa, *xi = b.xim, *et = b.etm, *ze = b.zem, *p = b.p, *de = b.de;
+  const double *ss = b.ss, *ya = b.yaco;
+  double *fx = b.fx, *tv = b.tv;
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
     const double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
     const double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
     const double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
-    const double q1 = de[N + l] + u0, q2 = de[2 * N + l] + u1, q3 = de[3 * N + l] + u2;
+    const double u = de[N + l], v = de[2 * N + l], ww = de[3 * N + l];
+    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (visc) {
+      double cb[4][3];
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        const double g1 = fx[(size_t)(3 * f) * N + l], g2 = fx[(size_t)(3 * f + 1) * N + l],
+                     g3 = fx[(size_t)(3 * f + 2) * N + l];
+        cb[f][0] = x1 * g1 + e1 * g2 + z1 * g3;
+        cb[f][1] = x2 * g1 + e2 * g2 + z2 * g3;
+        cb[f][2] = x3 * g1 + e3 * g2 + z3 * g3;
+      }
+      double txx = cb[0][0], hzz = cb[0][1], tzx = cb[0][2];
+      double txy = cb[1][0], tyy = cb[1][1], hxx = cb[1][2];
+      double hyy = cb[2][0], tyz = cb[2][1], tzz = cb[2][2];
+      const double sx = cb[3][0], sy = cb[3][1], sz = cb[3][2];
+      double r1 = ss[l] * ya[l];
+      double r2 = ph.gamm1prndtli * r1, d5;
+      if (c.les == 1) {
+        double d1 = txx * txx + tyy * tyy + tzz * tzz + (hzz + txy) * (hzz + txy) + (hyy + tzx) * (hyy + tzx) +
+                    (hxx + tyz) * (hxx + tyz);
+        double varr = (double)(float)pow(-1 / ya[l], 1.5);   // single-precision work array
+        double r3 = qa[l] * (c.smago1 * c.smago1) * varr * sqrt(2 * d1);
+        r1 = r1 + r3 * ya[l];
+        r2 = r2 + ph.tgamm1prndtli * r3 * ya[l];
+        r3 = ph.two_three * (qa[l] * c.smago2 * varr * d1);
+        d5 = ph.two_three * (txx + tyy + tzz);
+        double ntxx = r1 * (2 * txx - d5) - r3, ntyy = r1 * (2 * tyy - d5) - r3, ntzz = r1 * (2 * tzz - d5) - r3;
+        double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);
+        txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;
+      } else {
+        d5 = ph.two_three * (txx + tyy + tzz);
+        double ntxx = r1 * (2 * txx - d5), ntyy = r1 * (2 * tyy - d5), ntzz = r1 * (2 * tzz - d5);
+        double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);
+        txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;
+      }
+      hxx = r2 * sx + u * txx + v * txy + ww * tzx;
+      hyy = r2 * sy + u * txy + v * tyy + ww * tyz;
+      hzz = r2 * sz + u * tzx + v * tyz + ww * tzz;
+      t[0] = txx; t[1] = tyy; t[2] = tzz; t[3] = txy; t[4] = tyz; t[5] = tzx; t[6] = hxx; t[7] = hyy; t[8] = hzz;
+      if (a.opt || (wall_row >= 0 && (l / b.nx) % b.ny == wall_row)) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) tv[(size_t)k * N + l] = t[k];
+      }
+    }
+    const double q1 = u + u0, q2 = v + u1, q3 = ww + u2;
     const double s1 = x1 * q1 + x2 * q2 + x3 * q3;
     const double s2 = e1 * q1 + e2 * q2 + e3 * q3;
     const double s3 = z1 * q1 + z2 * q2 + z3 * q3;
     const double pl = p[l];
-    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (visc) {
-#pragma unroll
-      for (int k = 0; k < 9; k++) t[k] = tv[(size_t)k * N + l];
-    }
-    auto put = [&](int m, double r1, double r2, double r3) {
-      fx[(size_t)(3 * (m - 1)) * N + l] = r1;
-      fx[(size_t)(3 * (m - 1) + 1) * N + l] = r2;
-      fx[(size_t)(3 * (m - 1) + 2) * N + l] = r3;
-    };
     {
       const double q = qa[l];
-      put(1, q * s1, q * s2, q * s3);
+      flux_put(fx, N, l, 1, q * s1, q * s2, q * s3);
     }
 #pragma unroll
     for (int m = 2; m <= 4; m++) {
@@ -365,7 +357,7 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
       const double mz = m == 2 ? z1 : m == 3 ? z2 : z3;
       double r1 = q * s1 + mx * pl, r2 = q * s2 + me * pl, r3 = q * s3 + mz * pl;
       if (visc) {
-        // (txx, txy, tzx), (txy, tyy, tyz), (tzx, tyz, tzz): tv slots 0..5 = txx tyy tzz txy tyz tzx
+        // (txx, txy, tzx), (txy, tyy, tyz), (tzx, tyz, tzz): slots 0..5 = txx tyy tzz txy tyz tzx
         const double ta = m == 2 ? t[0] : m == 3 ? t[3] : t[5];
         const double tb = m == 2 ? t[3] : m == 3 ? t[1] : t[4];
         const double tc = m == 2 ? t[5] : m == 3 ? t[4] : t[2];
@@ -373,7 +365,7 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
         r2 = r2 - e1 * ta - e2 * tb - e3 * tc;
         r3 = r3 - z1 * ta - z2 * tb - z3 * tc;
       }
-      put(m, r1, r2, r3);
+      flux_put(fx, N, l, m, r1, r2, r3);
     }
     {
       const double d5 = qa[4 * N + l] + pl;
@@ -385,7 +377,7 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
         r2 = r2 - e1 * t[6] - e2 * t[7] - e3 * t[8];
         r3 = r3 - z1 * t[6] - z2 * t[7] - z3 * t[8];
       }
-      put(5, r1, r2, r3);
+      flux_put(fx, N, l, 5, r1, r2, r3);
     }
   }
 }
@@ -532,9 +524,10 @@ __device__ inline void face_jk(const Block &b, int d, int jk, int &j, int &k) {
 // ---- GCBC/GCIC preparation, Main3D.f90:780-795 ----
 __global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
   const Work w = a.work[blockIdx.x];
-  if ((int)threadIdx.x >= w.n) return;
   const Block &b = a.blocks[w.blk];
-  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   const int np = b.nbc[d][ip];
   const double *sc = a.sc + b.cs * S_STRIDE;
   int j, k;
@@ -558,9 +551,10 @@ __global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
 // (GCIC exchange Main3D.f90:799-813 and interface averaging :915-925) ----
 __global__ void __launch_bounds__(256) k_face_copy(StageArgs a) {
   const Work w = a.work[blockIdx.x];
-  if ((int)threadIdx.x >= w.n) return;
   const Block &b = a.blocks[w.blk];
-  const int d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   const Block &s = a.blocks[b.ncd[d][ip]];
   for (int c = 0; c < 5; c++) b.drvb[d][(ip * 5 + c) * F + jk] = s.drva[d][((1 - ip) * 5 + c) * F + jk];
 }
@@ -584,10 +578,11 @@ __device__ inline void point_ijk(const Block &b, int l, int &x, int &y, int &z) 
 // ---- GCBC/GCIC application along direction d (both ends), Main3D.f90:817-869 ----
 __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
   const Work w = a.work[blockIdx.x];
-  if ((int)threadIdx.x >= w.n) return;
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
-  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   const int np = b.nbc[d][ip];
   const double *sc = a.sc + b.cs * S_STRIDE;
   int j, k;
@@ -625,13 +620,24 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
   for (int v = 0; v < 5; v++) dv[v] = cha[v] - drva[(ip * 5 + v) * F + jk];
   xtr2q_apply(e, cm, dv, dq);
   const double res = 1.0 / b.yaco[l];
+  // the band's de values are all read before any is written back, so the
+  // loads are issued together (no point of the band is another thread's)
+  double band[MBCI + 1][5];
+  int lb[MBCI + 1];
+#pragma unroll
   for (int ii = 0; ii <= MBCI; ii++) {
-    int l2 = indx3(b, i + iq * ii, j, k, d);
+    lb[ii] = indx3(b, i + iq * ii, j, k, d);
+#pragma unroll
+    for (int v = 0; v < 5; v++) band[ii][v] = b.de[(size_t)v * N + lb[ii]];
+  }
+#pragma unroll
+  for (int ii = 0; ii <= MBCI; ii++) {
     int x, y, z;
-    point_ijk(b, l2, x, y, z);
+    point_ijk(b, lb[ii], x, y, z);
     double wgt = ii == 0 ? res : res * c.sbci[ii - 1];
     double sbcc = wgt / gcbc_count(b, x, y, z);
-    for (int v = 0; v < 5; v++) b.de[(size_t)v * N + l2] = b.de[(size_t)v * N + l2] + sbcc * dq[v];
+#pragma unroll
+    for (int v = 0; v < 5; v++) b.de[(size_t)v * N + lb[ii]] = band[ii][v] + sbcc * dq[v];
   }
 }
 
@@ -648,11 +654,12 @@ __global__ void __launch_bounds__(256) k_rk(StageArgs a) {
 // ---- wall temperature and velocity, Main3D.f90:886-895 ----
 __global__ void __launch_bounds__(256) k_wall(StageArgs a) {
   const Work w = a.work[blockIdx.x];
-  if ((int)threadIdx.x >= w.n) return;
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const Phys ph = phys();
-  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x;
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8;
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   const int np = b.nbc[d][ip];
   const double *umf = a.sc + b.cs * S_STRIDE + S_UMF1;
   int j, k;
@@ -702,9 +709,10 @@ __device__ inline bool avg_code(int np) { return np == 30 || np == 35 || np == 4
 // capture face values into drva (first loop of the reference)
 __global__ void __launch_bounds__(256) k_avg_capture(StageArgs a) {
   const Work w = a.work[blockIdx.x];
-  if ((int)threadIdx.x >= w.n) return;
   const Block &b = a.blocks[w.blk];
-  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   int j, k;
   face_jk(b, d, jk, j, k);
   int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
@@ -737,10 +745,11 @@ __device__ inline double avg_count(const Block &b, const CaseCoef &c, int x, int
 
 __global__ void __launch_bounds__(256) k_avg_apply(StageArgs a) {
   const Work w = a.work[blockIdx.x];
-  if ((int)threadIdx.x >= w.n) return;
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
-  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   const int np = b.nbc[d][ip];
   int j, k;
   face_jk(b, d, jk, j, k);
@@ -812,9 +821,10 @@ __global__ void __launch_bounds__(256) k_metrics(StageArgs a) {
 // ---- face unit normals, Main3D.f90:429-439 ----
 __global__ void __launch_bounds__(256) k_normals(StageArgs a) {
   const Work w = a.work[blockIdx.x];
-  if ((int)threadIdx.x >= w.n) return;
   const Block &b = a.blocks[w.blk];
-  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, jk = w.a + threadIdx.x, F = b.F[d];
+  const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
+  int jk;
+  if (!face_point(b, w, d, jk)) return;
   int j, k;
   face_jk(b, d, jk, j, k);
   int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
@@ -964,7 +974,6 @@ void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
     case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;
     case 10: LAUNCH(k_initialo, nwg, 256, st, a); break;
     case 11: LAUNCH(k_metrics, nwg, 256, st, a); break;
-    case 13: LAUNCH(k_grad_stress, nwg, 256, st, a); break;
     case 14: LAUNCH(k_flux_all, nwg, 256, st, a); break;
   }
 }

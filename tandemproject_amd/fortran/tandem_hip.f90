@@ -45,7 +45,7 @@ module tandem_hip
                         TP_S_DUDT1 = 10, TP_S_DUDT2 = 11, TP_S_DUDT3 = 12, TP_S_DTK = 13, &
                         TP_S_DTKO = 14, TP_S_AOA = 15, TP_S_NSCALAR = 16
   ! options
-  integer(c_int32_t), parameter :: TP_OPT_EXACT_LINES = 0, TP_OPT_CHUNKS = 1
+  integer(c_int32_t), parameter :: TP_OPT_EXACT_LINES = 0, TP_OPT_CHUNKS = 1, TP_OPT_STRESSES = 2
 
   interface
     integer(c_int32_t) function tp_create(ctx) bind(C, name="tp_create")

@@ -204,6 +204,7 @@ def test_stage_arrays(nviscous, exact):
     say("stage arrays, nviscous =", nviscous, "sequential" if exact else "chunked")
     lay, ss, p = aerofoil_case(nviscous=nviscous)
     o, s = run_steps(lay, ss, p, 2, exact=exact)
+    s.set_keep_stresses(True)   # the stresses everywhere, not only at the wall rows
     o.filter_step(); s.filter_step()
     o.stage(1); s.stage(1)
     s.sync()
@@ -222,6 +223,49 @@ def test_stage_arrays(nviscous, exact):
     say("  max rel diff over stage arrays", worst)
     for key in ("dt", "umf1", "dudt1", "dtk"):
         assert abs(o.scalars()[key] - s.scalars(0)[key]) <= 1e-14 * max(1.0, abs(o.scalars()[key]))
+
+
+def test_stresses_kept_only_where_read():
+    """By default the flux pass keeps the viscous stresses in registers and
+    leaves them in tv only at the wall rows of wall blocks on the first stage
+    (for the surface forces); with TP_OPT_STRESSES it writes them everywhere.
+    The stage's results are bit-identical either way, and so are the wall-row
+    stresses, on the first stage and across a whole step."""
+    say("stress option: default against kept everywhere")
+    lay, ss, p = aerofoil_case()
+    sols = []
+    for keep in (False, True):
+        s = api.Solver([(lay, p)], exact_lines=True)
+        s.set_keep_stresses(keep)
+        for b in range(lay.nb):
+            s.set_grid(0, b, ss[b])
+        s.setup_geometry(); s.initialo(0)
+        s.step(2)
+        s.filter_step(); s.stage(1)
+        s.sync()
+        sols.append(s)
+    d, k = sols
+    walls = 0
+    for b in range(lay.nb):
+        for name in ("qa", "qo", "p", "de"):
+            np.testing.assert_array_equal(d.download(0, b, name), k.download(0, b, name))
+        nbc = lay.nbc[b][1]
+        if 25 not in nbc:
+            continue
+        walls += 1
+        nx, ny, nz = lay.dims[b] + 1
+        j = 0 if nbc[0] == 25 else ny - 1
+        td = d.download(0, b, "tv").reshape(9, nz, ny, nx)[:, :, j, :]
+        tk = k.download(0, b, "tv").reshape(9, nz, ny, nx)[:, :, j, :]
+        np.testing.assert_array_equal(td, tk)
+        assert np.abs(tk).max() > 0
+    assert walls == 2
+    for s in sols:
+        for nk in range(2, 5):
+            s.stage(nk)
+        s.sync()
+    for b in range(lay.nb):
+        np.testing.assert_array_equal(d.download(0, b, "qa"), k.download(0, b, "qa"))
 
 
 @LINE_MODES
