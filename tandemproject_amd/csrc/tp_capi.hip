@@ -660,6 +660,10 @@ int make_groups(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
         const Block &b = c->blocks[g];
         const int P = block_chunks(c, g, d, kind);
         const int C = P ? col_capacity((b.L[d] + P - 1) / P) : 0;
+        // a chunked launch reads the block's carry-map and factor table
+        if (P && (!C || !b.col[d][kind]))
+          return fail(TP_E_STATE, "block " + std::to_string(g) + " direction " + std::to_string(d + 1) +
+                                      ": no chunked-solver table for " + std::to_string(P) + " chunks");
         size_t k = 0;
         while (k < groups.size() && (groups[k].P != P || groups[k].C != C)) k++;
         if (k == groups.size()) {
