@@ -593,7 +593,7 @@ void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0, in
 // column in_col + f in_step, writes out_col + f out_step, uses recv slot f
 // (filled by an exchange over the same fields) and captures drva column m + f
 void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_arr, int out_col, int m,
-              int acc, int nf = 1, int in_step = 0, int out_step = 0) {
+              int acc, int nf = 1, int in_step = 0, int out_step = 0, int copy_arr = -1, int copy_col = 0) {
   if (!c->L.lines[d].n) return;
   // algorithmic bytes: one field in, one out; an accumulated output is also read
   Timed t(c, kind == 0 ? K_DERIV : K_FILTER, (acc ? 24.0 : 16.0) * (double)c->npts * nf);
@@ -602,6 +602,7 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
   a.m = m; a.acc = acc; a.nlp = c->nlp[d];
   a.in_step = in_step; a.out_step = out_step;
+  a.copy_arr = copy_arr; a.copy_col = copy_col;
   if (c->exact_lines) {
     if (kind == 0) launch_deriv(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
     else launch_filter(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
@@ -700,12 +701,14 @@ int regroup(tp_context *c) {
 }
 
 void enqueue_filter_step(tp_context *c) {
-  // src/Main3D.f90:555-561: per field m, mpigo then the xi, eta, zeta filters
+  // src/Main3D.f90:555-560: per field m, mpigo then the xi, eta, zeta filters
   // in place; the five fields are independent, so each of these steps runs
-  // over all five in one launch
+  // over all five in one launch; the last also leaves the result in qo
+  // (qo = qa, :561)
   run_exchange(c, 1, 1, 0, F_QA, 0, 5, 1);
-  for (int d = 0; d < 3; d++) run_line(c, 1, d, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
-  run_points(c, PK_COPY5, base_args(c), K_MISC, 80.0);
+  run_line(c, 1, 0, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
+  run_line(c, 1, 1, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
+  run_line(c, 1, 2, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1, F_QO, 0);
 }
 
 void enqueue_stage(tp_context *c, int nk) {
@@ -733,7 +736,7 @@ void enqueue_stage(tp_context *c, int nk) {
   {
     StageArgs f = a;
     f.opt = c->keep_stress ? 1 : 0;
-    run_points(c, PK_FLUXALL, f, K_FLUX, 8.0 * (c->any_visc ? 32 + 15 : 18 + 15));
+    run_points(c, PK_FLUXALL, f, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 14 + 15));
   }
   if (nk == 1 && c->L.surf.n) {
     // surface forces and pressure statistics, src/Main3D.f90:695-706 (the

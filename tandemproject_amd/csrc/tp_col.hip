@@ -497,6 +497,7 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   // ---- store: filter adds the increment to the input, deriv may accumulate ----
   const bool add = KIND == 1 || a.acc;
   const double *ob = KIND == 1 ? src : dst;
+  double *cpy = KIND == 1 && a.copy_arr >= 0 ? field_ptr(b, a.copy_arr, a.copy_col + fi * a.out_step) : nullptr;
   if (D == 0) {
 #pragma unroll
     for (int t = 0; t < (C + 7) / 8; t++) {
@@ -517,7 +518,12 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int rl = (lane >> 3) + 8 * j;
-        if (k < n && rl < nl) at(dst, 8u * (unsigned)((w.a + rl) * b.nx + s0 + k)) = add ? ov[j] + xv[j] : xv[j];
+        if (k < n && rl < nl) {
+          const unsigned o = 8u * (unsigned)((w.a + rl) * b.nx + s0 + k);
+          const double y = add ? ov[j] + xv[j] : xv[j];
+          at(dst, o) = y;
+          if (KIND == 1 && cpy) at(cpy, o) = y;
+        }
       }
       wave_sync();
     }
@@ -530,7 +536,12 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     if (act) {
 #pragma unroll
       for (int k = 0; k < C; k++)
-        if (k < n) at(dst, b8 + (unsigned)(s0 + k) * st8) = add ? ov[k] + r[k] : r[k];
+        if (k < n) {
+          const unsigned o = b8 + (unsigned)(s0 + k) * st8;
+          const double y = add ? ov[k] + r[k] : r[k];
+          at(dst, o) = y;
+          if (KIND == 1 && cpy) at(cpy, o) = y;
+        }
     }
   }
 }

@@ -154,6 +154,8 @@ struct LineArgs {
   int in_step, out_step;  // column steps per field: field f = blockIdx.y uses
                           // in_col + f in_step, out_col + f out_step, recv slot f
                           // and drva column m + f
+  int copy_arr, copy_col; // filter: also write the result to column copy_col +
+                          // f out_step of copy_arr (copy_arr < 0: no copy)
 };
 
 struct ExchArgs {
@@ -203,7 +205,7 @@ void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, c
 // host: record the message of a failure (tp_last_error) and return its code
 int set_error(int code, const std::string &msg);
 
-enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPY5, PK_COPYF, PK_INIT, PK_METRICS,
+enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPYF = 9, PK_INIT, PK_METRICS,
                    PK_FLUXALL = 14 };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,
                   FK_SURF_SETUP, FK_SURF_SAMPLE };

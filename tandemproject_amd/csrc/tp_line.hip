@@ -54,13 +54,15 @@ __device__ inline void tile_load(const Block &b, const double *__restrict__ src,
 
 template <int D, int ADD>
 __device__ inline void tile_store(const Block &b, double *__restrict__ dst, int q0, int nl, int L,
-                                  int nlp, const double *lds) {
+                                  int nlp, const double *lds, double *copy = nullptr) {
   if (D == 0) {
     for (int r = 0; r < nl; r++) {
       double *row = dst + (size_t)(q0 + r) * b.nx;
       for (int p = threadIdx.x; p < L; p += blockDim.x) {
         double v = lds[p * nlp + r];
-        row[p] = ADD ? row[p] + v : v;
+        v = ADD ? row[p] + v : v;
+        row[p] = v;
+        if (copy) copy[(size_t)(q0 + r) * b.nx + p] = v;
       }
     }
   } else {
@@ -69,9 +71,11 @@ __device__ inline void tile_store(const Block &b, double *__restrict__ dst, int 
       int r = e % nl, p = e / nl;
       int base, stride, jk;
       line_addr<D>(b, q0 + r, base, stride, jk);
-      double *o = dst + base + (size_t)p * stride;
+      const size_t o = base + (size_t)p * stride;
       double v = lds[p * nlp + r];
-      *o = ADD ? *o + v : v;
+      v = ADD ? dst[o] + v : v;
+      dst[o] = v;
+      if (copy) copy[o] = v;
     }
   }
 }
@@ -275,7 +279,8 @@ __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
 #undef S
   }
   __syncthreads();
-  tile_store<D, 1>(b, io, q0, nl, L, nlp, lds);
+  tile_store<D, 1>(b, io, q0, nl, L, nlp, lds,
+                   a.copy_arr >= 0 ? field_ptr(b, a.copy_arr, a.copy_col + f * a.out_step) : nullptr);
 }
 
 // ---- closure sums received from the neighbour, Subroutines3D.f90:16-80 ----

@@ -204,20 +204,31 @@ __global__ void k_sc_end(double *sc, int ncase, const int *active) {
 }
 
 // ---- primitives, Main3D.f90:576-583 ----
+// 1/rho, u, v, w, gamma p/rho, p and the viscosity of the conserved state q
+struct Prim {
+  double d1, d2, d3, d4, d5, p, mu;
+};
+__device__ inline Prim primitives(const CaseCoef &c, const Phys &ph, double q1, double q2, double q3, double q4,
+                                  double q5) {
+  Prim r;
+  r.d1 = 1.0 / q1;
+  r.d2 = q2 * r.d1; r.d3 = q3 * r.d1; r.d4 = q4 * r.d1;
+  r.p = ph.gamm1 * (q5 - 0.5 * (q2 * r.d2 + q3 * r.d3 + q4 * r.d4));
+  r.d5 = GAM * r.p * r.d1;
+  r.mu = c.srefp1dre * pow(r.d5, 1.5) / (r.d5 + c.srefoo);
+  return r;
+}
+
 __global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
   const Phys ph = phys();
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
     const double *qa = b.qa;
-    double d1 = 1.0 / qa[l];
-    double q2 = qa[N + l], q3 = qa[2 * N + l], q4 = qa[3 * N + l];
-    double d2 = q2 * d1, d3 = q3 * d1, d4 = q4 * d1;
-    double p = ph.gamm1 * (qa[4 * N + l] - 0.5 * (q2 * d2 + q3 * d3 + q4 * d4));
-    double d5 = GAM * p * d1;
-    b.de[l] = d1; b.de[N + l] = d2; b.de[2 * N + l] = d3; b.de[3 * N + l] = d4; b.de[4 * N + l] = d5;
-    b.p[l] = p;
-    b.ss[l] = c.srefp1dre * pow(d5, 1.5) / (d5 + c.srefoo);
+    const Prim r = primitives(c, ph, qa[l], qa[N + l], qa[2 * N + l], qa[3 * N + l], qa[4 * N + l]);
+    b.de[l] = r.d1; b.de[N + l] = r.d2; b.de[2 * N + l] = r.d3; b.de[3 * N + l] = r.d4; b.de[4 * N + l] = r.d5;
+    b.p[l] = r.p;
+    b.ss[l] = r.mu;
   }
 }
 
@@ -272,7 +283,9 @@ __global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
 // de(:,1) and de(:,5) through as temporaries that the flux derivatives
 // overwrite; here the stresses stay in registers.  tv is written where it is
 // read: at the wall rows of blocks with a wall surface on the first stage
-// (the surface forces), everywhere when a.opt is set.
+// (the surface forces), everywhere when a.opt is set.  u, v, w, p and the
+// viscosity are formed again from q with k_primitives' operations (equal to
+// what it left in de, p and ss; q is read here anyway).
 __device__ inline void flux_put(double *fx, int N, int l, int m, double r1, double r2, double r3) {
   fx[(size_t)(3 * (m - 1)) * N + l] = r1;
   fx[(size_t)(3 * (m - 1) + 1) * N + l] = r2;
@@ -288,14 +301,16 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
   const int visc = c.nviscous == 1;
   // rows of this block whose stresses the surface forces read
   const int wall_row = a.nk == 1 && b.nsurf > 0 ? (b.sside ? b.ny - 1 : 0) : -1;
-  const double *qa = b.qa, *xi = b.xim, *et = b.etm, *ze = b.zem, *p = b.p, *de = b.de;
-  const double *ss = b.ss, *ya = b.yaco;
+  const double *qa = b.qa, *xi = b.xim, *et = b.etm, *ze = b.zem;
+  const double *ya = b.yaco;
   double *fx = b.fx, *tv = b.tv;
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
     const double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
     const double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
     const double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
-    const double u = de[N + l], v = de[2 * N + l], ww = de[3 * N + l];
+    const double qv[5] = {qa[l], qa[N + l], qa[2 * N + l], qa[3 * N + l], qa[4 * N + l]};
+    const Prim pr = primitives(c, ph, qv[0], qv[1], qv[2], qv[3], qv[4]);
+    const double u = pr.d2, v = pr.d3, ww = pr.d4;
     double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (visc) {
       double cb[4][3];
@@ -311,16 +326,16 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
       double txy = cb[1][0], tyy = cb[1][1], hxx = cb[1][2];
       double hyy = cb[2][0], tyz = cb[2][1], tzz = cb[2][2];
       const double sx = cb[3][0], sy = cb[3][1], sz = cb[3][2];
-      double r1 = ss[l] * ya[l];
+      double r1 = pr.mu * ya[l];
       double r2 = ph.gamm1prndtli * r1, d5;
       if (c.les == 1) {
         double d1 = txx * txx + tyy * tyy + tzz * tzz + (hzz + txy) * (hzz + txy) + (hyy + tzx) * (hyy + tzx) +
                     (hxx + tyz) * (hxx + tyz);
         double varr = (double)(float)pow(-1 / ya[l], 1.5);   // single-precision work array
-        double r3 = qa[l] * (c.smago1 * c.smago1) * varr * sqrt(2 * d1);
+        double r3 = qv[0] * (c.smago1 * c.smago1) * varr * sqrt(2 * d1);
         r1 = r1 + r3 * ya[l];
         r2 = r2 + ph.tgamm1prndtli * r3 * ya[l];
-        r3 = ph.two_three * (qa[l] * c.smago2 * varr * d1);
+        r3 = ph.two_three * (qv[0] * c.smago2 * varr * d1);
         d5 = ph.two_three * (txx + tyy + tzz);
         double ntxx = r1 * (2 * txx - d5) - r3, ntyy = r1 * (2 * tyy - d5) - r3, ntzz = r1 * (2 * tzz - d5) - r3;
         double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);
@@ -344,14 +359,14 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
     const double s1 = x1 * q1 + x2 * q2 + x3 * q3;
     const double s2 = e1 * q1 + e2 * q2 + e3 * q3;
     const double s3 = z1 * q1 + z2 * q2 + z3 * q3;
-    const double pl = p[l];
+    const double pl = pr.p;
     {
-      const double q = qa[l];
+      const double q = qv[0];
       flux_put(fx, N, l, 1, q * s1, q * s2, q * s3);
     }
 #pragma unroll
     for (int m = 2; m <= 4; m++) {
-      const double q = qa[(size_t)(m - 1) * N + l];
+      const double q = qv[m - 1];
       const double mx = m == 2 ? x1 : m == 3 ? x2 : x3;
       const double me = m == 2 ? e1 : m == 3 ? e2 : e3;
       const double mz = m == 2 ? z1 : m == 3 ? z2 : z3;
@@ -368,7 +383,7 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
       flux_put(fx, N, l, m, r1, r2, r3);
     }
     {
-      const double d5 = qa[4 * N + l] + pl;
+      const double d5 = qv[4] + pl;
       double r1 = d5 * s1 - pl * (u0 * x1 + u1 * x2 + u2 * x3);
       double r2 = d5 * s2 - pl * (u0 * e1 + u1 * e2 + u2 * e3);
       double r3 = d5 * s3 - pl * (u0 * z1 + u1 * z2 + u2 * z3);
@@ -777,13 +792,6 @@ __global__ void __launch_bounds__(256) k_avg_apply(StageArgs a) {
 }
 
 // ---- copies ----
-__global__ void __launch_bounds__(256) k_copy5(StageArgs a) {   // qo = qa
-  WORK_POINTS
-  for (int v = 0; v < 5; v++)
-    for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) b.qo[(size_t)v * N + l] = b.qa[(size_t)v * N + l];
-}
-
-
 __global__ void __launch_bounds__(256) k_copy_field(StageArgs a) {  // rr(:,1) = field(arr, col)
   WORK_POINTS
   const double *s = field_ptr(b, a.arr, a.col);
@@ -970,7 +978,6 @@ void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
     case 1: LAUNCH(k_dt_reduce, nwg, 256, st, a); break;
     case 6: LAUNCH(k_sponge, nwg, 256, st, a); break;
     case 7: LAUNCH(k_rk, nwg, 256, st, a); break;
-    case 8: LAUNCH(k_copy5, nwg, 256, st, a); break;
     case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;
     case 10: LAUNCH(k_initialo, nwg, 256, st, a); break;
     case 11: LAUNCH(k_metrics, nwg, 256, st, a); break;
