@@ -291,8 +291,17 @@ int32_t tp_case_bytes(tp_context *ctx, int32_t case_id, int64_t *bytes);
  * in every later launch.  Its state is zero: follow with tp_initialo or
  * tp_upload.  On failure the case stays pending and the context unchanged. */
 int32_t tp_admit(tp_context *ctx, int32_t case_id);
+/* the same for n pending cases at once (one rebuild of the launch tables and
+ * one geometry pass for all of them); status[i] is 0 for an admitted case,
+ * else the error that left case_ids[i] pending.  Returns 0 when every case
+ * was admitted, else the first error; a failure after the cases were built
+ * (linking, geometry) leaves all of them pending. */
+int32_t tp_admit_cases(tp_context *ctx, int32_t n, const int32_t *case_ids, int32_t *status);
 /* drop a case (pending or on the device) and free its device memory */
 int32_t tp_remove_case(tp_context *ctx, int32_t case_id);
+/* the same for n cases at once (one rebuild of the launch tables); nothing
+ * is removed if any id is unknown */
+int32_t tp_remove_cases(tp_context *ctx, int32_t n, const int32_t *case_ids);
 /* out[0] state (0 pending, 1 on the device, 2 with geometry), out[1] points,
  * out[2] sub-blocks, out[3] device bytes held */
 int32_t tp_case_info(tp_context *ctx, int32_t case_id, int64_t *out4);

@@ -30,7 +30,7 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_array_count", "tp_get_scalars", "tp_set_scalars", "tp_mpigo", "tp_deriv",
            "tp_filte", "tp_filter_step", "tp_stage", "tp_step", "tp_step_graph", "tp_sync",
            "tp_profile", "tp_profile_read", "tp_info", "tp_set_option", "tp_autotune",
-           "tp_col_config", "tp_inflow_modes", "tp_gust_mode", "tp_case_bytes", "tp_admit",
+           "tp_col_config", "tp_inflow_modes", "tp_gust_mode", "tp_case_bytes", "tp_admit", "tp_admit_cases", "tp_remove_cases",
            "tp_remove_case", "tp_case_info", "tp_server_create", "tp_server_destroy",
            "tp_server_context", "tp_server_error", "tp_server_clock", "tp_server_submit",
            "tp_server_iterate", "tp_server_run", "tp_server_result", "tp_server_result_count",
@@ -78,7 +78,8 @@ def load(path=LIB_PATH):
         "tp_inflow_modes": [i32, ctypes.c_uint64, dbl, dbl, pd, dbl, dbl, pd],
         "tp_gust_mode": [dbl, dbl, pd, pd, pd],
         "tp_case_bytes": [P, i32, ctypes.POINTER(i64)],
-        "tp_admit": [P, i32], "tp_remove_case": [P, i32],
+        "tp_admit": [P, i32], "tp_remove_case": [P, i32], "tp_admit_cases": [P, i32, pi32, pi32],
+        "tp_remove_cases": [P, i32, pi32],
         "tp_case_info": [P, i32, ctypes.POINTER(i64)],
         "tp_server_create": [i64, i32, i32, ctypes.POINTER(P)],
         "tp_server_destroy": [P],
@@ -230,9 +231,25 @@ class Solver:
         is zero until initialo() or upload()."""
         _check(load().tp_admit(self.h, case))
 
+    def admit_cases(self, cases):
+        """Put several pending cases on the device at once (one rebuild of
+        the launch tables); returns the per-case status codes."""
+        ids = np.ascontiguousarray(cases, np.int32)
+        st = np.zeros(len(ids), np.int32)
+        rc = load().tp_admit_cases(self.h, len(ids), _ip(ids), _ip(st))
+        if rc and not st.any():
+            _check(rc)
+        return st
+
     def remove_case(self, case):
         _check(load().tp_remove_case(self.h, case))
         self.layouts.pop(case, None)
+
+    def remove_cases(self, cases):
+        ids = np.ascontiguousarray(cases, np.int32)
+        _check(load().tp_remove_cases(self.h, len(ids), _ip(ids)))
+        for c in cases:
+            self.layouts.pop(c, None)
 
     def case_info(self, case):
         out = (ctypes.c_int64 * 4)()

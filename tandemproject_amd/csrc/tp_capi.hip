@@ -911,9 +911,6 @@ Block block_shape(const HostCase &h, int lb, int *nwall) {
   return b;
 }
 
-// sponge points of block lb from its coordinates, SingleTurbulence3D.f90:209-249:
-// the point list with the strength sg and the streamwise factor lm of each
-// (the coefficients are (1/yaco) sg lm and (1/yaco) sg/(gam-1))
 // run fn(lo, hi, t) over [0, n) in contiguous ranges on nt host threads
 // (range t covers [t n/nt, (t+1) n/nt))
 template <class F>
@@ -935,6 +932,9 @@ int host_threads() {
   return (int)std::max(1u, std::min(hc, 16u));
 }
 
+// sponge points of block lb from its coordinates, SingleTurbulence3D.f90:209-249:
+// the point list with the strength sg and the streamwise factor lm of each
+// (the coefficients are (1/yaco) sg lm and (1/yaco) sg/(gam-1))
 void sponge_select(const HostCase &h, int lb, const Block &b, const double *x, const double *y,
                    const double *z, std::vector<int> &lst, std::vector<double> &sgv, std::vector<double> &lmv) {
   const double domlen = h.dp[TP_D_DOMLEN], szth1 = h.dp[TP_D_SZTH1], szth2 = h.dp[TP_D_SZTH2];
@@ -957,8 +957,10 @@ void sponge_select(const HostCase &h, int lb, const Block &b, const double *x, c
       for (int d = 0; d < 3; d++)
         r[d] = nsz[d][0] * szr[0][d] * std::fmax(szp[0][d] - cx[d], 0.0) +
                nsz[d][1] * szr[1][d] * std::fmax(cx[d] - szp[1][d], 0.0);
-      const double sg = szco * (1 - 0.125 * (1 + std::cos(kPi * r[0])) * (1 + std::cos(kPi * r[1])) *
-                                        (1 + std::cos(kPi * r[2])));
+      // 1 + cos(pi r) is exactly 2 at r = 0 (outside the band of that
+      // direction): most points need no cosine
+      const auto f = [](double rr) { return rr == 0.0 ? 2.0 : 1 + std::cos(kPi * rr); };
+      const double sg = szco * (1 - 0.125 * f(r[0]) * f(r[1]) * f(r[2]));
       if (sg > 0) {
         const double lm = ra1 + 0.5 * (1 - ra1) * (1 + std::cos(ra2 * (std::fmin(x[l], domlen) + domlen)));
         tl[t].push_back(l);
@@ -1972,43 +1974,82 @@ int32_t tp_case_bytes(tp_context *c, int32_t cs, int64_t *bytes) {
   return 0;
 }
 
-int32_t tp_admit(tp_context *c, int32_t cs) {
+int32_t tp_admit_cases(tp_context *c, int32_t n, const int32_t *ids, int32_t *status) {
   if (int rc = check_ready(c)) return rc;
-  const int ci = case_index(c, cs);
-  if (ci < 0) return fail(TP_E_ARG, "bad case");
-  if (c->cases[ci].built) return fail(TP_E_STATE, "case already admitted");
-  for (int lb = 0; lb < c->cases[ci].nb; lb++)
-    if ((int)c->cases[ci].grid.size() <= lb || c->cases[ci].grid[lb].empty())
-      return fail(TP_E_STATE, "tp_admit needs the grid of every block (tp_set_grid)");
-  int rc = build_case(c, c->cases[ci]);
-  if (!rc) rc = link(c);
+  if (n < 0 || (n > 0 && (!ids || !status))) return fail(TP_E_ARG, "tp_admit_cases: bad arguments");
+  int first = 0;
+  std::vector<int> built;   // case ids built here
+  for (int i = 0; i < n; i++) {
+    const int ci = case_index(c, ids[i]);
+    int rc = 0;
+    if (ci < 0) rc = fail(TP_E_ARG, "bad case " + std::to_string(ids[i]));
+    else if (c->cases[ci].built) rc = fail(TP_E_STATE, "case " + std::to_string(ids[i]) + " already admitted");
+    else {
+      for (int lb = 0; lb < c->cases[ci].nb && !rc; lb++)
+        if ((int)c->cases[ci].grid.size() <= lb || c->cases[ci].grid[lb].empty())
+          rc = fail(TP_E_STATE, "admitting case " + std::to_string(ids[i]) +
+                                    " needs the grid of every block (tp_set_grid)");
+      if (!rc) rc = build_case(c, c->cases[ci]);
+      if (rc) {
+        HostCase &h = c->cases[ci];
+        h.mem.release();
+        h.built = false;
+        h.geom = false;
+      } else {
+        built.push_back(ids[i]);
+      }
+    }
+    status[i] = rc;
+    if (rc && !first) first = rc;
+  }
+  if (built.empty()) return first;
+  int rc = link(c);
   if (!rc) rc = tp_setup_geometry(c);
   if (rc) {
-    // leave the context as it was: the case stays pending
+    // leave the context as it was: these cases stay pending
     const std::string msg = g_err;
     hipStreamSynchronize(c->stream);
-    HostCase &h = c->cases[ci];
-    h.mem.release();
-    h.built = false;
-    h.geom = false;
+    for (int id : built) {
+      HostCase &h = c->cases[case_index(c, id)];
+      h.mem.release();
+      h.built = false;
+      h.geom = false;
+    }
+    for (int i = 0; i < n; i++)
+      if (std::find(built.begin(), built.end(), ids[i]) != built.end()) status[i] = rc;
     link(c);
     g_err = msg;
+    return rc;
   }
-  return rc;
+  return first;   // tp_last_error holds the last failing case's message
 }
 
-int32_t tp_remove_case(tp_context *c, int32_t cs) {
+int32_t tp_admit(tp_context *c, int32_t cs) {
+  int32_t st = 0;
+  const int rc = tp_admit_cases(c, 1, &cs, &st);
+  return rc ? rc : st;
+}
+
+int32_t tp_remove_cases(tp_context *c, int32_t n, const int32_t *ids) {
   if (!c) return fail(TP_E_ARG, "null context");
-  const int ci = case_index(c, cs);
-  if (ci < 0) return fail(TP_E_ARG, "bad case");
+  if (n < 0 || (n > 0 && !ids)) return fail(TP_E_ARG, "tp_remove_cases: bad arguments");
+  for (int i = 0; i < n; i++)
+    if (case_index(c, ids[i]) < 0) return fail(TP_E_ARG, "bad case " + std::to_string(ids[i]));
   HIPCHK(hipStreamSynchronize(c->stream));
-  const bool built = c->cases[ci].built;
-  c->cases[ci].mem.release();
-  c->cases.erase(c->cases.begin() + ci);
+  bool any_built = false;
+  for (int i = 0; i < n; i++) {
+    const int ci = case_index(c, ids[i]);
+    if (ci < 0) continue;   // named twice
+    any_built = any_built || c->cases[ci].built;
+    c->cases[ci].mem.release();
+    c->cases.erase(c->cases.begin() + ci);
+  }
   if (c->cases.empty()) c->nkrk = 0;
-  if (built && c->finalized) return link(c);
+  if (any_built && c->finalized) return link(c);
   return 0;
 }
+
+int32_t tp_remove_case(tp_context *c, int32_t cs) { return tp_remove_cases(c, 1, &cs); }
 
 int32_t tp_case_info(tp_context *c, int32_t cs, int64_t *out4) {
   if (!c || !out4) return fail(TP_E_ARG, "null argument");

@@ -75,29 +75,38 @@ int64_t qa_count(tp_context *ctx, int cid, int nb) {
   return n;
 }
 
-// copy a finished case's state to the host and take it off the device
-int retire(tp_server *s, int r) {
-  Request &q = s->reqs[r];
-  q.qa.resize((size_t)qa_count(s->ctx, q.cid, q.nb));
-  int64_t off = 0;
-  for (int b = 0; b < q.nb; b++) {
-    const int64_t n = tp_array_count(s->ctx, q.cid, b, TP_QA);
-    if (int rc = tp_download(s->ctx, q.cid, b, TP_QA, q.qa.data() + off, n)) return rc;
-    off += n;
+// copy the finished cases' states to the host and take them off the device
+// (one rebuild of the launch tables for all of them)
+int retire(tp_server *s, const std::vector<int> &fin) {
+  std::vector<int32_t> ids;
+  for (int r : fin) {
+    Request &q = s->reqs[r];
+    q.qa.resize((size_t)qa_count(s->ctx, q.cid, q.nb));
+    int64_t off = 0;
+    for (int b = 0; b < q.nb; b++) {
+      const int64_t n = tp_array_count(s->ctx, q.cid, b, TP_QA);
+      if (int rc = tp_download(s->ctx, q.cid, b, TP_QA, q.qa.data() + off, n)) return rc;
+      off += n;
+    }
+    ids.push_back(q.cid);
   }
-  if (int rc = tp_remove_case(s->ctx, q.cid)) return rc;
+  if (int rc = tp_remove_cases(s->ctx, (int32_t)ids.size(), ids.data())) return rc;
   s->changed = true;
-  q.t_done = s->now();
-  q.state = 2;
-  s->used -= q.bytes;
-  s->active_points -= q.points;
-  s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+  for (int r : fin) {
+    Request &q = s->reqs[r];
+    q.t_done = s->now();
+    q.state = 2;
+    s->used -= q.bytes;
+    s->active_points -= q.points;
+    s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+  }
   return 0;
 }
 
 // admit arrived requests in arrival order while they fit and the cases on
 // the device hold fewer than the fill target's points; a request that does
-// not fit the budget is passed over for later, smaller ones
+// not fit the budget is passed over for later, smaller ones.  The chosen
+// requests join in one tp_admit_cases call (one rebuild of the launch tables).
 int admit(tp_server *s, int *nadm) {
   *nadm = 0;
   const double t = s->now();
@@ -107,19 +116,37 @@ int admit(tp_server *s, int *nadm) {
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return s->reqs[a].t_arr < s->reqs[b].t_arr; });
   const bool busy = !s->active.empty();
+  std::vector<int> pick;
+  std::vector<int32_t> ids;
+  int64_t used = s->used, points = s->active_points;
+  size_t nact = s->active.size();
   for (int r : order) {
-    Request &q = s->reqs[r];
-    if ((int)s->active.size() >= s->max_active) break;
-    if (s->fill > 0 && !s->active.empty() && s->active_points >= s->fill) break;
-    if (s->used + q.bytes > s->budget) continue;
-    const double ta = s->now();
-    int rc = tp_admit(s->ctx, q.cid);
-    if (!rc) rc = tp_initialo(s->ctx, q.cid);
-    s->admit_s += s->now() - ta;
-    s->changed = true;
-    if (rc) {
+    const Request &q = s->reqs[r];
+    if ((int)nact >= s->max_active) break;
+    if (s->fill > 0 && nact > 0 && points >= s->fill) break;
+    if (used + q.bytes > s->budget) continue;
+    pick.push_back(r);
+    ids.push_back(q.cid);
+    used += q.bytes;
+    points += q.points;
+    nact++;
+  }
+  if (pick.empty()) return 0;
+  const double ta = s->now();
+  std::vector<int32_t> st(pick.size(), 0);
+  tp_admit_cases(s->ctx, (int32_t)ids.size(), ids.data(), st.data());
+  std::vector<std::string> err(pick.size());
+  for (size_t i = 0; i < pick.size(); i++) {
+    if (st[i]) err[i] = tp_last_error();
+    else if ((st[i] = tp_initialo(s->ctx, ids[i]))) err[i] = tp_last_error();
+  }
+  s->admit_s += s->now() - ta;
+  s->changed = true;
+  for (size_t i = 0; i < pick.size(); i++) {
+    Request &q = s->reqs[pick[i]];
+    if (st[i]) {
       q.state = 3;
-      q.error = tp_last_error();
+      q.error = err[i];
       tp_remove_case(s->ctx, q.cid);
       continue;
     }
@@ -128,7 +155,7 @@ int admit(tp_server *s, int *nadm) {
     s->used += q.bytes;
     s->active_points += q.points;
     s->peak_used = std::max(s->peak_used, s->used);
-    s->active.push_back(r);
+    s->active.push_back(pick[i]);
     s->max_seen = std::max(s->max_seen, (int)s->active.size());
     if (busy) s->admitted_while_busy++;
     (*nadm)++;
@@ -165,8 +192,8 @@ int iterate(tp_server *s, int max_steps, int32_t *status) {
     for (int r : s->active)
       if (s->reqs[r].steps >= s->reqs[r].nsteps) fin.push_back(r);
     const double tr = s->now();
-    for (int r : fin)
-      if (int rc = retire(s, r)) return srv_fail(s, rc, std::string("retire: ") + tp_last_error());
+    if (!fin.empty())
+      if (int rc = retire(s, fin)) return srv_fail(s, rc, std::string("retire: ") + tp_last_error());
     s->retire_s += s->now() - tr;
   }
   if (status) {

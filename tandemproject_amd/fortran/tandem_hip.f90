@@ -314,6 +314,19 @@ module tandem_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: case_id
     end function
+    integer(c_int32_t) function tp_admit_cases(ctx, n, case_ids, status) bind(C, name="tp_admit_cases")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(in) :: case_ids(*)
+      integer(c_int32_t), intent(out) :: status(*)
+    end function
+    integer(c_int32_t) function tp_remove_cases(ctx, n, case_ids) bind(C, name="tp_remove_cases")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(in) :: case_ids(*)
+    end function
     integer(c_int32_t) function tp_remove_case(ctx, case_id) bind(C, name="tp_remove_case")
       import :: c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -104,6 +104,42 @@ def test_admit_and_remove_around_a_running_case():
     assert sc_a["n"] == 10 and sc_b["n"] == 10
 
 
+def test_admit_cases_at_once():
+    """Two cases admitted in one call (one rebuild of the launch tables) next
+    to a running case, with a third in the same call that has no grid: that
+    one fails alone and stays pending; the others equal their runs alone."""
+    say("tp_admit_cases: two cases and one without its grid, next to a running case")
+    (la, sa, pa), (lb, sb, pb), (lc, sc_, pc) = CASES[0], CASES[1], CASES[2]
+    s = api.Solver([(la, pa)], exact_lines=True)
+    for b in range(la.nb):
+        s.set_grid(0, b, sa[b])
+    s.setup_geometry()
+    s.initialo(0)
+    s.step(4)
+    cb = s.add_case(lb, pb)
+    cd = s.add_case(lb, pb)
+    cc = s.add_case(lc, pc)
+    for b in range(lb.nb):
+        s.set_grid(cb, b, sb[b])
+    for b in range(lc.nb):
+        s.set_grid(cc, b, sc_[b])
+    st = s.admit_cases([cb, cd, cc])
+    assert st[0] == 0 and st[1] != 0 and st[2] == 0, st
+    assert s.case_info(cd)["state"] == 0 and s.case_info(cd)["device_bytes"] == 0
+    assert s.info()["cases"] == 3
+    s.initialo(cb); s.initialo(cc)
+    s.step(6)
+    s.sync()
+    for cid, (lay, ss, p) in ((cb, CASES[1]), (cc, CASES[2])):
+        ref, rsc = alone(lay, ss, p, 6)
+        for b in range(lay.nb):
+            np.testing.assert_array_equal(s.download(cid, b, "qa"), ref[b])
+        assert s.scalars(cid)["n"] == rsc["n"] == 6
+    ref, _ = alone(la, sa, pa, 10)
+    for b in range(la.nb):
+        np.testing.assert_array_equal(s.download(0, b, "qa"), ref[b])
+
+
 @pytest.mark.parametrize("mode", ["sequential-graph", "sequential-eager", "chunked-graph"])
 def test_server_mix_equals_alone(mode):
     """Four cases (two with inflow disturbances) through the request loop with
