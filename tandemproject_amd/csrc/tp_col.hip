@@ -260,7 +260,7 @@ __device__ inline void bwd_sweep(double (&r)[C], int n, const ChunkLU &f, double
 template <int D, int KIND, int C>
 __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   extern __shared__ double lds[];
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const int L = b.L[D];

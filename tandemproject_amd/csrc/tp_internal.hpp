@@ -116,6 +116,17 @@ struct Work {
   int d;      // direction (line and face launches), side in bit 8
 };
 
+// Work item of workgroup bid out of n.  Workgroups go to the GPU's 8 XCDs in
+// turn (bid mod 8), each XCD with its own L2; this hands each XCD a
+// contiguous run of work items instead (a bijection on [0, n)), so that
+// neighbouring items -- adjacent lines or face points, whose edge cache lines
+// they share -- meet in the same L2.  Not for kernels whose per-workgroup
+// outputs are summed by work-item ranges (those index by blockIdx.x).
+__device__ inline int xcd_item(int bid, int n) {
+  const int x = bid & 7, k = bid >> 3, per = n >> 3, rem = n & 7;
+  return x * per + min(x, rem) + k;
+}
+
 // Face launches: the face point jk of this thread (256 threads per work item).
 // A xi or zeta face takes 256 consecutive face indices: on a zeta face they
 // are consecutive in memory, on a xi face each is its own row.  An eta face

@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
 template <int NT>
 __global__ void __launch_bounds__(256) k_exchange(ExchArgs a) {
   constexpr int MP = NT == 0 ? LMD : LMF;
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const int d = w.d & 0xff, ip = w.d >> 8;
   const Block &b = a.blocks[w.blk];
   int jk;

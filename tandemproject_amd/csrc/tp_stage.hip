@@ -538,7 +538,7 @@ __device__ inline void face_jk(const Block &b, int d, int jk, int &j, int &k) {
 
 // ---- GCBC/GCIC preparation, Main3D.f90:780-795 ----
 __global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
   int jk;
@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
 // ---- face copy from the neighbour: drvb(:,:,ip) <- drva(:,:,1-ip) of ncd
 // (GCIC exchange Main3D.f90:799-813 and interface averaging :915-925) ----
 __global__ void __launch_bounds__(256) k_face_copy(StageArgs a) {
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const int d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
   int jk;
@@ -592,7 +592,7 @@ __device__ inline void point_ijk(const Block &b, int l, int &x, int &y, int &z) 
 
 // ---- GCBC/GCIC application along direction d (both ends), Main3D.f90:817-869 ----
 __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
@@ -668,7 +668,7 @@ __global__ void __launch_bounds__(256) k_rk(StageArgs a) {
 
 // ---- wall temperature and velocity, Main3D.f90:886-895 ----
 __global__ void __launch_bounds__(256) k_wall(StageArgs a) {
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const Phys ph = phys();
@@ -723,7 +723,7 @@ __device__ inline bool avg_code(int np) { return np == 30 || np == 35 || np == 4
 
 // capture face values into drva (first loop of the reference)
 __global__ void __launch_bounds__(256) k_avg_capture(StageArgs a) {
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
   int jk;
@@ -759,7 +759,7 @@ __device__ inline double avg_count(const Block &b, const CaseCoef &c, int x, int
 }
 
 __global__ void __launch_bounds__(256) k_avg_apply(StageArgs a) {
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
@@ -828,7 +828,7 @@ __global__ void __launch_bounds__(256) k_metrics(StageArgs a) {
 
 // ---- face unit normals, Main3D.f90:429-439 ----
 __global__ void __launch_bounds__(256) k_normals(StageArgs a) {
-  const Work w = a.work[blockIdx.x];
+  const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
   int jk;
