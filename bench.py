@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel events)")
+    ap.add_argument("--graph-steps", type=int, default=20,
+                    help="steps of the extra timed pass of graph replays (0: none)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default line-solver launch shapes")
     ap.add_argument("--ensemble-cases", type=int, default=64, help="0: skip the ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=40)
@@ -372,6 +374,20 @@ def main():
     finite = bool(np.isfinite(q).all())
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s; n={sc['n']:.0f} dt={sc['dt']:.3e} finite={finite}",
           file=sys.stderr, flush=True)
+    # the same number of steps again as replays of the captured step (one
+    # graph launch per step, no per-kernel events): the rate a host that
+    # drives the solver through tp_step_graph sees
+    graph_replay = None
+    if not args.graph and args.graph_steps > 0:
+        s.step(1, graph=True)   # capture
+        s.sync()
+        tg0 = time.perf_counter()
+        s.step(args.graph_steps, graph=True)
+        s.sync()
+        tg = time.perf_counter() - tg0
+        graph_replay = {"value": round(npts * args.graph_steps / tg / 1e6, 2), "steps": args.graph_steps,
+                        "ms_per_step": round(tg / args.graph_steps * 1e3, 3)}
+        log(f"graph replay: {graph_replay['value']} Mpoint-steps/s")
 
     value = npts * args.steps / elapsed / 1e6
     roof = None
@@ -418,7 +434,7 @@ def main():
                    "points": npts, "sub_blocks": lay.nb, "graph": bool(args.graph),
                    "line_chunks": line_chunks,
                    "ms_per_step_by_class": breakdown},
-        "roofline": roof, "cpu_baseline": cpu, "ensemble": ens, "serving": srv,
+        "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph_replay, "ensemble": ens, "serving": srv,
     }
     print(json.dumps(out))
 
