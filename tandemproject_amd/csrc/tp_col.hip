@@ -10,9 +10,10 @@
 // sub-block; its P waves own consecutive chunks of ceil(L/P) points of those
 // lines.  For eta and zeta lines the 64 lanes read and write 64 consecutive
 // doubles of each row (the lines are adjacent in xi), so no staging is needed;
-// xi lines are contiguous rows and go through a small per-wave LDS transpose
-// in pieces of 8 points.  The factors of a row are the same for every lane
-// and are read with scalar loads.
+// xi lines are contiguous rows: their windows come through a small per-wave
+// LDS transpose in pieces of 8 points, and their results are gathered in LDS
+// and stored by the whole workgroup as contiguous runs of rows.  The factors
+// of a row are the same for every lane and are read from LDS as broadcasts.
 //
 // Each wave keeps its chunk in registers and runs:
 //   forward sweep from zero carries -> the pair (y[e-1], y[e]) that is an
@@ -176,6 +177,12 @@ __device__ inline double filter_bnd_row(const Block &b, const CaseCoef &c, int D
 
 constexpr int COL_TR = 65;   // row pitch (doubles) of the xi transpose buffer
 constexpr int COL_DECAY = 48;  // points a carry travels sequentially before an amplifying closure
+
+// dynamic LDS of a k_col launch, in doubles: carry pairs [2][P][64] (double2),
+// chunk factors [P][192], and for xi the transposes [P][8][COL_TR]
+__host__ __device__ constexpr int col_lds_doubles(int d, int P) {
+  return 2 * P * 64 * 2 + P * 192 + (d == 0 ? P * 8 * COL_TR : 0);
+}
 
 // a zero the optimiser cannot see through: offsetting the factor pointer by
 // it at each sweep keeps the factor loads inside the sweep instead of being
@@ -498,7 +505,37 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   const bool add = KIND == 1 || a.acc;
   const double *ob = KIND == 1 ? src : dst;
   double *cpy = KIND == 1 && a.copy_arr >= 0 ? field_ptr(b, a.copy_arr, a.copy_col + fi * a.out_step) : nullptr;
-  if (D == 0) {
+  // xi: the work item's lines are consecutive rows of the block, one
+  // contiguous span of nl * L doubles.  Once every wave is done with the
+  // carry, factor and transpose areas, the whole LDS holds R of these rows at
+  // a time (R the largest power of two that fits, at least 8), and the
+  // workgroup stores them as one run of whole cache lines: a wave's own
+  // pieces (8 rows x 8 points) would straddle 64-byte write granules.
+  const int pitch = L + 1;
+  int R = 64;
+  while (R >= 8 && R * pitch > col_lds_doubles(0, P)) R >>= 1;
+  if (D == 0 && R >= 8) {
+    double *tile = lds;
+    __syncthreads();
+    for (int h = 0; h * R < nl; h++) {
+      if (act && lane >= h * R && lane < (h + 1) * R) {
+        double *trow = tile + (lane - h * R) * pitch + s0;
+#pragma unroll
+        for (int k = 0; k < C; k++)
+          if (k < n) trow[k] = r[k];
+      }
+      __syncthreads();
+      const int rows = min(R, nl - h * R);
+      const unsigned base0 = (unsigned)(w.a + h * R) * (unsigned)b.nx;
+      for (int idx = threadIdx.x; idx < rows * L; idx += blockDim.x) {
+        const int row = idx / L;
+        const unsigned o = 8u * (base0 + (unsigned)idx);
+        const double x = tile[row * pitch + idx - row * L];
+        at(dst, o) = add ? at(ob, o) + x : x;
+      }
+      __syncthreads();   // the tile is read before the next rows overwrite it
+    }
+  } else if (D == 0) {
 #pragma unroll
     for (int t = 0; t < (C + 7) / 8; t++) {
       if (8 * t >= n) break;   // pieces past the chunk hold no rows of it
@@ -546,10 +583,7 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   }
 }
 
-size_t col_lds_bytes(int d, int P) {
-  return (size_t)(2 * P * 64) * sizeof(double2) + (size_t)P * 192 * sizeof(double) +
-         (d == 0 ? (size_t)P * 8 * COL_TR * sizeof(double) : 0);
-}
+size_t col_lds_bytes(int d, int P) { return (size_t)col_lds_doubles(d, P) * sizeof(double); }
 
 template <int D, int KIND>
 static void launch_col_c(int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a) {
