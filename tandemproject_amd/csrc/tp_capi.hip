@@ -433,12 +433,14 @@ struct DevList {
 // lines use P chunks (P = 0: the sequential kernel) of capacity C points
 struct ColGroup {
   int P = 0, C = 0;
+  bool self = false;   // a block of the group has a closure face onto itself
   long long npts = 0;
   DevList work;
 };
 
 struct ListSet {
   DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  DevList exch_x;   // the closure faces whose neighbour is another block
   DevList surf;                     // wall-surface points, 256 per item
   std::vector<ColGroup> cg[3][2];   // line-solve launches per direction and kind
   std::vector<int> wr;   // first point-work item of each linked case, then the end
@@ -580,20 +582,25 @@ StageArgs base_args(tp_context *c) {
 
 // closure sums of nf fields (columns col0 + f fstep, f < nf) in one launch,
 // into recv slots 0..nf-1
-void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0, int nf = 1, int fstep = 0) {
-  if (!c->L.exch.n) return;
+// others_only: only the faces whose neighbour is another block (the line
+// solves form the sums of a block's faces onto itself)
+void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0, int nf = 1, int fstep = 0,
+                  bool others_only = false) {
+  const DevList &L = others_only ? c->L.exch_x : c->L.exch;
+  if (!L.n) return;
   Timed t(c, K_EXCH, 0.0);
   ExchArgs a;
-  a.blocks = c->d_blocks; a.work = c->L.exch.d; a.coef = c->d_coef;
+  a.blocks = c->d_blocks; a.work = L.d; a.coef = c->d_coef;
   a.nt = nt; a.nrt = nrt; a.n45 = n45; a.arr = arr; a.col0 = col0; a.fstep = fstep;
-  launch_exchange(c->L.exch.n, nf, c->stream, a);
+  launch_exchange(L.n, nf, c->stream, a);
 }
 
 // line solves of direction d over nf fields in one launch: field f reads
 // column in_col + f in_step, writes out_col + f out_step, uses recv slot f
 // (filled by an exchange over the same fields) and captures drva column m + f
 void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_arr, int out_col, int m,
-              int acc, int nf = 1, int in_step = 0, int out_step = 0, int copy_arr = -1, int copy_col = 0) {
+              int acc, int nf = 1, int in_step = 0, int out_step = 0, int copy_arr = -1, int copy_col = 0,
+              bool direct = false) {
   if (!c->L.lines[d].n) return;
   // algorithmic bytes: one field in, one out; an accumulated output is also read
   Timed t(c, kind == 0 ? K_DERIV : K_FILTER, (acc ? 24.0 : 16.0) * (double)c->npts * nf);
@@ -603,6 +610,8 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   a.m = m; a.acc = acc; a.nlp = c->nlp[d];
   a.in_step = in_step; a.out_step = out_step;
   a.copy_arr = copy_arr; a.copy_col = copy_col;
+  a.direct = direct ? 1 : 0;
+  a.n45 = 0;
   if (c->exact_lines) {
     if (kind == 0) launch_deriv(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
     else launch_filter(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
@@ -611,6 +620,7 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   for (const ColGroup &g : c->L.cg[d][kind]) {
     if (!g.work.n) continue;
     a.work = g.work.d;
+    a.direct = direct && g.self ? 1 : 0;
     if (g.P) launch_col(d, kind, g.C, g.P, g.work.n, nf, c->stream, a);
     else if (kind == 0) launch_deriv(d, g.work.n, nf, c->lds[d], c->stream, a);
     else launch_filter(d, g.work.n, nf, c->lds[d], c->stream, a);
@@ -673,6 +683,8 @@ int make_groups(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
           groups.back().C = C;
           works.emplace_back();
         }
+        for (int ip = 0; ip < 2; ip++)
+          if (b.ndf[d][ip] && b.ncd[d][ip] == g) groups[k].self = true;
         const int nlines = b.N / b.L[d];
         const int step = P ? 64 : c->nlp[d] - 1;
         for (int q = 0; q < nlines; q += step) works[k].push_back({g, q, std::min(step, nlines - q), d});
@@ -700,11 +712,28 @@ int regroup(tp_context *c) {
   return 0;
 }
 
+// the chunked line solves of this kind can form the closure sums of a block's
+// faces onto itself (periodic lines within one block) themselves, reading the
+// other end of the same lines, which the workgroup's other waves load at the
+// same time (every launch group chunked; the sequential solves read the
+// exchange's recv slots).  The exchange then covers only the faces between
+// blocks.  Only for solves whose input no earlier solve of the same exchange
+// changed (the derivatives; not the in-place filters).
+bool direct_ok(const tp_context *c, int kind) {
+  if (c->exact_lines) return false;
+  for (int d = 0; d < 3; d++)
+    for (const ColGroup &g : c->L.cg[d][kind])
+      if (g.work.n && !g.P) return false;
+  return true;
+}
+
 void enqueue_filter_step(tp_context *c) {
   // src/Main3D.f90:555-560: per field m, mpigo then the xi, eta, zeta filters
   // in place; the five fields are independent, so each of these steps runs
   // over all five in one launch; the last also leaves the result in qo
   // (qo = qa, :561)
+  // (the filters work in place, so the eta and zeta filters take the closure
+  // sums of the unfiltered field: one exchange before the three, as mpigo)
   run_exchange(c, 1, 1, 0, F_QA, 0, 5, 1);
   run_line(c, 1, 0, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
   run_line(c, 1, 1, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
@@ -725,10 +754,11 @@ void enqueue_stage(tp_context *c, int nk) {
     // src/Main3D.f90:627-651: gradients of u, v, w, T (de(:,2:5)) along the
     // three index directions into fx(:,3f+d) -- the four fields in one launch
     // per direction
-    run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1);
-    run_line(c, 0, 2, F_DE, 1, F_FX, 2, 0, 0, 4, 1, 3);
-    run_line(c, 0, 1, F_DE, 1, F_FX, 1, 0, 0, 4, 1, 3);
-    run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3);
+    const bool dir = direct_ok(c, 0);
+    run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1, dir);
+    run_line(c, 0, 2, F_DE, 1, F_FX, 2, 0, 0, 4, 1, 3, -1, 0, dir);
+    run_line(c, 0, 1, F_DE, 1, F_FX, 1, 0, 0, 4, 1, 3, -1, 0, dir);
+    run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3, -1, 0, dir);
   }
   // src/Main3D.f90:653-768: stresses and heat fluxes, then all 15 flux
   // components, in one pass over the gradients; then
@@ -745,10 +775,11 @@ void enqueue_stage(tp_context *c, int nk) {
     launch_surf_reduce(c->d_coef, c->d_sc, c->d_wpart, c->d_wwr, ncase, c->stream);
   }
   // the five fields m in one launch per step: flux components fx(:,3(m-1)+d)
-  run_exchange(c, 0, 0, 0, F_FX, 0, 5, 3);
-  run_line(c, 0, 0, F_FX, 0, F_DE, 0, 1, 0, 5, 3, 1);
-  run_line(c, 0, 1, F_FX, 1, F_DE, 0, 1, 1, 5, 3, 1);
-  run_line(c, 0, 2, F_FX, 2, F_DE, 0, 1, 1, 5, 3, 1);
+  const bool dirf = direct_ok(c, 0);
+  run_exchange(c, 0, 0, 0, F_FX, 0, 5, 3, dirf);
+  run_line(c, 0, 0, F_FX, 0, F_DE, 0, 1, 0, 5, 3, 1, -1, 0, dirf);
+  run_line(c, 0, 1, F_FX, 1, F_DE, 0, 1, 1, 5, 3, 1, -1, 0, dirf);
+  run_line(c, 0, 2, F_FX, 2, F_DE, 0, 1, 1, 5, 3, 1, -1, 0, dirf);
   if (c->L.sponge.n) {
     Timed t(c, K_MISC, 0.0);
     StageArgs s = a;
@@ -1090,7 +1121,7 @@ int build_case(tp_context *c, HostCase &h) {
 
 // launch tables over the global blocks gs (sorted by case)
 int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
-  std::vector<Work> pts, sp, lw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm, sf;
+  std::vector<Work> pts, sp, lw[3], ex, exx, gp, ga[3], gc, wl, ac, aa[3], nm, sf;
   L.wr.assign(c->linked.size() + 1, 0);
   L.wwr.assign(c->linked.size() + 1, 0);
   int slot = -1;
@@ -1127,6 +1158,7 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
         const bool gc_face = np == 10 || np == 20 || np == 25 || np == 30;
         const bool wall = np == 20 || np == 25;
         if (b.ndf[d][s]) add(ex);
+        if (b.ndf[d][s] && b.ncd[d][s] != g) add(exx);
         if (gc_face) { add(gp); add(ga[d]); }
         if (np == 30 || (gcic && wall)) add(gc);
         if (wall) add(wl);
@@ -1144,6 +1176,7 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
   if ((rc = upload_list(m, L.points, pts))) return rc;
   if ((rc = upload_list(m, L.sponge, sp))) return rc;
   if ((rc = upload_list(m, L.exch, ex))) return rc;
+  if ((rc = upload_list(m, L.exch_x, exx))) return rc;
   if ((rc = upload_list(m, L.gprep, gp))) return rc;
   if ((rc = upload_list(m, L.gcopy, gc))) return rc;
   if ((rc = upload_list(m, L.wall, wl))) return rc;

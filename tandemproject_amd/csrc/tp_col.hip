@@ -62,12 +62,26 @@ struct GLine {
   __device__ double operator()(int i) const { return p[(size_t)i * s]; }
 };
 
-// boundary rows of the derivative RHS (rows 0..2 and L-3..L-1)
-template <class SA>
-__device__ inline double deriv_bnd_row(const Block &b, const CaseCoef &c, int D, int i, int L, int jk,
-                                       const SA &sa, const double *rcv) {
+// closure sums r0, r1, r2 of one face: read from the exchange's recv slots
+// where they are used, or held as values
+struct RcvFace {
+  const double *rcv;
+  int F, jk, ip;
+  __device__ double operator()(int q) const { return rcv[(ip * 3 + q) * F + jk]; }
+};
+struct ValFace {
+  double r[3];
+  // a select, not an index: q may be known only at run time, and an indexed
+  // register array would go to scratch memory
+  __device__ double operator()(int q) const { return q == 0 ? r[0] : q == 1 ? r[1] : r[2]; }
+};
+
+// boundary rows of the derivative RHS (rows 0..2 and L-3..L-1); fs, fe: the
+// closure sums received at the start and end faces
+template <class SA, class FA>
+__device__ inline double deriv_bnd_row(const Block &b, const CaseCoef &c, int D, int i, int L, const SA &sa,
+                                       const FA &fs, const FA &fe) {
   const double aa = 0.6431406736919156, ab = 0.2586011023495066, ac = 0.007140953479797375;
-  const int F = b.F[D];
   const int ie = L - 1;
   if (i <= 2) {
     if (b.ndf[D][0] == 0) {
@@ -88,9 +102,9 @@ __device__ inline double deriv_bnd_row(const Block &b, const CaseCoef &c, int D,
     if (i <= 1) {
       double t = 0;
       for (int q = 0; q <= LMD; q++) t = t + c.pbci[0][i][q] * sa(q);
-      return t + rcv[(0 * 3 + i) * F + jk];
+      return t + fs(i);
     }
-    return aa * (sa(3) - sa(1)) + ab * (sa(4) - sa(0)) + ac * (sa(5) - rcv[(0 * 3 + 2) * F + jk]);
+    return aa * (sa(3) - sa(1)) + ab * (sa(4) - sa(0)) + ac * (sa(5) - fs(2));
   }
   const int row = ie - i;  // 0: last row
   if (b.ndf[D][1] == 0) {
@@ -112,16 +126,15 @@ __device__ inline double deriv_bnd_row(const Block &b, const CaseCoef &c, int D,
   if (row <= 1) {
     double t = 0;
     for (int q = 0; q <= LMD; q++) t = t + c.pbci[0][row][q] * sa(ie - q);
-    return -t - rcv[(1 * 3 + row) * F + jk];
+    return -t - fe(row);
   }
-  return aa * (sa(ie - 1) - sa(ie - 3)) + ab * (sa(ie) - sa(ie - 4)) + ac * (rcv[(1 * 3 + 2) * F + jk] - sa(ie - 5));
+  return aa * (sa(ie - 1) - sa(ie - 3)) + ab * (sa(ie) - sa(ie - 4)) + ac * (fe(2) - sa(ie - 5));
 }
 
 // boundary rows of the filter RHS
-template <class SA>
-__device__ inline double filter_bnd_row(const Block &b, const CaseCoef &c, int D, int i, int L, int jk,
-                                        const SA &sa, const double *rcv) {
-  const int F = b.F[D];
+template <class SA, class FA>
+__device__ inline double filter_bnd_row(const Block &b, const CaseCoef &c, int D, int i, int L, const SA &sa,
+                                        const FA &fs, const FA &fe) {
   const int ie = L - 1;
   const double fa = c.fa, fb = c.fb, fc = c.fc;
   if (i <= 2) {
@@ -143,10 +156,10 @@ __device__ inline double filter_bnd_row(const Block &b, const CaseCoef &c, int D
     if (i <= 1) {
       double t = 0;
       for (int q = 0; q <= LMF; q++) t = t + c.pbci[1][i][q] * sa(q);
-      return t + rcv[(0 * 3 + i) * F + jk];
+      return t + fs(i);
     }
     const double ra2 = 2.0 * sa(2);
-    return fa * (sa(1) + sa(3) - ra2) + fb * (sa(0) + sa(4) - ra2) + fc * (rcv[(0 * 3 + 2) * F + jk] + sa(5) - ra2);
+    return fa * (sa(1) + sa(3) - ra2) + fb * (sa(0) + sa(4) - ra2) + fc * (fs(2) + sa(5) - ra2);
   }
   const int row = ie - i;
   if (b.ndf[D][1] == 0) {
@@ -167,11 +180,11 @@ __device__ inline double filter_bnd_row(const Block &b, const CaseCoef &c, int D
   if (row <= 1) {
     double t = 0;
     for (int q = 0; q <= LMF; q++) t = t + c.pbci[1][row][q] * sa(ie - q);
-    return t + rcv[(1 * 3 + row) * F + jk];
+    return t + fe(row);
   }
   const double ra2 = 2.0 * sa(ie - 2);
   return fa * (sa(ie - 3) + sa(ie - 1) - ra2) + fb * (sa(ie - 4) + sa(ie) - ra2) +
-         fc * (sa(ie - 5) + rcv[(1 * 3 + 2) * F + jk] - ra2);
+         fc * (sa(ie - 5) + fe(2) - ra2);
 }
 
 
@@ -264,7 +277,8 @@ __device__ inline void bwd_sweep(double (&r)[C], int n, const ChunkLU &f, double
   }
 }
 
-template <int D, int KIND, int C>
+// DIRECT: the launch forms the closure sums of faces onto the same block (a.direct)
+template <int D, int KIND, int C, bool DIRECT>
 __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   extern __shared__ double lds[];
   const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
@@ -374,30 +388,54 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   // ---- right-hand side ----
   // rows 0..2 and L-3..L-1 use the closure formulas, evaluated once, outside
   // the unrolled loop, by the (at most two) chunks that hold them
+  // the closure rows at the line ends; the closure sums of a face: in a
+  // DIRECT launch those of a block's faces onto itself formed here from the
+  // other end of the same lines, as the exchange forms them; otherwise read
+  // from the exchange's recv slots
   double rb0 = 0.0, rb1 = 0.0, rb2 = 0.0, re2 = 0.0, re1 = 0.0, re0 = 0.0;
   {
     const GLine g{src + base, stride};
-    if (n > 0 && s0 <= 2) {
-      if (KIND == 0) {
-        rb0 = deriv_bnd_row(b, c, D, 0, L, jk, g, rcv);
-        rb1 = deriv_bnd_row(b, c, D, 1, L, jk, g, rcv);
-        rb2 = deriv_bnd_row(b, c, D, 2, L, jk, g, rcv);
-      } else {
-        rb0 = filter_bnd_row(b, c, D, 0, L, jk, g, rcv);
-        rb1 = filter_bnd_row(b, c, D, 1, L, jk, g, rcv);
-        rb2 = filter_bnd_row(b, c, D, 2, L, jk, g, rcv);
+    auto rows = [&](const auto &fs, const auto &fe) {
+      if (n > 0 && s0 <= 2) {
+        if (KIND == 0) {
+          rb0 = deriv_bnd_row(b, c, D, 0, L, g, fs, fe);
+          rb1 = deriv_bnd_row(b, c, D, 1, L, g, fs, fe);
+          rb2 = deriv_bnd_row(b, c, D, 2, L, g, fs, fe);
+        } else {
+          rb0 = filter_bnd_row(b, c, D, 0, L, g, fs, fe);
+          rb1 = filter_bnd_row(b, c, D, 1, L, g, fs, fe);
+          rb2 = filter_bnd_row(b, c, D, 2, L, g, fs, fe);
+        }
       }
-    }
-    if (n > 0 && s0 + n - 1 >= L - 3) {
-      if (KIND == 0) {
-        re2 = deriv_bnd_row(b, c, D, L - 3, L, jk, g, rcv);
-        re1 = deriv_bnd_row(b, c, D, L - 2, L, jk, g, rcv);
-        re0 = deriv_bnd_row(b, c, D, L - 1, L, jk, g, rcv);
-      } else {
-        re2 = filter_bnd_row(b, c, D, L - 3, L, jk, g, rcv);
-        re1 = filter_bnd_row(b, c, D, L - 2, L, jk, g, rcv);
-        re0 = filter_bnd_row(b, c, D, L - 1, L, jk, g, rcv);
+      if (n > 0 && s0 + n - 1 >= L - 3) {
+        if (KIND == 0) {
+          re2 = deriv_bnd_row(b, c, D, L - 3, L, g, fs, fe);
+          re1 = deriv_bnd_row(b, c, D, L - 2, L, g, fs, fe);
+          re0 = deriv_bnd_row(b, c, D, L - 1, L, g, fs, fe);
+        } else {
+          re2 = filter_bnd_row(b, c, D, L - 3, L, g, fs, fe);
+          re1 = filter_bnd_row(b, c, D, L - 2, L, g, fs, fe);
+          re0 = filter_bnd_row(b, c, D, L - 1, L, g, fs, fe);
+        }
       }
+    };
+    const int F = b.F[D];
+    if (DIRECT) {
+      ValFace fv[2] = {{{0.0, 0.0, 0.0}}, {{0.0, 0.0, 0.0}}};
+#pragma unroll
+      for (int ip = 0; ip < 2; ip++) {
+        const bool need = ip == 0 ? n > 0 && s0 <= 2 : n > 0 && s0 + n - 1 >= L - 3;
+        if (!need || !b.ndf[D][ip]) continue;
+        if (b.ncd[D][ip] == w.blk) {   // the block's own lines (a periodic face)
+          closure_sums<KIND>(b, b, c, src, src, D, ip, jk, a.n45, fv[ip].r[0], fv[ip].r[1], fv[ip].r[2]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 3; q++) fv[ip].r[q] = rcv[(ip * 3 + q) * F + jk];
+        }
+      }
+      rows(fv[0], fv[1]);
+    } else {
+      rows(RcvFace{rcv, F, jk, 0}, RcvFace{rcv, F, jk, 1});
     }
   }
   // interior stencil on every row, then the closure rows patched in by the
@@ -589,7 +627,13 @@ template <int D, int KIND>
 static void launch_col_c(int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a) {
   const size_t lds = col_lds_bytes(D, P);
   switch (C) {
-#define CASE_C(CV) case CV: hipLaunchKernelGGL((k_col<D, KIND, CV>), dim3(nwg, nf), dim3(64 * P), lds, st, a); break;
+#define CASE_C(CV)                                                                                   \
+  case CV:                                                                                         \
+    if (KIND == 0 && a.direct)                                                                     \
+      hipLaunchKernelGGL((k_col<D, KIND, CV, KIND == 0>), dim3(nwg, nf), dim3(64 * P), lds, st, a); \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_col<D, KIND, CV, false>), dim3(nwg, nf), dim3(64 * P), lds, st, a);    \
+    break;
     CASE_C(8) CASE_C(16) CASE_C(24) CASE_C(32)
 #undef CASE_C
   }

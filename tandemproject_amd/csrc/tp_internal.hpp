@@ -127,6 +127,50 @@ __device__ inline int xcd_item(int bid, int n) {
   return x * per + min(x, rem) + k;
 }
 
+// Closure sums that face (d, ip) of block b receives from its neighbour s
+// (mpigo, Subroutines3D.f90:16-80): over the neighbour's MP+1 points next to
+// the shared face (NT 0: the derivative's closure, MP = LMD; 1: the filter's,
+// MP = LMF), r0 and r1 the sums with the closure rows pbco, r2 the first of
+// those points, each less the periodic jump terms (n45 = 1 and a periodic
+// face: the neighbour's face point, and b's own face point added back).
+// src_s: the neighbour's field; own: b's field; jk: the face index.
+template <int NT>
+__device__ inline void closure_sums(const Block &b, const Block &s, const CaseCoef &c, const double *src_s,
+                                    const double *own, int d, int ip, int jk, int n45, double &r0, double &r1,
+                                    double &r2) {
+  constexpr int MP = NT == 0 ? LMD : LMF;
+  const int sp = 1 - ip;
+  const int snb = s.nbc[d][sp];
+  const int ii = snb == 40 ? 0 : 1;
+  const double ra0 = snb == 45 ? (double)n45 : 0.0;
+  const int fd = d == 0 ? s.ny : d == 1 ? s.nz : s.nx;   // ijk(2,nn)+1
+  const int j = jk % fd, k = jk / fd;
+  const int is = sp * (s.L[d] - 1), step = 1 - 2 * sp;
+  // every load first, then the sums in the reference's order
+  double v[MP + 1];
+  const double e0 = src_s[indx3(s, is, j, k, d)];
+#pragma unroll
+  for (int i = 0; i <= MP; i++) v[i] = src_s[indx3(s, is + step * (i + ii), j, k, d)];
+  double ow = 0.0;
+  const bool self45 = n45 == 1 && b.nbc[d][ip] == 45;
+  if (self45) ow = own[indx3(b, ip * (b.L[d] - 1), j, k, d)];
+  const double res = ra0 * e0;
+  double t0 = 0, t1 = 0;
+#pragma unroll
+  for (int i = 0; i <= MP; i++) {
+    t0 = t0 + c.pbco[NT][0][i] * v[i];
+    t1 = t1 + c.pbco[NT][1][i] * v[i];
+  }
+  r0 = t0 - res * c.pbcot[NT][0];
+  r1 = t1 - res * c.pbcot[NT][1];
+  r2 = v[0] - res;
+  if (self45) {
+    r0 = r0 + ow * c.pbcot[NT][0];
+    r1 = r1 + ow * c.pbcot[NT][1];
+    r2 = r2 + ow;
+  }
+}
+
 // Face launches: the face point jk of this thread (256 threads per work item).
 // A xi or zeta face takes 256 consecutive face indices: on a zeta face they
 // are consecutive in memory, on a xi face each is its own row.  An eta face
@@ -167,6 +211,9 @@ struct LineArgs {
                           // and drva column m + f
   int copy_arr, copy_col; // filter: also write the result to column copy_col +
                           // f out_step of copy_arr (copy_arr < 0: no copy)
+  int direct, n45;        // chunked solves: form the closure sums of a block's
+                          // faces onto itself here (closure_sums, periodic jump
+                          // n45) instead of reading the exchange's recv slots
 };
 
 struct ExchArgs {

@@ -289,47 +289,17 @@ __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
 // derivative's closure (LMD + 1 points), 1 the filter's (LMF + 1).
 template <int NT>
 __global__ void __launch_bounds__(256) k_exchange(ExchArgs a) {
-  constexpr int MP = NT == 0 ? LMD : LMF;
   const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const int d = w.d & 0xff, ip = w.d >> 8;
   const Block &b = a.blocks[w.blk];
   int jk;
   if (!face_point(b, w, d, jk)) return;
   const Block &s = a.blocks[b.ncd[d][ip]];
-  const CaseCoef &c = a.coef[b.cs];
   const int f = blockIdx.y;
   const int col = a.col0 + f * a.fstep + (1 - a.nrt) * d;
-  const int sp = 1 - ip;
-  const int snb = s.nbc[d][sp];
-  const int ii = snb == 40 ? 0 : 1;
-  const double ra0 = snb == 45 ? (double)a.n45 : 0.0;
-  const int fd = d == 0 ? s.ny : d == 1 ? s.nz : s.nx;   // ijk(2,nn)+1
-  const int j = jk % fd, k = jk / fd;
-  const double *src = field_ptr(s, a.arr, col);
-  const int is = sp * (s.L[d] - 1), step = 1 - 2 * sp;
-  // every load first, then the sums in the reference's order
-  double v[MP + 1];
-  const double e0 = src[indx3(s, is, j, k, d)];
-#pragma unroll
-  for (int i = 0; i <= MP; i++) v[i] = src[indx3(s, is + step * (i + ii), j, k, d)];
-  double own = 0.0;
-  const bool self45 = a.n45 == 1 && b.nbc[d][ip] == 45;
-  if (self45) own = field_ptr(b, a.arr, col)[indx3(b, ip * (b.L[d] - 1), j, k, d)];
-  const double res = ra0 * e0;
-  double t0 = 0, t1 = 0;
-#pragma unroll
-  for (int i = 0; i <= MP; i++) {
-    t0 = t0 + c.pbco[NT][0][i] * v[i];
-    t1 = t1 + c.pbco[NT][1][i] * v[i];
-  }
-  double r0 = t0 - res * c.pbcot[NT][0];
-  double r1 = t1 - res * c.pbcot[NT][1];
-  double r2 = v[0] - res;
-  if (self45) {
-    r0 = r0 + own * c.pbcot[NT][0];
-    r1 = r1 + own * c.pbcot[NT][1];
-    r2 = r2 + own;
-  }
+  double r0, r1, r2;
+  closure_sums<NT>(b, s, a.coef[b.cs], field_ptr(s, a.arr, col), field_ptr(b, a.arr, col), d, ip, jk, a.n45, r0,
+                   r1, r2);
   const int F = b.F[d];
   double *rcv = b.recv[d] + (size_t)f * 6 * F;
   rcv[(ip * 3 + 0) * F + jk] = r0;
