@@ -358,8 +358,13 @@ def main():
     s.sync()
     npts = lay.npoints()
 
-    s.profile(not args.graph)
-    s.profile_read("deriv", reset=True)
+    # HIP events around the launches of the dominant kernel class only (each
+    # event pair costs the stream a few microseconds; around every launch
+    # they would slow the step by ~3 %)
+    if not args.graph:
+        s.profile_classes(["deriv"])
+    for k in api.KCLASS:
+        s.profile_read(k, reset=True)
     s.sync()
     t0 = time.perf_counter()
     s.step(args.steps, graph=args.graph)
@@ -368,6 +373,15 @@ def main():
     elapsed = t1 - t0
     s.profile(False)
     prof = {k: s.profile_read(k) for k in api.KCLASS}
+    s.profile_read("deriv", reset=True)   # clears every class
+    # the per-class breakdown from a short pass with every class timed
+    breakdown_steps = max(1, min(5, args.steps))
+    s.profile(True)
+    s.step(breakdown_steps)
+    s.sync()
+    s.profile(False)
+    prof_all = {k: s.profile_read(k) for k in api.KCLASS}
+    s.profile_read("deriv", reset=True)
     sc = s.scalars(0)
     q = s.download(0, 0, "qa")
     line_chunks = {f"{'xez'[d]}-{k}": c for (d, k), c in s.col_config().items()}
@@ -406,7 +420,7 @@ def main():
                 "kernel": "compact derivative line solves (k_col / k_deriv, 3 directions)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl,
                 "bytes_per_launch": round(by / nl)}
-    breakdown = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1]}
+    breakdown = {k: round(v[0] / breakdown_steps, 3) for k, v in prof_all.items() if v[1]}
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
     s.close()

@@ -361,6 +361,10 @@ int32_t tp_server_stats(tp_server *srv, double *out16);
 /* ---- measurement ---- */
 /* record HIP events around every launch (on the context's stream) */
 int32_t tp_profile(tp_context *ctx, int32_t on);
+/* the same for the kernel classes in mask only (bit k: class TP_K_* k; 0:
+ * off): each event pair costs the stream a few microseconds, so timing one
+ * class disturbs the step less than timing all of them */
+int32_t tp_profile_classes(tp_context *ctx, int32_t mask);
 /* accumulated device time (ms) and launch count of a kernel class since the
  * last reset; algorithmic HBM bytes those launches move (bytes per point of
  * the class times points); reset != 0 clears the counters afterwards */

@@ -29,7 +29,7 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_set_grid", "tp_setup_geometry", "tp_initialo", "tp_upload", "tp_download",
            "tp_array_count", "tp_get_scalars", "tp_set_scalars", "tp_mpigo", "tp_deriv",
            "tp_filte", "tp_filter_step", "tp_stage", "tp_step", "tp_step_graph", "tp_sync",
-           "tp_profile", "tp_profile_read", "tp_info", "tp_set_option", "tp_autotune",
+           "tp_profile", "tp_profile_classes", "tp_profile_read", "tp_info", "tp_set_option", "tp_autotune",
            "tp_col_config", "tp_inflow_modes", "tp_gust_mode", "tp_case_bytes", "tp_admit", "tp_admit_cases", "tp_remove_cases",
            "tp_remove_case", "tp_case_info", "tp_server_create", "tp_server_destroy",
            "tp_server_context", "tp_server_error", "tp_server_clock", "tp_server_submit",
@@ -69,7 +69,7 @@ def load(path=LIB_PATH):
         "tp_set_scalars": [P, i32, pd], "tp_mpigo": [P, i32, i32, i32],
         "tp_deriv": [P, i32, i32, i32], "tp_filte": [P, i32, i32], "tp_filter_step": [P],
         "tp_stage": [P, i32], "tp_step": [P, i32], "tp_step_graph": [P, i32], "tp_sync": [P],
-        "tp_profile": [P, i32],
+        "tp_profile": [P, i32], "tp_profile_classes": [P, i32],
         "tp_profile_read": [P, i32, pd, ctypes.POINTER(i64), pd, i32],
         "tp_info": [P, ctypes.POINTER(i64)],
         "tp_set_option": [P, i32, i64],
@@ -395,6 +395,13 @@ class Solver:
     # ---- measurement ----
     def profile(self, on=True):
         _check(load().tp_profile(self.h, 1 if on else 0))
+
+    def profile_classes(self, classes):
+        """Time only these kernel classes (names from KCLASS)."""
+        mask = 0
+        for k in classes:
+            mask |= 1 << KCLASS.index(k)
+        _check(load().tp_profile_classes(self.h, mask))
 
     def profile_read(self, kclass, reset=False):
         ms = ctypes.c_double(); n = ctypes.c_int64(); by = ctypes.c_double()

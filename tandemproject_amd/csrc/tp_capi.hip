@@ -481,6 +481,7 @@ struct tp_context {
   bool any_visc = false;
   // measurement
   bool prof = false;
+  int prof_mask = 0;   // kernel classes timed while prof is on (bit k: class k)
   std::vector<ProfRec> pend[K_NCLASS];
   std::vector<hipEvent_t> pool;
   double ms[K_NCLASS] = {0}, bytes[K_NCLASS] = {0};
@@ -538,13 +539,13 @@ struct Timed {
   double bytes;
   hipEvent_t a = nullptr;
   Timed(tp_context *cc, int k, double by) : c(cc), cls(k), bytes(by) {
-    if (c->prof) {
+    if (c->prof && (c->prof_mask >> cls & 1)) {
       a = take_event(c);
       if (a) hipEventRecord(a, c->stream);
     }
   }
   ~Timed() {
-    if (c->prof && a) {
+    if (a) {
       hipEvent_t b = take_event(c);
       if (b) {
         hipEventRecord(b, c->stream);
@@ -1956,7 +1957,13 @@ int32_t tp_col_config(tp_context *c, int32_t *chunks6) {
 
 int32_t tp_profile(tp_context *c, int32_t on) {
   if (!c) return fail(TP_E_ARG, "null context");
-  c->prof = on != 0;
+  return tp_profile_classes(c, on ? (1 << K_NCLASS) - 1 : 0);
+}
+
+int32_t tp_profile_classes(tp_context *c, int32_t mask) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  c->prof_mask = mask & ((1 << K_NCLASS) - 1);
+  c->prof = c->prof_mask != 0;
   return 0;
 }
 

@@ -184,6 +184,11 @@ module tandem_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: on
     end function
+    integer(c_int32_t) function tp_profile_classes(ctx, mask) bind(C, name="tp_profile_classes")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mask
+    end function
     integer(c_int32_t) function tp_profile_read(ctx, kclass, ms, launches, bytes, reset) &
         bind(C, name="tp_profile_read")
       import :: c_ptr, c_int32_t, c_int64_t, c_double
