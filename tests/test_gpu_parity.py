@@ -477,3 +477,31 @@ def test_surface_statistics_long_run():
         np.testing.assert_allclose(cm, (ro[0] - 1 / 1.4) * 2 / m2, rtol=0, atol=0.01 * np.abs(cm).max())
         np.testing.assert_allclose(cr, ro[1] * 2 / m2, rtol=0, atol=0.01 * cr.max())
     say("  worst relative difference of the statistics", worst)
+
+
+def test_profile_classes():
+    """tp_profile_classes times only the classes asked for; tp_profile times
+    them all; the launches and algorithmic bytes of a step are counted."""
+    say("profiling by kernel class")
+    lay, ss, p = aerofoil_case(lxibk=(16, 20, 16), letbk=(14, 14))
+    s = api.Solver([(lay, p)])
+    for b in range(lay.nb):
+        s.set_grid(0, b, ss[b])
+    s.setup_geometry(); s.initialo(0)
+    s.step(1)
+    s.profile_classes(["deriv"])
+    s.step(2)
+    s.sync()
+    s.profile(False)
+    got = {k: s.profile_read(k) for k in api.KCLASS}
+    s.profile_read("deriv", reset=True)
+    assert got["deriv"][1] == 2 * 4 * 6 and got["deriv"][0] > 0 and got["deriv"][2] > 0
+    assert all(got[k][1] == 0 for k in api.KCLASS if k != "deriv")
+    s.profile(True)
+    s.step(1)
+    s.sync()
+    s.profile(False)
+    got = {k: s.profile_read(k) for k in api.KCLASS}
+    for k in ("deriv", "filter", "point", "flux", "gcbc", "rk"):
+        assert got[k][1] > 0, k
+    assert got["deriv"][1] == 4 * 6
