@@ -433,14 +433,13 @@ struct DevList {
 // lines use P chunks (P = 0: the sequential kernel) of capacity C points
 struct ColGroup {
   int P = 0, C = 0;
-  bool self = false;   // a block of the group has a closure face onto itself
+  bool closure = false;   // a block of the group has a closure face in this direction
   long long npts = 0;
   DevList work;
 };
 
 struct ListSet {
   DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
-  DevList exch_x;   // the closure faces whose neighbour is another block
   DevList surf;                     // wall-surface points, 256 per item
   std::vector<ColGroup> cg[3][2];   // line-solve launches per direction and kind
   std::vector<int> wr;   // first point-work item of each linked case, then the end
@@ -583,11 +582,8 @@ StageArgs base_args(tp_context *c) {
 
 // closure sums of nf fields (columns col0 + f fstep, f < nf) in one launch,
 // into recv slots 0..nf-1
-// others_only: only the faces whose neighbour is another block (the line
-// solves form the sums of a block's faces onto itself)
-void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0, int nf = 1, int fstep = 0,
-                  bool others_only = false) {
-  const DevList &L = others_only ? c->L.exch_x : c->L.exch;
+void run_exchange(tp_context *c, int nt, int nrt, int n45, int arr, int col0, int nf = 1, int fstep = 0) {
+  const DevList &L = c->L.exch;
   if (!L.n) return;
   Timed t(c, K_EXCH, 0.0);
   ExchArgs a;
@@ -621,7 +617,7 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   for (const ColGroup &g : c->L.cg[d][kind]) {
     if (!g.work.n) continue;
     a.work = g.work.d;
-    a.direct = direct && g.self ? 1 : 0;
+    a.direct = direct && g.closure ? 1 : 0;
     if (g.P) launch_col(d, kind, g.C, g.P, g.work.n, nf, c->stream, a);
     else if (kind == 0) launch_deriv(d, g.work.n, nf, c->lds[d], c->stream, a);
     else launch_filter(d, g.work.n, nf, c->lds[d], c->stream, a);
@@ -685,11 +681,44 @@ int make_groups(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
           works.emplace_back();
         }
         for (int ip = 0; ip < 2; ip++)
-          if (b.ndf[d][ip] && b.ncd[d][ip] == g) groups[k].self = true;
+          if (b.ndf[d][ip]) groups[k].closure = true;
         const int nlines = b.N / b.L[d];
         const int step = P ? 64 : c->nlp[d] - 1;
         for (int q = 0; q < nlines; q += step) works[k].push_back({g, q, std::min(step, nlines - q), d});
         groups[k].npts += b.N;
+      }
+      // a block and its neighbour across the end face of a closure (the two
+      // halves of a split line) interleave their work items, so that the line
+      // solves that form their closure sums from each other's lines run side
+      // by side (with xcd_item, on one XCD) and find those lines in its L2
+      for (auto &wl : works) {
+        std::vector<Work> out;
+        out.reserve(wl.size());
+        std::vector<char> done(wl.size(), 0);
+        for (size_t i = 0; i < wl.size(); i++) {
+          if (done[i]) continue;
+          const int ga = wl[i].blk;
+          size_t ea = i;
+          while (ea < wl.size() && wl[ea].blk == ga) ea++;   // the items of ga: [i, ea)
+          const Block &ba = c->blocks[ga];
+          const int gb = ba.ndf[d][1] ? ba.ncd[d][1] : -1;
+          size_t sb = wl.size();
+          if (gb >= 0 && gb != ga)
+            for (size_t q = ea; q < wl.size(); q++)
+              if (wl[q].blk == gb && !done[q]) { sb = q; break; }
+          size_t eb = sb;
+          while (eb < wl.size() && wl[eb].blk == gb) eb++;
+          if (sb < wl.size() && eb - sb == ea - i) {
+            for (size_t q = 0; q < ea - i; q++) {
+              out.push_back(wl[i + q]); done[i + q] = 1;
+              out.push_back(wl[sb + q]); done[sb + q] = 1;
+            }
+          } else {
+            for (size_t q = i; q < ea; q++) { out.push_back(wl[q]); done[q] = 1; }
+          }
+          i = ea - 1;
+        }
+        wl.swap(out);
       }
       for (size_t k = 0; k < groups.size(); k++)
         if (int rc = upload_list(m, groups[k].work, works[k])) return rc;
@@ -713,13 +742,14 @@ int regroup(tp_context *c) {
   return 0;
 }
 
-// the chunked line solves of this kind can form the closure sums of a block's
-// faces onto itself (periodic lines within one block) themselves, reading the
-// other end of the same lines, which the workgroup's other waves load at the
-// same time (every launch group chunked; the sequential solves read the
-// exchange's recv slots).  The exchange then covers only the faces between
-// blocks.  Only for solves whose input no earlier solve of the same exchange
-// changed (the derivatives; not the in-place filters).
+// the chunked line solves of this kind can form their closure sums
+// themselves (every launch group chunked; the sequential solves read the
+// exchange's recv slots), so that no exchange launch is needed: a periodic
+// block reads the other end of its own lines, which the workgroup's other
+// waves load at the same time; a split line reads its partner block's lines,
+// whose work items run beside its own.  Only for solves whose input no earlier
+// solve of the same exchange changed (the derivatives; not the in-place
+// filters).
 bool direct_ok(const tp_context *c, int kind) {
   if (c->exact_lines) return false;
   for (int d = 0; d < 3; d++)
@@ -756,7 +786,7 @@ void enqueue_stage(tp_context *c, int nk) {
     // three index directions into fx(:,3f+d) -- the four fields in one launch
     // per direction
     const bool dir = direct_ok(c, 0);
-    run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1, dir);
+    if (!dir) run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1);
     run_line(c, 0, 2, F_DE, 1, F_FX, 2, 0, 0, 4, 1, 3, -1, 0, dir);
     run_line(c, 0, 1, F_DE, 1, F_FX, 1, 0, 0, 4, 1, 3, -1, 0, dir);
     run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3, -1, 0, dir);
@@ -777,7 +807,7 @@ void enqueue_stage(tp_context *c, int nk) {
   }
   // the five fields m in one launch per step: flux components fx(:,3(m-1)+d)
   const bool dirf = direct_ok(c, 0);
-  run_exchange(c, 0, 0, 0, F_FX, 0, 5, 3, dirf);
+  if (!dirf) run_exchange(c, 0, 0, 0, F_FX, 0, 5, 3);
   run_line(c, 0, 0, F_FX, 0, F_DE, 0, 1, 0, 5, 3, 1, -1, 0, dirf);
   run_line(c, 0, 1, F_FX, 1, F_DE, 0, 1, 1, 5, 3, 1, -1, 0, dirf);
   run_line(c, 0, 2, F_FX, 2, F_DE, 0, 1, 1, 5, 3, 1, -1, 0, dirf);
@@ -1122,7 +1152,7 @@ int build_case(tp_context *c, HostCase &h) {
 
 // launch tables over the global blocks gs (sorted by case)
 int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
-  std::vector<Work> pts, sp, lw[3], ex, exx, gp, ga[3], gc, wl, ac, aa[3], nm, sf;
+  std::vector<Work> pts, sp, lw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm, sf;
   L.wr.assign(c->linked.size() + 1, 0);
   L.wwr.assign(c->linked.size() + 1, 0);
   int slot = -1;
@@ -1159,7 +1189,6 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
         const bool gc_face = np == 10 || np == 20 || np == 25 || np == 30;
         const bool wall = np == 20 || np == 25;
         if (b.ndf[d][s]) add(ex);
-        if (b.ndf[d][s] && b.ncd[d][s] != g) add(exx);
         if (gc_face) { add(gp); add(ga[d]); }
         if (np == 30 || (gcic && wall)) add(gc);
         if (wall) add(wl);
@@ -1177,7 +1206,6 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
   if ((rc = upload_list(m, L.points, pts))) return rc;
   if ((rc = upload_list(m, L.sponge, sp))) return rc;
   if ((rc = upload_list(m, L.exch, ex))) return rc;
-  if ((rc = upload_list(m, L.exch_x, exx))) return rc;
   if ((rc = upload_list(m, L.gprep, gp))) return rc;
   if ((rc = upload_list(m, L.gcopy, gc))) return rc;
   if ((rc = upload_list(m, L.wall, wl))) return rc;

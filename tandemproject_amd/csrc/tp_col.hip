@@ -277,7 +277,8 @@ __device__ inline void bwd_sweep(double (&r)[C], int n, const ChunkLU &f, double
   }
 }
 
-// DIRECT: the launch forms the closure sums of faces onto the same block (a.direct)
+// DIRECT: the launch forms the closure sums itself (a.direct) instead of
+// reading the exchange's recv slots
 template <int D, int KIND, int C, bool DIRECT>
 __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   extern __shared__ double lds[];
@@ -342,6 +343,24 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     }
   }
 
+  // DIRECT: the closure sums of the faces this wave's boundary rows need,
+  // formed from the neighbour's field (its lines on the far side of the
+  // face: the other end of the same lines for a periodic block, the partner
+  // block's lines -- whose work items run beside these -- otherwise), as the
+  // exchange forms them.  First, so that only the three sums per face stay
+  // live while the window is loaded.
+  ValFace fsum[2] = {{{0.0, 0.0, 0.0}}, {{0.0, 0.0, 0.0}}};
+  if (DIRECT) {
+#pragma unroll
+    for (int ip = 0; ip < 2; ip++) {
+      const bool need = ip == 0 ? n > 0 && s0 <= 2 : n > 0 && s0 + n - 1 >= L - 3;
+      if (!need || !b.ndf[D][ip]) continue;
+      const Block &nb = a.blocks[b.ncd[D][ip]];
+      closure_sums<KIND>(b, nb, c, field_ptr(nb, a.in_arr, a.in_col + fi * a.in_step), src, D, ip, jk, a.n45,
+                         fsum[ip].r[0], fsum[ip].r[1], fsum[ip].r[2]);
+    }
+  }
+
   // ---- window rows s0-3 .. s0+n+2, clamped to the line; the rows of the
   // capacity padding past them are not read (zero) ----
   double v[C + 6];
@@ -388,10 +407,8 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
   // ---- right-hand side ----
   // rows 0..2 and L-3..L-1 use the closure formulas, evaluated once, outside
   // the unrolled loop, by the (at most two) chunks that hold them
-  // the closure rows at the line ends; the closure sums of a face: in a
-  // DIRECT launch those of a block's faces onto itself formed here from the
-  // other end of the same lines, as the exchange forms them; otherwise read
-  // from the exchange's recv slots
+  // the closure rows at the line ends, with the closure sums formed above
+  // (DIRECT) or read from the exchange's recv slots
   double rb0 = 0.0, rb1 = 0.0, rb2 = 0.0, re2 = 0.0, re1 = 0.0, re0 = 0.0;
   {
     const GLine g{src + base, stride};
@@ -419,24 +436,8 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
         }
       }
     };
-    const int F = b.F[D];
-    if (DIRECT) {
-      ValFace fv[2] = {{{0.0, 0.0, 0.0}}, {{0.0, 0.0, 0.0}}};
-#pragma unroll
-      for (int ip = 0; ip < 2; ip++) {
-        const bool need = ip == 0 ? n > 0 && s0 <= 2 : n > 0 && s0 + n - 1 >= L - 3;
-        if (!need || !b.ndf[D][ip]) continue;
-        if (b.ncd[D][ip] == w.blk) {   // the block's own lines (a periodic face)
-          closure_sums<KIND>(b, b, c, src, src, D, ip, jk, a.n45, fv[ip].r[0], fv[ip].r[1], fv[ip].r[2]);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 3; q++) fv[ip].r[q] = rcv[(ip * 3 + q) * F + jk];
-        }
-      }
-      rows(fv[0], fv[1]);
-    } else {
-      rows(RcvFace{rcv, F, jk, 0}, RcvFace{rcv, F, jk, 1});
-    }
+    if (DIRECT) rows(fsum[0], fsum[1]);
+    else rows(RcvFace{rcv, b.F[D], jk, 0}, RcvFace{rcv, b.F[D], jk, 1});
   }
   // interior stencil on every row, then the closure rows patched in by the
   // (at most two) chunks at each end that hold them, then the padding rows

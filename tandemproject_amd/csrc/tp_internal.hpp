@@ -148,7 +148,8 @@ __device__ inline void closure_sums(const Block &b, const Block &s, const CaseCo
   const int is = sp * (s.L[d] - 1), step = 1 - 2 * sp;
   // every load first, then the sums in the reference's order
   double v[MP + 1];
-  const double e0 = src_s[indx3(s, is, j, k, d)];
+  // the face point itself enters only through the periodic jump (ra0 != 0)
+  const double e0 = ra0 != 0.0 ? src_s[indx3(s, is, j, k, d)] : 0.0;
 #pragma unroll
   for (int i = 0; i <= MP; i++) v[i] = src_s[indx3(s, is + step * (i + ii), j, k, d)];
   double ow = 0.0;
