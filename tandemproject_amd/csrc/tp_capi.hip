@@ -391,7 +391,7 @@ struct Mem {
   std::vector<void *> p;
   size_t bytes = 0;
   void release() {
-    for (void *q : p) hipFree(q);
+    for (void *q : p) (void)hipFree(q);
     p.clear();
     bytes = 0;
   }
@@ -540,14 +540,14 @@ struct Timed {
   Timed(tp_context *cc, int k, double by) : c(cc), cls(k), bytes(by) {
     if (c->prof && (c->prof_mask >> cls & 1)) {
       a = take_event(c);
-      if (a) hipEventRecord(a, c->stream);
+      if (a) (void)hipEventRecord(a, c->stream);
     }
   }
   ~Timed() {
     if (a) {
       hipEvent_t b = take_event(c);
       if (b) {
-        hipEventRecord(b, c->stream);
+        (void)hipEventRecord(b, c->stream);
         c->pend[cls].push_back({a, b, bytes});
       }
     }
@@ -626,8 +626,8 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
 
 // drop a captured step: it holds the launch configuration it was built with
 void drop_graph(tp_context *c) {
-  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
 }
 
 // chunks per line of block g's line solves in direction d, kind: the tuned
@@ -1093,7 +1093,7 @@ size_t case_bytes(const HostCase &h) {
 // device memory of a case: block arenas (zeroed), LU factors of the
 // derivative and filter matrices (src/Main3D.f90:316-332), chunked-solver
 // tables for every chunk count the block's lines allow, and the grid if set
-int build_case(tp_context *c, HostCase &h) {
+int build_case(HostCase &h) {
   h.blk.assign(h.nb, Block{});
   h.coltab.assign(h.nb, ColTab{});
   h.grid.resize(h.nb);
@@ -1415,16 +1415,16 @@ int32_t tp_create(tp_context **out) {
 
 int32_t tp_destroy(tp_context *c) {
   if (!c) return 0;
-  hipStreamSynchronize(c->stream);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   for (int k = 0; k < K_NCLASS; k++)
-    for (auto &r : c->pend[k]) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
-  for (auto e : c->pool) hipEventDestroy(e);
+    for (auto &r : c->pend[k]) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
   for (auto &h : c->cases) h.mem.release();
   c->link.release();
   c->lmem.release();
-  hipStreamDestroy(c->stream);
+  (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
@@ -1567,7 +1567,7 @@ int32_t tp_finalize(tp_context *c) {
   HIPCHK(hipSetDevice(c->device));
   int rc;
   for (auto &h : c->cases)
-    if ((rc = build_case(c, h))) return rc;
+    if ((rc = build_case(h))) return rc;
   if ((rc = link(c))) return rc;
   c->finalized = true;
   return 0;
@@ -1921,7 +1921,7 @@ int32_t tp_autotune(tp_context *c, int32_t reps, int32_t *chosen, double *us) {
     if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipGetLastError() != hipSuccess)
       return fail(TP_E_HIP, "autotune launch failed");
-    hipEventElapsedTime(ms, e0, e1);
+    if (hipEventElapsedTime(ms, e0, e1) != hipSuccess) return fail(TP_E_HIP, "autotune timing failed");
     return 0;
   };
   for (int d = 0; d < 3 && !rc; d++) {
@@ -1963,8 +1963,8 @@ int32_t tp_autotune(tp_context *c, int32_t reps, int32_t *chosen, double *us) {
     }
   }
   if (!rc) rc = regroup(c);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   c->prof = prof;
   c->exact_lines = exact;
   return rc;
@@ -2057,7 +2057,7 @@ int32_t tp_admit_cases(tp_context *c, int32_t n, const int32_t *ids, int32_t *st
         if ((int)c->cases[ci].grid.size() <= lb || c->cases[ci].grid[lb].empty())
           rc = fail(TP_E_STATE, "admitting case " + std::to_string(ids[i]) +
                                     " needs the grid of every block (tp_set_grid)");
-      if (!rc) rc = build_case(c, c->cases[ci]);
+      if (!rc) rc = build_case(c->cases[ci]);
       if (rc) {
         HostCase &h = c->cases[ci];
         h.mem.release();
@@ -2076,7 +2076,7 @@ int32_t tp_admit_cases(tp_context *c, int32_t n, const int32_t *ids, int32_t *st
   if (rc) {
     // leave the context as it was: these cases stay pending
     const std::string msg = g_err;
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     for (int id : built) {
       HostCase &h = c->cases[case_index(c, id)];
       h.mem.release();

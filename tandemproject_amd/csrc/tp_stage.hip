@@ -157,7 +157,7 @@ __global__ void k_sc_dt(const CaseCoef *coef, double *sc, const double *part, co
   mi[threadIdx.x] = fi; mv[threadIdx.x] = fv;
   __syncthreads();
   for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
+    if ((int)threadIdx.x < h) {
       mi[threadIdx.x] = fmax(mi[threadIdx.x], mi[threadIdx.x + h]);
       mv[threadIdx.x] = fmax(mv[threadIdx.x], mv[threadIdx.x + h]);
     }
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
   mi[threadIdx.x] = fi; mv[threadIdx.x] = fv;
   __syncthreads();
   for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
+    if ((int)threadIdx.x < h) {
       mi[threadIdx.x] = fmax(mi[threadIdx.x], mi[threadIdx.x + h]);
       mv[threadIdx.x] = fmax(mv[threadIdx.x], mv[threadIdx.x + h]);
     }
@@ -737,7 +737,7 @@ __global__ void __launch_bounds__(256) k_avg_capture(StageArgs a) {
 // running-average weight of face (d, ip) at (x,y,z): the reference's rr(l,1)
 // counter after this face's increment (faces processed in the order
 // xi start, xi end, eta start, ..., each adding one (or rpex for GCIC walls))
-__device__ inline double avg_count(const Block &b, const CaseCoef &c, int x, int y, int z, int d, int ip,
+__device__ inline double avg_count(const Block &b, const CaseCoef &c, int x, int y, int z, int d,
                                    double self_inc) {
   int idx[3] = {x, y, z};
   double cnt = 1.0;
@@ -776,14 +776,14 @@ __global__ void __launch_bounds__(256) k_avg_apply(StageArgs a) {
   double *qa = b.qa;
   const double *drvb = b.drvb[d];
   if (avg_code(np)) {
-    double r1 = avg_count(b, c, x, y, z, d, ip, 1.0);
+    double r1 = avg_count(b, c, x, y, z, d, 1.0);
     double r2 = 1.0 / r1;
     for (int v = 0; v < 5; v++)
       qa[(size_t)v * N + l] = r2 * ((r1 - 1.0) * qa[(size_t)v * N + l] + drvb[(ip * 5 + v) * F + jk]);
   } else {
     double rp = b.rpex[b.wall_off[d][ip] + jk];
     double ra1 = 1.0 - rp;
-    double r1 = avg_count(b, c, x, y, z, d, ip, rp);
+    double r1 = avg_count(b, c, x, y, z, d, rp);
     double r2 = 1.0 / r1;
     for (int v = 0; v < 5; v++)
       qa[(size_t)v * N + l] = rp * r2 * ((r1 - 1.0) * qa[(size_t)v * N + l] + drvb[(ip * 5 + v) * F + jk]) +
