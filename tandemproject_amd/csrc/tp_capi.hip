@@ -1166,7 +1166,7 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
     for (int a0 = 0; a0 < b.nsurf; a0 += 256) sf.push_back({g, a0, std::min(256, b.nsurf - a0), 1});
     const int CH = 256;
     for (int a0 = 0; a0 < b.N; a0 += CH) pts.push_back({g, a0, std::min(CH, b.N - a0), 0});
-    for (int a0 = 0; a0 < b.nsponge; a0 += 1024) sp.push_back({g, a0, std::min(1024, b.nsponge - a0), 0});
+    for (int a0 = 0; a0 < b.nsponge; a0 += 256) sp.push_back({g, a0, std::min(256, b.nsponge - a0), 0});
     for (int d = 0; d < 3; d++) {
       const int nlines = b.N / b.L[d], nl = c->nlp[d] - 1;
       for (int q = 0; q < nlines; q += nl) lw[d].push_back({g, q, std::min(nl, nlines - q), d});
