@@ -153,7 +153,7 @@ int32_t tp_sync(tp_context *ctx);
  *   waves, the chunks joined by composing their 2x2 carry maps -- equal to
  *   the sequential sweep up to rounding of the chunk carries (~1e-16
  *   relative).  May be changed between steps.
- * TP_OPT_CHUNKS = P (1, 2, 4, 8, 16) fixes the chunks per line of the
+ * TP_OPT_CHUNKS = P (1 to 16) fixes the chunks per line of the
  *   chunked sweeps for every block and direction whose lines L allow chunks
  *   of ceil(L/P) <= 32 points; 0 (default) lets the library choose per block
  *   and direction.  The chunk count of a block depends only on its own line

@@ -171,7 +171,7 @@ class Solver:
 
     def __init__(self, cases, chunks=0, exact_lines=False):
         """cases: list of (layout, params) pairs.  chunks: fixed number of
-        chunks per line of the chunked line sweeps (1, 2, 4, 8, 16; 0: chosen
+        chunks per line of the chunked line sweeps (1 to 16; 0: chosen
         per direction); exact_lines: sequential sweeps, one lane per line."""
         L = load()
         self.h = ctypes.c_void_p()

@@ -330,9 +330,11 @@ std::vector<double> col_tables(int L, int P, const double *l1, const double *l2,
   return T;
 }
 
-// chunk counts the chunked solver can run with (P waves of 64 lanes)
-constexpr int kColP[] = {1, 2, 4, 8, 16};
-constexpr int kNColP = 5;
+// chunk counts the chunked solver can run with (P waves of 64 lanes); the
+// defaults (no autotuning) come from the powers of two among them
+constexpr int kColP[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+constexpr int kNColP = 16;
+constexpr int kColDefault[] = {1, 2, 4, 8, 16};
 
 // chunk length of P chunks over L points, 0 if the chunked solver cannot run it
 int col_chunk(int L, int P) {
@@ -340,12 +342,29 @@ int col_chunk(int L, int P) {
   return col_capacity(Cl) ? Cl : 0;
 }
 
-// default chunk count of lines of L points: the fewest chunks whose length
-// stays within 24 points (registers for 4+ waves per SIMD), else within 32;
-// 0 (the sequential kernel) if none fits
+// doubles of the chunked-solver table of P chunks over L points (col_tables),
+// padded to whole 128-byte lines
+size_t col_table_doubles(int L, int P) {
+  const int Cl = (L + P - 1) / P, Pe = (L + Cl - 1) / Cl;
+  return (128 + 192 * (size_t)Pe + 15) / 16 * 16;
+}
+
+// doubles of all chunked-solver tables of a block with lines L[3]: every
+// chunk count its lines allow, both kinds
+size_t col_block_doubles(const int *L) {
+  size_t n = 0;
+  for (int d = 0; d < 3; d++)
+    for (int P : kColP)
+      if (col_chunk(L[d], P)) n += 2 * col_table_doubles(L[d], P);
+  return n;
+}
+
+// default chunk count of lines of L points: the fewest chunks (a power of
+// two) whose length stays within 24 points (registers for 4+ waves per SIMD),
+// else within 32; 0 (the sequential kernel) if none fits
 int col_default(int L) {
   for (int lim : {24, 32})
-    for (int P : kColP)
+    for (int P : kColDefault)
       if ((L + P - 1) / P <= lim) return P;
   return 0;
 }
@@ -398,7 +417,7 @@ struct Mem {
 };
 
 // chunked-solver tables of one block: direction x kind x chunk-count index
-using ColTab = std::array<const double *, 3 * 2 * 5>;
+using ColTab = std::array<const double *, 3 * 2 * kNColP>;
 
 struct HostCase {
   int id = 0;                         // stable handle returned by tp_add_case
@@ -1069,11 +1088,7 @@ size_t case_bytes(const HostCase &h) {
     int nwall = 0;
     const Block b = block_shape(h, lb, &nwall);
     total += acct(arena_layout(b, nwall).total);
-    for (int d = 0; d < 3; d++)
-      for (int P : kColP) {
-        const int Cl = col_chunk(b.L[d], P);
-        if (Cl) total += 2 * acct(sizeof(double) * (128 + 192 * (size_t)((b.L[d] + Cl - 1) / Cl)));
-      }
+    total += acct(sizeof(double) * col_block_doubles(b.L));
     if (lb >= (int)h.grid.size() || h.grid[lb].empty()) continue;
     const double *x = h.grid[lb].data(), *y = x + b.N, *z = y + b.N;
     std::vector<int> lst, gix, pts;
@@ -1126,19 +1141,28 @@ int build_case(HostCase &h) {
     }
     HIPCHK(hipMemcpy(b.lud, lud.data(), sizeof(double) * lud.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.luf, luf.data(), sizeof(double) * luf.size(), hipMemcpyHostToDevice));
-    for (int d = 0; d < 3; d++) {
-      const int L = b.L[d], o = b.lu_off[d];
-      for (int pi = 0; pi < kNColP; pi++) {
-        if (!col_chunk(L, kColP[pi])) continue;
-        for (int kind = 0; kind < 2; kind++) {
-          const std::vector<double> &lu = kind == 0 ? lud : luf;
-          std::vector<double> T = col_tables(L, kColP[pi], &lu[0 * nl + o], &lu[1 * nl + o], &lu[2 * nl + o],
-                                             &lu[3 * nl + o], &lu[4 * nl + o]);
-          double *dptr = nullptr;
-          if ((rc = upload_vec(h.mem, &dptr, T))) return rc;
-          h.coltab[lb][(d * 2 + kind) * kNColP + pi] = dptr;
+    {
+      // every table of the block in one allocation, each at a 128-byte offset
+      std::vector<double> all;
+      all.reserve(col_block_doubles(b.L));
+      std::vector<std::pair<int, size_t>> slot;
+      for (int d = 0; d < 3; d++) {
+        const int L = b.L[d], o = b.lu_off[d];
+        for (int pi = 0; pi < kNColP; pi++) {
+          if (!col_chunk(L, kColP[pi])) continue;
+          for (int kind = 0; kind < 2; kind++) {
+            const std::vector<double> &lu = kind == 0 ? lud : luf;
+            std::vector<double> T = col_tables(L, kColP[pi], &lu[0 * nl + o], &lu[1 * nl + o], &lu[2 * nl + o],
+                                               &lu[3 * nl + o], &lu[4 * nl + o]);
+            slot.push_back({(d * 2 + kind) * kNColP + pi, all.size()});
+            T.resize(col_table_doubles(L, kColP[pi]), 0.0);
+            all.insert(all.end(), T.begin(), T.end());
+          }
         }
       }
+      double *dptr = nullptr;
+      if ((rc = upload_vec(h.mem, &dptr, all))) return rc;
+      for (const auto &e : slot) h.coltab[lb][e.first] = dptr + e.second;
     }
     if (!h.grid[lb].empty())
       HIPCHK(hipMemcpy(b.ss, h.grid[lb].data(), sizeof(double) * 3 * b.N, hipMemcpyHostToDevice));
@@ -1884,8 +1908,8 @@ int32_t tp_set_option(tp_context *c, int32_t option, int64_t value) {
   }
   if (option == TP_OPT_CHUNKS) {
     const int P = (int)value;
-    if (P != 0 && P != 1 && P != 2 && P != 4 && P != 8 && P != 16)
-      return fail(TP_E_ARG, "chunks per line must be 0, 1, 2, 4, 8 or 16, not " + std::to_string(value));
+    if (P < 0 || P > 16)
+      return fail(TP_E_ARG, "chunks per line must be 0 to 16, not " + std::to_string(value));
     c->col_fixed = P;
     for (auto &a : c->tuned)
       for (auto &b : a)

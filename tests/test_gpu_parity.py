@@ -39,8 +39,9 @@ def pair(layout, params, exact=False, chunks=0):
 # line-solver configurations of the per-call tests: the sequential sweep, and
 # chunked sweeps with fixed chunk counts so that every line is split over
 # several waves (the automatic choice may keep short lines in one chunk)
-CALL_MODES = {"sequential": dict(exact=True), "chunks2": dict(chunks=2),
-              "chunks4": dict(chunks=4), "chunks16": dict(chunks=16), "auto": dict()}
+CALL_MODES = {"sequential": dict(exact=True), "chunks2": dict(chunks=2), "chunks3": dict(chunks=3),
+              "chunks4": dict(chunks=4), "chunks7": dict(chunks=7), "chunks16": dict(chunks=16),
+              "auto": dict()}
 
 
 LINE_CASES = {
