@@ -348,7 +348,9 @@ int32_t tp_server_iterate(tp_server *srv, int32_t max_steps, int32_t *status6);
 int32_t tp_server_run(tp_server *srv, int32_t max_steps);
 /* info6 = state (0 queued, 1 on the device, 2 finished, 3 failed), arrival,
  * admission and finish times, steps made, device bytes; qa (may be NULL):
- * the final qa of every sub-block in order, count = tp_server_result_count */
+ * the final qa of every sub-block in order, count = tp_server_result_count.
+ * The server keeps each final state in page-locked host memory (the
+ * retirement copy then runs at full link speed) until tp_server_destroy. */
 int32_t tp_server_result(tp_server *srv, int32_t req, double *qa, int64_t count, double *info6);
 int64_t tp_server_result_count(tp_server *srv, int32_t req);
 /* out16 = finished, failed, p50, p99, mean and max time to solution (arrival

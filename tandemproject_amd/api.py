@@ -166,6 +166,31 @@ def case_args(layout, params):
     return arrs, ipar, dpar
 
 
+class PinnedBuffer:
+    """n float64 values of page-locked host memory (hipHostMalloc).  Device
+    downloads into it run at full link speed; a freshly allocated pageable
+    array takes a page fault per page during the copy, several times slower
+    for states of hundreds of MB.  `array` is valid until free()."""
+
+    def __init__(self, n):
+        self._hip = ctypes.CDLL("libamdhip64.so")
+        self._p = ctypes.c_void_p()
+        m = max(int(n), 1)
+        if self._hip.hipHostMalloc(ctypes.byref(self._p), ctypes.c_size_t(8 * m), ctypes.c_uint(0)) != 0:
+            self._p = None
+            raise RuntimeError("hipHostMalloc failed")
+        self.array = np.ctypeslib.as_array((ctypes.c_double * m).from_address(self._p.value))[:int(n)]
+
+    def free(self):
+        if self._p:
+            self.array = None
+            self._hip.hipHostFree(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.free()
+
+
 class Solver:
     """Device-resident state of one or more cases (each a list of sub-blocks)."""
 
@@ -314,9 +339,15 @@ class Solver:
         v = np.ascontiguousarray(values, np.float64).ravel()
         _check(load().tp_upload(self.h, case, block, ARRAYS.index(name), _dp(v), v.size))
 
-    def download(self, case, block, name):
+    def download(self, case, block, name, out=None):
+        """The array's values in the reference's layout.  out: a C-contiguous
+        float64 array of count() values to fill (e.g. a PinnedBuffer's);
+        default a new array."""
         n = self.count(case, block, name)
-        out = np.empty(max(n, 0), np.float64)
+        if out is None:
+            out = np.empty(max(n, 0), np.float64)
+        elif out.dtype != np.float64 or not out.flags.c_contiguous or out.size != n:
+            raise ValueError(f"download: out must hold {n} contiguous float64 values")
         _check(load().tp_download(self.h, case, block, ARRAYS.index(name), _dp(out), n))
         return out
 

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -18,6 +19,54 @@
 #include "../../include/tandem_hip.h"
 
 namespace {
+
+// host buffer of page-locked memory (hipHostMalloc), which the device writes
+// at full link speed: a fresh pageable buffer takes a page fault per page
+// during the copy, about 4x slower for a state of hundreds of MB.  Plain heap
+// memory if the page-locked allocation fails.
+struct HostBuf {
+  double *p = nullptr;
+  size_t n = 0;
+  bool pinned = false;
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  HostBuf(HostBuf &&o) noexcept { swap(o); }
+  HostBuf &operator=(HostBuf &&o) noexcept {
+    swap(o);
+    return *this;
+  }
+  ~HostBuf() { reset(); }
+  void swap(HostBuf &o) noexcept {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(pinned, o.pinned);
+  }
+  void reset() {
+    if (p) {
+      if (pinned) (void)hipHostFree(p);
+      else delete[] p;
+    }
+    p = nullptr;
+    n = 0;
+    pinned = false;
+  }
+  bool alloc(size_t count) {
+    reset();
+    if (count == 0) return true;
+    void *q = nullptr;
+    if (hipHostMalloc(&q, sizeof(double) * count, hipHostMallocDefault) == hipSuccess) {
+      p = static_cast<double *>(q);
+      pinned = true;
+    } else {
+      (void)hipGetLastError();
+      p = new (std::nothrow) double[count];
+      if (!p) return false;
+    }
+    n = count;
+    return true;
+  }
+};
 
 struct Request {
   int cid = -1;                  // case handle in the context
@@ -27,7 +76,7 @@ struct Request {
   int state = 0;                 // 0 queued, 1 on the device, 2 done, 3 failed
   double t_arr = 0, t_adm = -1, t_done = -1;
   int64_t bytes = 0;
-  std::vector<double> qa;        // final qa of every sub-block, concatenated
+  HostBuf qa;                    // final qa of every sub-block, concatenated
   std::string error;
 };
 
@@ -81,16 +130,19 @@ int retire(tp_server *s, const std::vector<int> &fin) {
   std::vector<int32_t> ids;
   for (int r : fin) {
     Request &q = s->reqs[r];
-    q.qa.resize((size_t)qa_count(s->ctx, q.cid, q.nb));
+    if (!q.qa.alloc((size_t)qa_count(s->ctx, q.cid, q.nb)))
+      return srv_fail(s, TP_E_SIZE, "retire: no host memory for a final state");
     int64_t off = 0;
     for (int b = 0; b < q.nb; b++) {
       const int64_t n = tp_array_count(s->ctx, q.cid, b, TP_QA);
-      if (int rc = tp_download(s->ctx, q.cid, b, TP_QA, q.qa.data() + off, n)) return rc;
+      if (int rc = tp_download(s->ctx, q.cid, b, TP_QA, q.qa.p + off, n))
+        return srv_fail(s, rc, std::string("retire: ") + tp_last_error());
       off += n;
     }
     ids.push_back(q.cid);
   }
-  if (int rc = tp_remove_cases(s->ctx, (int32_t)ids.size(), ids.data())) return rc;
+  if (int rc = tp_remove_cases(s->ctx, (int32_t)ids.size(), ids.data()))
+    return srv_fail(s, rc, std::string("retire: ") + tp_last_error());
   s->changed = true;
   for (int r : fin) {
     Request &q = s->reqs[r];
@@ -193,7 +245,7 @@ int iterate(tp_server *s, int max_steps, int32_t *status) {
       if (s->reqs[r].steps >= s->reqs[r].nsteps) fin.push_back(r);
     const double tr = s->now();
     if (!fin.empty())
-      if (int rc = retire(s, fin)) return srv_fail(s, rc, std::string("retire: ") + tp_last_error());
+      if (int rc = retire(s, fin)) return rc;
     s->retire_s += s->now() - tr;
   }
   if (status) {
@@ -339,16 +391,16 @@ int32_t tp_server_result(tp_server *s, int32_t req, double *qa, int64_t count, d
   }
   if (qa) {
     if (q.state != 2) return srv_fail(s, TP_E_STATE, "request not finished");
-    if (count != (int64_t)q.qa.size())
-      return srv_fail(s, TP_E_SIZE, "expected " + std::to_string(q.qa.size()) + " values");
-    std::memcpy(qa, q.qa.data(), sizeof(double) * q.qa.size());
+    if (count != (int64_t)q.qa.n)
+      return srv_fail(s, TP_E_SIZE, "expected " + std::to_string(q.qa.n) + " values");
+    if (q.qa.n) std::memcpy(qa, q.qa.p, sizeof(double) * q.qa.n);
   }
   return 0;
 }
 
 int64_t tp_server_result_count(tp_server *s, int32_t req) {
   if (!s || req < 0 || req >= (int)s->reqs.size()) return -1;
-  return (int64_t)s->reqs[req].qa.size();
+  return (int64_t)s->reqs[req].qa.n;
 }
 
 int32_t tp_server_stats(tp_server *s, double *out16) {

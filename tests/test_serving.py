@@ -236,3 +236,28 @@ def test_server_fill_target_runs_cases_in_turn():
         for q, rr in zip(qa, ref):
             np.testing.assert_array_equal(q, rr)
     srv.close()
+
+
+def test_download_into_pinned_buffer():
+    """download(out=) into page-locked memory (as the request loop keeps final
+    states) gives the values of a plain download"""
+    lay, ss, p = CASES[0]
+    say("pinned download:", lay.npoints(), "points")
+    s = api.Solver([(lay, p)], exact_lines=True)
+    for b in range(lay.nb):
+        s.set_grid(0, b, ss[b])
+    s.setup_geometry()
+    s.initialo(0)
+    s.step(3)
+    s.sync()
+    counts = [s.count(0, b, "qa") for b in range(lay.nb)]
+    buf = api.PinnedBuffer(sum(counts))
+    off = 0
+    for b in range(lay.nb):
+        got = s.download(0, b, "qa", out=buf.array[off:off + counts[b]])
+        assert np.array_equal(got, s.download(0, b, "qa"))
+        off += counts[b]
+    with pytest.raises(ValueError):
+        s.download(0, 0, "qa", out=np.empty(counts[0] + 1))
+    buf.free()
+    s.close()
