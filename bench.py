@@ -343,8 +343,8 @@ def main():
                          "of known traffic for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding by block is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(sharding the sub-blocks over GPUs is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
     lay, ss, params, dims = workload(args.scale)
