@@ -1736,13 +1736,24 @@ int32_t tp_setup_geometry(tp_context *c) {
           }
           if ((rc = dalloc(h.mem, (void **)&b.vito, sizeof(double) * 3 * (h.coef.ntz + 1) * (size_t)b.ngust)))
             return rc;
+          // per mode the cosine and sine of its phase step between table
+          // times, fctr k.u: the table kernel advances each mode by rotation
+          std::vector<double> rot(2 * (size_t)nm);
+          for (int n = 0; n < nm; n++) {
+            const double *m = &modes[8 * (size_t)n];
+            const double st =
+                h.coef.fctr * (m[0] * h.coef.uoo[0] + m[1] * h.coef.uoo[1] + m[2] * h.coef.uoo[2]);
+            rot[2 * (size_t)n] = std::cos(st);
+            rot[2 * (size_t)n + 1] = std::sin(st);
+          }
           Mem tmp;
-          double *dmodes = nullptr, *dpos = nullptr;
-          if ((rc = upload_vec(tmp, &dmodes, modes)) || (rc = upload_vec(tmp, &dpos, pos))) {
+          double *dmodes = nullptr, *dpos = nullptr, *drot = nullptr;
+          if ((rc = upload_vec(tmp, &dmodes, modes)) || (rc = upload_vec(tmp, &dpos, pos)) ||
+              (rc = upload_vec(tmp, &drot, rot))) {
             tmp.release();
             return rc;
           }
-          launch_inflow_table(dmodes, nm, dpos, b.ngust, h.coef.ntz, h.coef.fctr, h.coef.uoo, b.vito,
+          launch_inflow_table(dmodes, drot, nm, dpos, b.ngust, h.coef.ntz, h.coef.fctr, h.coef.uoo, b.vito,
                               c->stream);
           const hipError_t e = hipStreamSynchronize(c->stream);
           tmp.release();

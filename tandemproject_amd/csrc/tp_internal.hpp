@@ -240,8 +240,8 @@ struct StageArgs {
 
 
 // launchers (tp_line.hip, tp_col.hip, tp_stage.hip)
-void launch_inflow_table(const double *modes, int nmodes, const double *pos, int ng, int ntz, double fctr,
-                         const double *uoo, double *vito, hipStream_t st);
+void launch_inflow_table(const double *modes, const double *rot, int nmodes, const double *pos, int ng, int ntz,
+                         double fctr, const double *uoo, double *vito, hipStream_t st);
 // chunked line solves: P waves per 64 lines, chunk capacity C points per lane
 int col_capacity(int chunk);
 size_t col_lds_bytes(int d, int P);

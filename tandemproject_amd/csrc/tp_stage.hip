@@ -440,32 +440,53 @@ __global__ void __launch_bounds__(256) k_sponge(StageArgs a) {
 // ---- inflow velocity table: frozen turbulence convected with uoo, sampled
 // at the inflow points at the table times tt(i) = i*fctr (the reference fills
 // vito(ll, i, :) the same way from its eddies, SingleTurbulence3D.f90:262-292) ----
-__global__ void __launch_bounds__(256) k_inflow_table(const double *modes, int nmodes, const double *pos,
-                                                      int ng, int ntz, double fctr, double u0, double u1,
-                                                      double u2, double *vito) {
+// the table at IT_BLK consecutive times per thread: mode n's phase at time
+// index it is a_n - it b_n (a_n = k.x + phi, b_n = fctr k.u), so one sincos
+// per mode gives the first time and a rotation by -b_n (cos b_n, sin b_n in
+// rot) each next one
+constexpr int IT_BLK = 8;
+__global__ void __launch_bounds__(256) k_inflow_table(const double *modes, const double *rot, int nmodes,
+                                                      const double *pos, int ng, int ntz, double fctr, double u0,
+                                                      double u1, double u2, double *vito) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nt1 = ntz + 1;
-  if (e >= ng * nt1) return;
-  const int ll = e % ng, it = e / ng;
-  const double tt = it * fctr;
-  const double x0 = pos[ll] - tt * u0, x1 = pos[ng + ll] - tt * u1, x2 = pos[2 * (size_t)ng + ll] - tt * u2;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const int nt1 = ntz + 1, nblk = (nt1 + IT_BLK - 1) / IT_BLK;
+  if (e >= ng * nblk) return;
+  const int ll = e % ng, it0 = (e / ng) * IT_BLK;
+  const double px = pos[ll], py = pos[ng + ll], pz = pos[2 * (size_t)ng + ll];
+  double s0[IT_BLK], s1[IT_BLK], s2[IT_BLK];
+#pragma unroll
+  for (int k = 0; k < IT_BLK; k++) s0[k] = s1[k] = s2[k] = 0.0;
   for (int n = 0; n < nmodes; n++) {
     const double *m = modes + 8 * (size_t)n;
-    const double cs = cos(m[0] * x0 + m[1] * x1 + m[2] * x2 + m[6]);
-    s0 = s0 + m[3] * cs;
-    s1 = s1 + m[4] * cs;
-    s2 = s2 + m[5] * cs;
+    const double a = m[0] * px + m[1] * py + m[2] * pz + m[6];
+    const double b = fctr * (m[0] * u0 + m[1] * u1 + m[2] * u2);
+    const double cb = rot[2 * n], sb = rot[2 * n + 1];
+    double sn, cs;
+    sincos(a - it0 * b, &sn, &cs);
+#pragma unroll
+    for (int k = 0; k < IT_BLK; k++) {
+      s0[k] = s0[k] + m[3] * cs;
+      s1[k] = s1[k] + m[4] * cs;
+      s2[k] = s2[k] + m[5] * cs;
+      const double c2 = cs * cb + sn * sb;
+      sn = sn * cb - cs * sb;
+      cs = c2;
+    }
   }
-  vito[((size_t)0 * nt1 + it) * ng + ll] = s0;
-  vito[((size_t)1 * nt1 + it) * ng + ll] = s1;
-  vito[((size_t)2 * nt1 + it) * ng + ll] = s2;
+#pragma unroll
+  for (int k = 0; k < IT_BLK; k++) {
+    const int it = it0 + k;
+    if (it > ntz) break;
+    vito[((size_t)0 * nt1 + it) * ng + ll] = s0[k];
+    vito[((size_t)1 * nt1 + it) * ng + ll] = s1[k];
+    vito[((size_t)2 * nt1 + it) * ng + ll] = s2[k];
+  }
 }
 
-void launch_inflow_table(const double *modes, int nmodes, const double *pos, int ng, int ntz, double fctr,
-                         const double *uoo, double *vito, hipStream_t st) {
-  const int tot = ng * (ntz + 1);
-  hipLaunchKernelGGL(k_inflow_table, dim3((tot + 255) / 256), dim3(256), 0, st, modes, nmodes, pos, ng, ntz,
+void launch_inflow_table(const double *modes, const double *rot, int nmodes, const double *pos, int ng, int ntz,
+                         double fctr, const double *uoo, double *vito, hipStream_t st) {
+  const int tot = ng * ((ntz + IT_BLK) / IT_BLK);
+  hipLaunchKernelGGL(k_inflow_table, dim3((tot + 255) / 256), dim3(256), 0, st, modes, rot, nmodes, pos, ng, ntz,
                      fctr, uoo[0], uoo[1], uoo[2], vito);
 }
 
