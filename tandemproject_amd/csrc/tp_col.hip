@@ -54,6 +54,8 @@ __device__ inline const double &at(const double *base, unsigned off) {
 __device__ inline double &at(double *base, unsigned off) {
   return *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off);
 }
+// an output value, stored with the streaming hint
+__device__ inline void st(double *base, unsigned off, double v) { st_stream(&at(base, off), v); }
 
 // strided view of one line in global memory (boundary rows)
 struct GLine {
@@ -570,7 +572,7 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
         const int row = idx / L;
         const unsigned o = 8u * (base0 + (unsigned)idx);
         const double x = tile[row * pitch + idx - row * L];
-        at(dst, o) = add ? at(ob, o) + x : x;
+        st(dst, o, add ? ld_stream(&at(ob, o)) + x : x);
       }
       __syncthreads();   // the tile is read before the next rows overwrite it
     }
@@ -597,8 +599,8 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
         if (k < n && rl < nl) {
           const unsigned o = 8u * (unsigned)((w.a + rl) * b.nx + s0 + k);
           const double y = add ? ov[j] + xv[j] : xv[j];
-          at(dst, o) = y;
-          if (KIND == 1 && cpy) at(cpy, o) = y;
+          st(dst, o, y);
+          if (KIND == 1 && cpy) st(cpy, o, y);
         }
       }
       wave_sync();
@@ -615,8 +617,8 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
         if (k < n) {
           const unsigned o = b8 + (unsigned)(s0 + k) * st8;
           const double y = add ? ov[k] + r[k] : r[k];
-          at(dst, o) = y;
-          if (KIND == 1 && cpy) at(cpy, o) = y;
+          st(dst, o, y);
+          if (KIND == 1 && cpy) st(cpy, o, y);
         }
     }
   }

@@ -116,6 +116,14 @@ struct Work {
   int d;      // direction (line and face launches), side in bit 8
 };
 
+// Streaming access with the nontemporal hint, for values a pass reads or
+// writes once in arrays far larger than the caches (the next reader finds
+// them in HBM in any case): the pointwise passes' inputs and outputs and the
+// line solves' outputs.  Not for the line solves' window loads, whose halo
+// rows the neighbouring chunks read again from the cache.
+__device__ inline double ld_stream(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ inline void st_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
+
 // Work item of workgroup bid out of n.  Workgroups go to the GPU's 8 XCDs in
 // turn (bid mod 8), each XCD with its own L2; this hands each XCD a
 // contiguous run of work items instead (a bijection on [0, n)), so that

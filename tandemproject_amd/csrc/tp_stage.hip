@@ -225,10 +225,15 @@ __global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
   const Phys ph = phys();
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
     const double *qa = b.qa;
-    const Prim r = primitives(c, ph, qa[l], qa[N + l], qa[2 * N + l], qa[3 * N + l], qa[4 * N + l]);
-    b.de[l] = r.d1; b.de[N + l] = r.d2; b.de[2 * N + l] = r.d3; b.de[3 * N + l] = r.d4; b.de[4 * N + l] = r.d5;
-    b.p[l] = r.p;
-    b.ss[l] = r.mu;
+    const Prim r = primitives(c, ph, ld_stream(&qa[l]), ld_stream(&qa[N + l]), ld_stream(&qa[2 * N + l]),
+                              ld_stream(&qa[3 * N + l]), ld_stream(&qa[4 * N + l]));
+    st_stream(&b.de[l], r.d1);
+    st_stream(&b.de[N + l], r.d2);
+    st_stream(&b.de[2 * N + l], r.d3);
+    st_stream(&b.de[3 * N + l], r.d4);
+    st_stream(&b.de[4 * N + l], r.d5);
+    st_stream(&b.p[l], r.p);
+    st_stream(&b.ss[l], r.mu);
   }
 }
 
@@ -287,9 +292,9 @@ __global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
 // viscosity are formed again from q with k_primitives' operations (equal to
 // what it left in de, p and ss; q is read here anyway).
 __device__ inline void flux_put(double *fx, int N, int l, int m, double r1, double r2, double r3) {
-  fx[(size_t)(3 * (m - 1)) * N + l] = r1;
-  fx[(size_t)(3 * (m - 1) + 1) * N + l] = r2;
-  fx[(size_t)(3 * (m - 1) + 2) * N + l] = r3;
+  st_stream(&fx[(size_t)(3 * (m - 1)) * N + l], r1);
+  st_stream(&fx[(size_t)(3 * (m - 1) + 1) * N + l], r2);
+  st_stream(&fx[(size_t)(3 * (m - 1) + 2) * N + l], r3);
 }
 
 __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
@@ -305,10 +310,11 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
   const double *ya = b.yaco;
   double *fx = b.fx, *tv = b.tv;
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    const double x1 = xi[l], x2 = xi[N + l], x3 = xi[2 * N + l];
-    const double e1 = et[l], e2 = et[N + l], e3 = et[2 * N + l];
-    const double z1 = ze[l], z2 = ze[N + l], z3 = ze[2 * N + l];
-    const double qv[5] = {qa[l], qa[N + l], qa[2 * N + l], qa[3 * N + l], qa[4 * N + l]};
+    const double x1 = ld_stream(&xi[l]), x2 = ld_stream(&xi[N + l]), x3 = ld_stream(&xi[2 * N + l]);
+    const double e1 = ld_stream(&et[l]), e2 = ld_stream(&et[N + l]), e3 = ld_stream(&et[2 * N + l]);
+    const double z1 = ld_stream(&ze[l]), z2 = ld_stream(&ze[N + l]), z3 = ld_stream(&ze[2 * N + l]);
+    const double qv[5] = {ld_stream(&qa[l]), ld_stream(&qa[N + l]), ld_stream(&qa[2 * N + l]),
+                          ld_stream(&qa[3 * N + l]), ld_stream(&qa[4 * N + l])};
     const Prim pr = primitives(c, ph, qv[0], qv[1], qv[2], qv[3], qv[4]);
     const double u = pr.d2, v = pr.d3, ww = pr.d4;
     double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -316,8 +322,8 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
       double cb[4][3];
 #pragma unroll
       for (int f = 0; f < 4; f++) {
-        const double g1 = fx[(size_t)(3 * f) * N + l], g2 = fx[(size_t)(3 * f + 1) * N + l],
-                     g3 = fx[(size_t)(3 * f + 2) * N + l];
+        const double g1 = ld_stream(&fx[(size_t)(3 * f) * N + l]), g2 = ld_stream(&fx[(size_t)(3 * f + 1) * N + l]),
+                     g3 = ld_stream(&fx[(size_t)(3 * f + 2) * N + l]);
         cb[f][0] = x1 * g1 + e1 * g2 + z1 * g3;
         cb[f][1] = x2 * g1 + e2 * g2 + z2 * g3;
         cb[f][2] = x3 * g1 + e3 * g2 + z3 * g3;
@@ -682,8 +688,9 @@ __global__ void __launch_bounds__(256) k_rk(StageArgs a) {
   WORK_POINTS
   const double dtk = a.sc[b.cs * S_STRIDE + S_DTK];
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    double r1 = dtk * b.yaco[l];
-    for (int v = 0; v < 5; v++) b.qa[(size_t)v * N + l] = b.qo[(size_t)v * N + l] - r1 * b.de[(size_t)v * N + l];
+    double r1 = dtk * ld_stream(&b.yaco[l]);
+    for (int v = 0; v < 5; v++)
+      st_stream(&b.qa[(size_t)v * N + l], ld_stream(&b.qo[(size_t)v * N + l]) - r1 * ld_stream(&b.de[(size_t)v * N + l]));
   }
 }
 
