@@ -838,7 +838,7 @@ void enqueue_stage(tp_context *c, int nk) {
   }
   run_faces(c, c->L.gprep, FK_GCBC_PREP, a, K_GCBC);
   run_faces(c, c->L.gcopy, FK_COPY, a, K_GCBC);
-  for (int d = 0; d < 3; d++) run_faces(c, c->L.gapply[d], FK_GCBC_APPLY, a, K_GCBC);
+  for (int d = 0; d < 3; d++) run_faces(c, c->L.gapply[d], d == 0 ? FK_GCBC_APPLY_XI : FK_GCBC_APPLY, a, K_GCBC);
   launch_sc_rk(c->d_coef, c->d_sc, ncase, nk, c->stream);
   run_points(c, PK_RK, a, K_RK, 128.0);
   run_faces(c, c->L.wall, FK_WALL, a, K_GCBC);

@@ -275,6 +275,6 @@ int set_error(int code, const std::string &msg);
 enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPYF = 9, PK_INIT, PK_METRICS,
                    PK_FLUXALL = 14 };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,
-                  FK_SURF_SETUP, FK_SURF_SAMPLE };
+                  FK_SURF_SETUP, FK_SURF_SAMPLE, FK_GCBC_APPLY_XI };
 
 }  // namespace tp

@@ -618,68 +618,101 @@ __device__ inline void point_ijk(const Block &b, int l, int &x, int &y, int &z) 
 }
 
 // ---- GCBC/GCIC application along direction d (both ends), Main3D.f90:817-869 ----
+// XI (the xi faces): a face point's band (it and MBCI points inward) is a
+// contiguous piece of one row, and the rows of neighbouring face points are a
+// row pitch apart.  A thread per band would touch its own cache lines with
+// every access, so the workgroup first forms each face point's correction
+// (dq, 1/yaco) into LDS, then updates the bands together, consecutive lanes
+// on consecutive band points.  The other directions' bands run across
+// neighbouring face points' threads, coalesced as they are.
+template <bool XI>
 __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
+  __shared__ double sdq[XI ? 256 : 1][6];
   const Work w = a.work[xcd_item(blockIdx.x, gridDim.x)];
   const Block &b = a.blocks[w.blk];
   const CaseCoef &c = a.coef[b.cs];
   const int N = b.N, d = w.d & 0xff, ip = w.d >> 8, F = b.F[d];
-  int jk;
-  if (!face_point(b, w, d, jk)) return;
-  const int np = b.nbc[d][ip];
-  const double *sc = a.sc + b.cs * S_STRIDE;
-  int j, k;
-  face_jk(b, d, jk, j, k);
   const int i = ip * (b.L[d] - 1), iq = 1 - 2 * ip;
-  const double ra0 = iq;
-  int l = indx3(b, i, j, k, d);
-  double q[5], cm[3], cha[5], dha[5], dv[5], dq[5];
-  for (int v = 0; v < 5; v++) q[v] = b.qa[(size_t)v * N + l];
-  for (int v = 0; v < 3; v++) cm[v] = b.cm[d][(ip * 3 + v) * F + jk];
-  Elem e;
-  eleme(q, b.p[l], cm, sc + S_UMF1, e);
-  const double *drva = b.drva[d], *drvb = b.drvb[d];
-  for (int v = 0; v < 5; v++) { cha[v] = drva[(ip * 5 + v) * F + jk]; dha[v] = drvb[(ip * 5 + v) * F + jk]; }
-  double vv = e.vn + e.vs;
-  if (np == 10) {
-    if (ra0 * (vv + e.ao) > 0.0) cha[3] = 0.0;
-    if (ra0 * (vv - e.ao) > 0.0) cha[4] = 0.0;
-  } else if (np == 20 || np == 25) {
-    double rp = b.rpex[b.wall_off[d][ip] + jk];
-    if (rp == 0.0) {
-      const double *dd = sc + S_DUDT1;
-      double s = cm[0] * dd[0] + cm[1] * dd[1] + cm[2] * dd[2];
-      cha[3 + ip] = cha[4 - ip] + 2.0 * ra0 * e.aoi * q[0] * (s + 50.0 * vv);
+  int jk;
+  const bool on = face_point(b, w, d, jk);
+  if (on) {
+    const int np = b.nbc[d][ip];
+    const double *sc = a.sc + b.cs * S_STRIDE;
+    int j, k;
+    face_jk(b, d, jk, j, k);
+    const double ra0 = iq;
+    int l = indx3(b, i, j, k, d);
+    double q[5], cm[3], cha[5], dha[5], dv[5], dq[5];
+    for (int v = 0; v < 5; v++) q[v] = b.qa[(size_t)v * N + l];
+    for (int v = 0; v < 3; v++) cm[v] = b.cm[d][(ip * 3 + v) * F + jk];
+    Elem e;
+    eleme(q, b.p[l], cm, sc + S_UMF1, e);
+    const double *drva = b.drva[d], *drvb = b.drvb[d];
+    for (int v = 0; v < 5; v++) { cha[v] = drva[(ip * 5 + v) * F + jk]; dha[v] = drvb[(ip * 5 + v) * F + jk]; }
+    double vv = e.vn + e.vs;
+    if (np == 10) {
+      if (ra0 * (vv + e.ao) > 0.0) cha[3] = 0.0;
+      if (ra0 * (vv - e.ao) > 0.0) cha[4] = 0.0;
+    } else if (np == 20 || np == 25) {
+      double rp = b.rpex[b.wall_off[d][ip] + jk];
+      if (rp == 0.0) {
+        const double *dd = sc + S_DUDT1;
+        double s = cm[0] * dd[0] + cm[1] * dd[1] + cm[2] * dd[2];
+        cha[3 + ip] = cha[4 - ip] + 2.0 * ra0 * e.aoi * q[0] * (s + 50.0 * vv);
+      } else {
+        if (ra0 * vv > 0.0) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }
+        if (ra0 * (vv + e.ao) > 0.0) cha[3] = dha[3];
+        if (ra0 * (vv - e.ao) > 0.0) cha[4] = dha[4];
+      }
     } else {
       if (ra0 * vv > 0.0) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }
       if (ra0 * (vv + e.ao) > 0.0) cha[3] = dha[3];
       if (ra0 * (vv - e.ao) > 0.0) cha[4] = dha[4];
     }
-  } else {
-    if (ra0 * vv > 0.0) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }
-    if (ra0 * (vv + e.ao) > 0.0) cha[3] = dha[3];
-    if (ra0 * (vv - e.ao) > 0.0) cha[4] = dha[4];
+    for (int v = 0; v < 5; v++) dv[v] = cha[v] - drva[(ip * 5 + v) * F + jk];
+    xtr2q_apply(e, cm, dv, dq);
+    const double res = 1.0 / b.yaco[l];
+    if (XI) {
+      for (int v = 0; v < 5; v++) sdq[threadIdx.x][v] = dq[v];
+      sdq[threadIdx.x][5] = res;
+    } else {
+      // the band's de values are all read before any is written back, so
+      // the loads are issued together (no point of the band is another
+      // thread's)
+      double band[MBCI + 1][5];
+      int lb[MBCI + 1];
+#pragma unroll
+      for (int ii = 0; ii <= MBCI; ii++) {
+        lb[ii] = indx3(b, i + iq * ii, j, k, d);
+#pragma unroll
+        for (int v = 0; v < 5; v++) band[ii][v] = b.de[(size_t)v * N + lb[ii]];
+      }
+#pragma unroll
+      for (int ii = 0; ii <= MBCI; ii++) {
+        int x, y, z;
+        point_ijk(b, lb[ii], x, y, z);
+        double wgt = ii == 0 ? res : res * c.sbci[ii - 1];
+        double sbcc = wgt / gcbc_count(b, x, y, z);
+#pragma unroll
+        for (int v = 0; v < 5; v++) b.de[(size_t)v * N + lb[ii]] = band[ii][v] + sbcc * dq[v];
+      }
+    }
   }
-  for (int v = 0; v < 5; v++) dv[v] = cha[v] - drva[(ip * 5 + v) * F + jk];
-  xtr2q_apply(e, cm, dv, dq);
-  const double res = 1.0 / b.yaco[l];
-  // the band's de values are all read before any is written back, so the
-  // loads are issued together (no point of the band is another thread's)
-  double band[MBCI + 1][5];
-  int lb[MBCI + 1];
-#pragma unroll
-  for (int ii = 0; ii <= MBCI; ii++) {
-    lb[ii] = indx3(b, i + iq * ii, j, k, d);
-#pragma unroll
-    for (int v = 0; v < 5; v++) band[ii][v] = b.de[(size_t)v * N + lb[ii]];
-  }
-#pragma unroll
-  for (int ii = 0; ii <= MBCI; ii++) {
-    int x, y, z;
-    point_ijk(b, lb[ii], x, y, z);
-    double wgt = ii == 0 ? res : res * c.sbci[ii - 1];
-    double sbcc = wgt / gcbc_count(b, x, y, z);
-#pragma unroll
-    for (int v = 0; v < 5; v++) b.de[(size_t)v * N + lb[ii]] = band[ii][v] + sbcc * dq[v];
+  if (!XI) return;
+  __syncthreads();
+  // xi faces: element e = (v, face point t, band point ii), ii fastest; the
+  // work item's face points are jk = w.a + t, t < w.n (at most 256)
+  constexpr int NB = MBCI + 1;
+  for (int e = threadIdx.x; e < 5 * NB * w.n; e += blockDim.x) {
+    const int ii = e % NB, r = e / NB, t = r % w.n, v = r / w.n;
+    int j, k;
+    face_jk(b, 0, w.a + t, j, k);
+    const int x = i + iq * ii;
+    const int l = indx3(b, x, j, k, 0);
+    const double res = sdq[t][5];
+    const double wgt = ii == 0 ? res : res * c.sbci[ii - 1];
+    const double sbcc = wgt / gcbc_count(b, x, j, k);
+    b.de[(size_t)v * N + l] = b.de[(size_t)v * N + l] + sbcc * sdq[t][v];
   }
 }
 
@@ -1016,13 +1049,14 @@ void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a) {
   switch (which) {
     case 0: LAUNCH(k_gcbc_prep, nwg, 256, st, a); break;
     case 1: LAUNCH(k_face_copy, nwg, 256, st, a); break;
-    case 2: LAUNCH(k_gcbc_apply, nwg, 256, st, a); break;
+    case 2: LAUNCH(k_gcbc_apply<false>, nwg, 256, st, a); break;
     case 3: LAUNCH(k_wall, nwg, 256, st, a); break;
     case 4: LAUNCH(k_avg_capture, nwg, 256, st, a); break;
     case 5: LAUNCH(k_avg_apply, nwg, 256, st, a); break;
     case 6: LAUNCH(k_normals, nwg, 256, st, a); break;
     case 7: LAUNCH(k_surf_setup, nwg, 256, st, a); break;
     case 8: LAUNCH(k_surf_sample, nwg, 256, st, a); break;
+    case 9: LAUNCH(k_gcbc_apply<true>, nwg, 256, st, a); break;
   }
 }
 void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, const int *wwr, int ncase,
