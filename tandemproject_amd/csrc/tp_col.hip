@@ -637,13 +637,16 @@ static void launch_col_c(int C, int P, int nwg, int nf, hipStream_t st, const Li
     else                                                                                           \
       hipLaunchKernelGGL((k_col<D, KIND, CV, false>), dim3(nwg, nf), dim3(64 * P), lds, st, a);    \
     break;
-    CASE_C(8) CASE_C(16) CASE_C(24) CASE_C(32)
+    CASE_C(8) CASE_C(12) CASE_C(16) CASE_C(20) CASE_C(24) CASE_C(28) CASE_C(32)
 #undef CASE_C
   }
 }
 
+// register capacity of a chunk of `chunk` points: the next multiple of 4 from
+// 8 (rows past the chunk are padding the sweeps run over)
 int col_capacity(int chunk) {
-  return chunk <= 8 ? 8 : chunk <= 16 ? 16 : chunk <= 24 ? 24 : chunk <= 32 ? 32 : 0;
+  if (chunk > 32) return 0;
+  return chunk <= 8 ? 8 : (chunk + 3) / 4 * 4;
 }
 
 void launch_col(int d, int kind, int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a) {
