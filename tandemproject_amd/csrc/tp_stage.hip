@@ -315,6 +315,15 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
     const double z1 = ld_stream(&ze[l]), z2 = ld_stream(&ze[N + l]), z3 = ld_stream(&ze[2 * N + l]);
     const double qv[5] = {ld_stream(&qa[l]), ld_stream(&qa[N + l]), ld_stream(&qa[2 * N + l]),
                           ld_stream(&qa[3 * N + l]), ld_stream(&qa[4 * N + l])};
+    // the 12 gradients loaded with the rest, before the primitives are
+    // formed, so that all of the point's loads are in flight at once
+    double gr[4][3];
+    if (visc) {
+#pragma unroll
+      for (int f = 0; f < 4; f++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) gr[f][m] = ld_stream(&fx[(size_t)(3 * f + m) * N + l]);
+    }
     const Prim pr = primitives(c, ph, qv[0], qv[1], qv[2], qv[3], qv[4]);
     const double u = pr.d2, v = pr.d3, ww = pr.d4;
     double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -322,8 +331,7 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
       double cb[4][3];
 #pragma unroll
       for (int f = 0; f < 4; f++) {
-        const double g1 = ld_stream(&fx[(size_t)(3 * f) * N + l]), g2 = ld_stream(&fx[(size_t)(3 * f + 1) * N + l]),
-                     g3 = ld_stream(&fx[(size_t)(3 * f + 2) * N + l]);
+        const double g1 = gr[f][0], g2 = gr[f][1], g3 = gr[f][2];
         cb[f][0] = x1 * g1 + e1 * g2 + z1 * g3;
         cb[f][1] = x2 * g1 + e2 * g2 + z2 * g3;
         cb[f][2] = x3 * g1 + e3 * g2 + z3 * g3;
