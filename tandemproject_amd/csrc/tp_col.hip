@@ -54,7 +54,7 @@ __device__ inline const double &at(const double *base, unsigned off) {
 __device__ inline double &at(double *base, unsigned off) {
   return *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off);
 }
-// an output value, stored with the streaming hint
+// a xi output value, stored with the streaming hint
 __device__ inline void st(double *base, unsigned off, double v) { st_stream(&at(base, off), v); }
 
 // strided view of one line in global memory (boundary rows)
@@ -617,8 +617,11 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
         if (k < n) {
           const unsigned o = b8 + (unsigned)(s0 + k) * st8;
           const double y = add ? ov[k] + r[k] : r[k];
-          st(dst, o, y);
-          if (KIND == 1 && cpy) st(cpy, o, y);
+          // the default policy: a row's 512 bytes start anywhere in a cache
+          // line, and the neighbouring work item writes the rest of its edge
+          // lines; the streaming hint evicts them half-written
+          at(dst, o) = y;
+          if (KIND == 1 && cpy) at(cpy, o) = y;
         }
     }
   }

@@ -119,8 +119,10 @@ struct Work {
 // Streaming access with the nontemporal hint, for values a pass reads or
 // writes once in arrays far larger than the caches (the next reader finds
 // them in HBM in any case): the pointwise passes' inputs and outputs and the
-// line solves' outputs.  Not for the line solves' window loads, whose halo
-// rows the neighbouring chunks read again from the cache.
+// xi line solves' outputs (whole rows).  Not for the line solves' window
+// loads, whose halo rows the neighbouring chunks read again from the cache,
+// nor for the eta/zeta solves' row stores, whose edge cache lines the
+// neighbouring work item completes.
 __device__ inline double ld_stream(const double *p) { return __builtin_nontemporal_load(p); }
 __device__ inline void st_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
 
