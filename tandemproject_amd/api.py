@@ -173,6 +173,8 @@ class PinnedBuffer:
     for states of hundreds of MB.  `array` is valid until free()."""
 
     def __init__(self, n):
+        self._p = None
+        self.array = None
         self._hip = ctypes.CDLL("libamdhip64.so")
         self._p = ctypes.c_void_p()
         m = max(int(n), 1)
