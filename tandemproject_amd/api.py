@@ -312,7 +312,8 @@ class Solver:
         """Time every line-solver configuration on these blocks and keep the
         fastest per direction and kind.  Returns {(direction, kind): (chunks,
         microseconds per launch)}; chunks 0 is the sequential kernel.
-        Overwrites the scratch array rr."""
+        Overwrites the stage's gradient and flux scratch (fx), so call it
+        between steps."""
         ch = np.zeros(6, np.int32); us = np.zeros(6)
         _check(load().tp_autotune(self.h, int(reps), _ip(ch), _dp(us)))
         return {(i // 2, ("deriv", "filter")[i % 2]): (int(ch[i]), float(us[i])) for i in range(6)}
