@@ -38,10 +38,12 @@ def pair(layout, params, exact=False, chunks=0):
 
 # line-solver configurations of the per-call tests: the sequential sweep, and
 # chunked sweeps with fixed chunk counts so that every line is split over
-# several waves (the automatic choice may keep short lines in one chunk)
+# several waves (the automatic choice may keep short lines in one chunk);
+# over the cases below they use every chunk capacity from 8 to 32 (the
+# 64-point lines 32, 24 and 12, the 240-point lines 28, 20 and 16)
 CALL_MODES = {"sequential": dict(exact=True), "chunks2": dict(chunks=2), "chunks3": dict(chunks=3),
-              "chunks4": dict(chunks=4), "chunks7": dict(chunks=7), "chunks16": dict(chunks=16),
-              "auto": dict()}
+              "chunks4": dict(chunks=4), "chunks7": dict(chunks=7), "chunks9": dict(chunks=9),
+              "chunks12": dict(chunks=12), "chunks16": dict(chunks=16), "auto": dict()}
 
 
 LINE_CASES = {
