@@ -259,7 +259,11 @@ def cpu_baseline(seconds_target=15.0):
     """The oracle (C restatement, OpenMP) on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    threads = min(os.cpu_count() or 1, 16)
+    # the host cores this process may run on, and the share of them the
+    # process is given (OMP_NUM_THREADS, when the host sets one)
+    avail = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, share) if share > 0 else avail
     po.set_threads(threads)
     lay, ss, params, dims = workload(scale=0.2)
     o = po.Oracle(lay, params)
@@ -278,7 +282,8 @@ def cpu_baseline(seconds_target=15.0):
         o.step(2); steps += 2
     dt = time.time() - t0
     pts = lay.npoints()
-    return {"value": pts * steps / dt / 1e6, "unit": "Mpoint-steps/s", "cores": threads, "kind": "port",
+    return {"value": pts * steps / dt / 1e6, "unit": "Mpoint-steps/s", "cores": threads,
+            "cores_available": avail, "kind": "port",
             "sample": f"oracle/tp_oracle.c on {lay.nb} sub-blocks, {pts / 1e6:.3f} M points "
                       f"(lxibk={dims['lxibk']}, letbk={dims['letbk']}, nz={dims['lzebk'][0] + 1}), "
                       f"{steps} steps, {dt:.1f} s"}

@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB: another build of the oracle (the tests that plant mistakes in a
+# copy of it load that copy instead)
+LIB_PATH = os.environ.get("ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 
 I_NKRK, I_NVISCOUS, I_NSMF, I_LES, I_NEXTRABC, I_NEXTGCIC, I_INFLOW, I_NMODES, I_SEED, I_NTZ = range(10)
 N_IPARAM = 16
@@ -23,10 +25,16 @@ A_NAMES = ["qa", "qo", "de", "rr", "ss", "xim", "etm", "zem", "p", "yaco", "tv",
            "drva1", "drva2", "drva3", "drvb1", "drvb2", "drvb3", "send1", "send2", "send3",
            "recv1", "recv2", "recv3", "cm1", "cm2", "cm3", "xu", "xl", "yu", "yl",
            "rpex", "sbcc", "asz", "bsz", "vito", "atz"]
+# sections of a stage, in the order orc_stage runs them (tp_oracle.h)
+PARTS = ["begin", "viscous", "flux", "sponge", "gcbc", "rk", "wall", "junction", "avg"]
 S_NAMES = ["n", "ndt", "dt", "dts", "dte", "timo", "umf1", "umf2", "umf3", "dudt1",
            "dudt2", "dudt3", "dtk", "dtko", "aoa"]
 
 _lib = None
+
+
+class OracleError(RuntimeError):
+    """A case or selector the oracle refused (its message from orc_last_error)."""
 
 
 def lib():
@@ -51,10 +59,15 @@ def lib():
         L.orc_coefficients.argtypes = [P, dp, dp, dp, dp, dp, dp]
         for f in ("orc_metrics", "orc_initialo", "orc_filter_step"):
             getattr(L, f).argtypes = [P]
+        L.orc_last_error.restype = ctypes.c_char_p
         L.orc_mpigo.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_deriv.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_filte.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        for f in ("orc_mpigo", "orc_deriv", "orc_filte"):
+            getattr(L, f).restype = ctypes.c_int
         L.orc_stage.argtypes = [P, ctypes.c_int]
+        L.orc_stage_part.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_stage_part.restype = ctypes.c_int
         L.orc_step.argtypes = [P, ctypes.c_int]
         L.orc_stats_start.argtypes = [P, ctypes.c_int]
         L.orc_stats_forces.argtypes = [P, dp]
@@ -69,11 +82,18 @@ def lib():
         L.orc_xtr2q.argtypes = [dp, ctypes.c_double, dp, dp, dp]
         L.orc_movef.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4 + [dp] + \
             [ctypes.c_double] * 5 + [dp, dp, dp]
+        L.orc_sponge_list.argtypes = [P, ctypes.c_int, ip, ip]
+        L.orc_sponge_list.restype = ctypes.c_int
         L.orc_set_inflow_modes.argtypes = [P, ctypes.c_int, dp]
         L.orc_inflow_weights.restype = ctypes.c_int
         L.orc_inflow_weights.argtypes = [P, ctypes.c_double, ctypes.c_double, ip, dp, dp]
         _lib = L
     return _lib
+
+
+def _check(rc):
+    if rc:
+        raise OracleError(lib().orc_last_error().decode())
 
 
 def _dp(a):
@@ -113,6 +133,8 @@ class Oracle:
         d, b, n, s, j, v = self._keep
         self.h = L.orc_create(layout.nb, _ip(d), _ip(b), _ip(n), _ip(s), layout.njct,
                               _ip(j), _ip(v), _ip(ipar), _dp(dpar))
+        if not self.h:
+            raise OracleError(L.orc_last_error().decode())
         self.nb = layout.nb
 
     def set_inflow_modes(self, modes):
@@ -126,6 +148,13 @@ class Oracle:
         il = np.zeros(5, np.int32); tl = np.zeros(5); w = np.zeros(5)
         on = lib().orc_inflow_weights(self.h, float(timo), float(dtk), _ip(il), _dp(tl), _dp(w))
         return (il, tl, w) if on else None
+
+    def sponge_list(self, b):
+        """(point index, inflow index or -1) of every sponge point of block b"""
+        n = lib().orc_sponge_list(self.h, int(b), None, None)
+        l = np.zeros(max(n, 1), np.int32); g = np.zeros(max(n, 1), np.int32)
+        lib().orc_sponge_list(self.h, int(b), _ip(l), _ip(g))
+        return l[:n], g[:n]
 
     def stats_start(self, every):
         """Sample the surface statistics every `every` steps (0: off)."""
@@ -183,19 +212,23 @@ class Oracle:
         lib().orc_initialo(self.h)
 
     def mpigo(self, nt, nrt, n45):
-        lib().orc_mpigo(self.h, nt, nrt, n45)
+        _check(lib().orc_mpigo(self.h, nt, nrt, n45))
 
     def deriv(self, nn, nz, m):
-        lib().orc_deriv(self.h, nn, nz, m)
+        _check(lib().orc_deriv(self.h, nn, nz, m))
 
     def filte(self, nn, nz):
-        lib().orc_filte(self.h, nn, nz)
+        _check(lib().orc_filte(self.h, nn, nz))
 
     def filter_step(self):
         lib().orc_filter_step(self.h)
 
     def stage(self, nk):
         lib().orc_stage(self.h, nk)
+
+    def stage_part(self, nk, part):
+        """One section of stage nk (a name of PARTS)."""
+        _check(lib().orc_stage_part(self.h, nk, PARTS.index(part)))
 
     def step(self, n=1):
         lib().orc_step(self.h, n)

@@ -556,9 +556,17 @@ void orc_xtr2q(const double *q5, double p, const double *cm3, const double *umf3
 /* case construction                                                        */
 /* ---------------------------------------------------------------------- */
 
+/* Errors: the functions that take caller input return NULL or a nonzero code
+ * and leave a message here instead of ending the process, so that a test
+ * that feeds the oracle a bad case fails alone (orc_last_error). */
+static char g_err[256];
+static int g_oom;
+const char *orc_last_error(void) { return g_err; }
+static void set_err(const char *msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
+
 static double *dalloc(size_t n) {
   double *p = (double *)calloc(n ? n : 1, sizeof(double));
-  if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+  if (!p) { g_oom = 1; set_err("oracle: out of memory"); }
   return p;
 }
 
@@ -611,6 +619,16 @@ orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
                 const int32_t *jct, const int32_t *jvel,
                 const int32_t *ip, const double *dp) {
   init_constants();
+  g_oom = 0;
+  g_err[0] = 0;
+  if (nb < 1) { set_err("oracle: a case needs at least one block"); return NULL; }
+  for (int b = 0; b < nb; b++)
+    for (int nn = 0; nn < 3; nn++)
+      if (dims[b * 3 + nn] < 1 || dims[b * 3 + nn] + 1 > MAXLINE) {
+        snprintf(g_err, sizeof(g_err), "oracle: block %d direction %d has %d points (1 < points <= %d)",
+                 b, nn + 1, dims[b * 3 + nn] + 1, MAXLINE);
+        return NULL;
+      }
   orc *c = (orc *)calloc(1, sizeof(orc));
   c->nb = nb;
   c->b = (oblk *)calloc(nb, sizeof(oblk));
@@ -682,7 +700,7 @@ orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
       int is = nn == 0 ? 0 : nn == 1 ? B->lxi + 1 : B->lxi + B->let + 2;
       int n = L[nn] + 1;
       double xu[3 * MAXLINE], xl[2 * MAXLINE];
-      if (n > MAXLINE) { fprintf(stderr, "oracle: line too long\n"); abort(); }
+      if (g_oom) { orc_destroy(c); return NULL; }
       int ns = B->ndf[nn][0], ne = B->ndf[nn][1];
       orc_penta(n, &c->albed[ns][0][0], &c->albed[ne][0][0], alpha, beta, xu, xl);
       for (int i = 0; i < n; i++) {
@@ -705,9 +723,10 @@ orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
         if ((np == 35 || np == 45) && need < LMD + 2) need = LMD + 2;
       }
       if (L[nn] + 1 < need) {
-        fprintf(stderr, "oracle: block %d direction %d has %d points; its faces need >= %d\n",
-                b, nn + 1, L[nn] + 1, need);
-        abort();
+        snprintf(g_err, sizeof(g_err), "oracle: block %d direction %d has %d points; its faces need >= %d",
+                 b, nn + 1, L[nn] + 1, need);
+        orc_destroy(c);
+        return NULL;
       }
     }
     B->xbc_ra0 = c->cbca_ra0;
@@ -789,6 +808,18 @@ double *orc_array(orc *c, int b, int w) {
 
 double *orc_scalars(orc *c) { return c->sc; }
 
+int orc_sponge_list(const orc *c, int b, int32_t *l, int32_t *g) {
+  const oblk *B = &c->b[b];
+  const int n = B->lsz + 1;
+  if (l)
+    for (int q = 0; q < n; q++) l[q] = B->lcsz[q];
+  if (g) {
+    for (int q = 0; q < n; q++) g[q] = -1;
+    for (int k = 0; k <= B->ltz; k++) g[B->lctz[k]] = k;
+  }
+  return n;
+}
+
 void orc_set_inflow_modes(orc *c, int nmodes, const double *modes8) {
   free(c->modes);
   c->nmodes = nmodes;
@@ -848,7 +879,11 @@ void orc_coefficients(orc *c, double *pbci, double *pbco, double *pbcot,
 /* ---------------------------------------------------------------------- */
 /* mpigo: src/Subroutines3D.f90:16-80, over all blocks                      */
 /* ---------------------------------------------------------------------- */
-void orc_mpigo(orc *c, int nt, int nrt, int n45) {
+int orc_mpigo(orc *c, int nt, int nrt, int n45) {
+  if (nt < 0 || nt > 1 || nrt < 0 || nrt > 1 || n45 < 0 || n45 > 1) {
+    snprintf(g_err, sizeof(g_err), "oracle: mpigo(%d, %d, %d) out of range", nt, nrt, n45);
+    return 1;
+  }
   int mp = nt == 0 ? LMD : LMF;
   for (int b = 0; b < c->nb; b++) {
     oblk *B = &c->b[b];
@@ -922,6 +957,7 @@ void orc_mpigo(orc *c, int nt, int nrt, int n45) {
       }
     }
   }
+  return 0;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -998,12 +1034,13 @@ static void deriv_block(orc *c, oblk *B, int nn, int nz, int m) {
   }
 }
 
-void orc_deriv(orc *c, int nn, int nz, int m) {
+int orc_deriv(orc *c, int nn, int nz, int m) {
   if (nn < 1 || nn > 3 || nz < 1 || nz > 3 || m < 0 || m > 5) {
-    fprintf(stderr, "oracle: deriv(%d, %d, %d) out of range\n", nn, nz, m);
-    abort();
+    snprintf(g_err, sizeof(g_err), "oracle: deriv(%d, %d, %d) out of range", nn, nz, m);
+    return 1;
   }
   for (int b = 0; b < c->nb; b++) deriv_block(c, &c->b[b], nn - 1, nz, m);
+  return 0;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1092,8 +1129,13 @@ static void filte_block(orc *c, oblk *B, int nn, int nz) {
   }
 }
 
-void orc_filte(orc *c, int nn, int nz) {
+int orc_filte(orc *c, int nn, int nz) {
+  if (nn < 1 || nn > 3 || nz < 1 || nz > 3) {
+    snprintf(g_err, sizeof(g_err), "oracle: filte(%d, %d) out of range", nn, nz);
+    return 1;
+  }
   for (int b = 0; b < c->nb; b++) filte_block(c, &c->b[b], nn - 1, nz);
+  return 0;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1510,17 +1552,20 @@ int orc_stats_wall(const orc *c, int b, double *mean_p, double *rms_p, double *a
   return nwl;
 }
 
-void orc_stage(orc *c, int nk) {
-  int nkrk = c->nkrk;
+/* stage start (frame velocity for the stage, src/Main3D.f90:571-572), primitives
+ * (:576-583) and the time step size at the first stage (:587-622) */
+static void stage_begin(orc *c, int nk) {
   double *sc = c->sc;
+  const int nkrk = c->nkrk;
   double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* src/Main3D.f90:571-572 */
   {
     double dtko = dt * (double)(nk - 2 > 0 ? (nk - 2 < 1 ? nk - 2 : 1) : 0) / (nkrk - nk + 3);
     double dtk = dt * (double)(nk - 1 < 1 ? nk - 1 : 1) / (nkrk - nk + 2);
     movef_c(c, dtko, dtk);
   }
-  const double *umf = &sc[ORC_S_UMF1];
   /* primitives, src/Main3D.f90:576-583 */
   for (int b = 0; b < c->nb; b++) {
     oblk *B = &c->b[b];
@@ -1576,6 +1621,15 @@ void orc_stage(orc *c, int nk) {
     sc[ORC_S_DT] = sc[ORC_S_DTS] + (sc[ORC_S_DTE] - sc[ORC_S_DTS]) * (s * s);
     dt = sc[ORC_S_DT];
   }
+}
+
+/* viscous stresses and heat fluxes into tv, src/Main3D.f90:626-693 */
+static void stage_viscous(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* viscous stresses and heat fluxes, src/Main3D.f90:626-693 */
   if (c->nviscous == 1) {
     double **dst = (double **)malloc(sizeof(double *) * 3 * c->nb);
@@ -1648,6 +1702,16 @@ void orc_stage(orc *c, int nk) {
       }
     }
   }
+}
+
+/* surface sample at the first stage (:695-706), contravariant velocities and the
+ * divergence of the metric-weighted fluxes into de, src/Main3D.f90:708-768 */
+static void stage_flux(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* surface forces and pressure statistics at the first stage, src/Main3D.f90:695-706 */
   if (nk == 1) wall_sample(c);
   /* fluxes, src/Main3D.f90:710-768 */
@@ -1722,6 +1786,15 @@ void orc_stage(orc *c, int nk) {
       for (int l = 0; l < N; l++) de[l] = rr[l] + rr[N + l] + rr[2 * N + l];
     }
   }
+}
+
+/* sponge relaxation with the inflow target, src/SingleTurbulence3D.f90:332-374 */
+static void stage_sponge(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* sponge, src/SingleTurbulence3D.f90:332-372 */
   int lag_on = 0, il5[5];
   double tl5[5], w5[5];
@@ -1757,6 +1830,15 @@ void orc_stage(orc *c, int nk) {
       B->de[4 * N + l] = B->de[4 * N + l] + B->bsz[q] * (B->p[l] - ham * pow(B->rr[l], gam));
     }
   }
+}
+
+/* characteristic boundary and interface conditions, src/Main3D.f90:778-869 */
+static void stage_gcbc(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* GCBC/GCIC preparation, src/Main3D.f90:780-795 */
   for (int b = 0; b < c->nb; b++) {
     oblk *B = &c->b[b];
@@ -1863,6 +1945,15 @@ void orc_stage(orc *c, int nk) {
       }
     }
   }
+}
+
+/* frame velocity for the update (:873-874) and the RK update, src/Main3D.f90:876-881 */
+static void stage_rk(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* RK update, src/Main3D.f90:873-881 */
   {
     double dtko = dt * (double)(nk - 1 < 1 ? nk - 1 : 1) / (nkrk - nk + 2);
@@ -1881,6 +1972,15 @@ void orc_stage(orc *c, int nk) {
         B->qa[(size_t)v * N + l] = B->qo[(size_t)v * N + l] - r1 * B->de[(size_t)v * N + l];
     }
   }
+}
+
+/* wall temperature and velocity, src/Main3D.f90:884-895 */
+static void stage_wall(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* wall temperature and velocity, src/Main3D.f90:886-895 */
   for (int b = 0; b < c->nb; b++) {
     oblk *B = &c->b[b];
@@ -1905,6 +2005,15 @@ void orc_stage(orc *c, int nk) {
           }
       }
   }
+}
+
+/* corner junctions, src/SingleTurbulence3D.f90:378-443 */
+static void stage_junction(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* junction, src/SingleTurbulence3D.f90:378-443 */
   for (int jn = 0; jn < c->njct; jn++) {
     const int *J = c->jct + jn * 12;
@@ -1929,6 +2038,15 @@ void orc_stage(orc *c, int nk) {
         for (int m = 0; m < 5; m++) M[q]->qa[(size_t)m * M[q]->N + lq[q]] = r[m];
     }
   }
+}
+
+/* interface averaging, src/Main3D.f90:897-952 */
+static void stage_avg(orc *c, int nk) {
+  double *sc = c->sc;
+  const int nkrk = c->nkrk;
+  double dt = sc[ORC_S_DT];
+  const double *umf = &sc[ORC_S_UMF1];
+  (void)sc; (void)nkrk; (void)dt; (void)umf; (void)nk;
   /* interface averaging, src/Main3D.f90:901-952 */
   for (int b = 0; b < c->nb; b++) {
     oblk *B = &c->b[b];
@@ -1998,6 +2116,24 @@ void orc_stage(orc *c, int nk) {
       }
     }
   }
+}
+
+/* One stage, src/Main3D.f90:567-958: its sections in the reference's order. */
+static void (*const stage_parts[ORC_P_NPART])(orc *, int) = {
+    stage_begin, stage_viscous, stage_flux, stage_sponge, stage_gcbc,
+    stage_rk, stage_wall, stage_junction, stage_avg};
+
+int orc_stage_part(orc *c, int nk, int part) {
+  if (nk < 1 || nk > c->nkrk || part < 0 || part >= ORC_P_NPART) {
+    snprintf(g_err, sizeof(g_err), "oracle: stage part (%d, %d) out of range", nk, part);
+    return 1;
+  }
+  stage_parts[part](c, nk);
+  return 0;
+}
+
+void orc_stage(orc *c, int nk) {
+  for (int part = 0; part < ORC_P_NPART; part++) stage_parts[part](c, nk);
 }
 
 void orc_step(orc *c, int nsteps) {

@@ -63,17 +63,24 @@ typedef struct orc orc;
  * jct[j*12 + mbr*3 + {0,1,2}] : (block, ip, jp) of member mbr (0..3) of junction j
  * jvel[j*2 + {0,1}]           : members whose velocity is averaged
  */
+/* NULL for a case the oracle cannot hold (lines shorter than their faces
+ * need or longer than 8192 points, no memory); orc_last_error() says why */
 orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
                 const int32_t *ncd, const int32_t *nsz, int njct,
                 const int32_t *jct, const int32_t *jvel,
                 const int32_t *iparams, const double *dparams);
 void orc_destroy(orc *c);
+const char *orc_last_error(void);
 /* inflow: the Fourier modes (8 doubles each: k, a, phase, 0) the velocity
  * table is built from; call before orc_metrics */
 void orc_set_inflow_modes(orc *c, int nmodes, const double *modes8);
 /* time interpolation of the inflow table at (timo, dtk): table rows il[5],
  * node times tl[5] and weights w[5]; returns 0 before tgusto */
 int orc_inflow_weights(const orc *c, double timo, double dtk, int *il5, double *tl5, double *w5);
+/* sponge points of block b (set by orc_metrics): their point indices l[q]
+ * and inflow indices g[q] (-1: not an inflow point); returns the count;
+ * either pointer may be NULL */
+int orc_sponge_list(const orc *c, int b, int32_t *l, int32_t *g);
 int orc_npoints(const orc *c, int b);
 int orc_count(const orc *c, int b, int which); /* element count of an array */
 double *orc_array(orc *c, int b, int which);
@@ -98,13 +105,28 @@ void orc_metrics(orc *c);
 void orc_initialo(orc *c);
 
 /* per-call drop-ins over all blocks */
-void orc_mpigo(orc *c, int nt, int nrt, int n45);
-void orc_deriv(orc *c, int nn, int nz, int m);
-void orc_filte(orc *c, int nn, int nz);
+/* these three return 0, or 1 for a selector out of range (orc_last_error) */
+int orc_mpigo(orc *c, int nt, int nrt, int n45);
+int orc_deriv(orc *c, int nn, int nz, int m);
+int orc_filte(orc *c, int nn, int nz);
 
 /* step pieces and whole steps */
 void orc_filter_step(orc *c);
 void orc_stage(orc *c, int nk);
+/* the sections of a stage, in the order orc_stage runs them:
+ *   BEGIN     frame velocity, primitives, time step (src/Main3D.f90:571-622)
+ *   VISCOUS   stresses and heat fluxes into tv (:626-693; nothing when inviscid)
+ *   FLUX      surface sample, flux divergence into de (:695-768)
+ *   SPONGE    sponge and inflow target (src/SingleTurbulence3D.f90:332-374)
+ *   GCBC      characteristic boundary/interface conditions (:778-869)
+ *   RK        RK update of qa (:873-881)
+ *   WALL      wall temperature and velocity (:884-895)
+ *   JUNCTION  corner junctions (src/SingleTurbulence3D.f90:378-443)
+ *   AVG       interface averaging (:897-952)
+ * returns 1 for a stage or section out of range */
+enum { ORC_P_BEGIN = 0, ORC_P_VISCOUS, ORC_P_FLUX, ORC_P_SPONGE, ORC_P_GCBC, ORC_P_RK,
+       ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG, ORC_P_NPART };
+int orc_stage_part(orc *c, int nk, int part);
 void orc_step(orc *c, int nsteps);
 
 /* standalone numerics (setup routines) */

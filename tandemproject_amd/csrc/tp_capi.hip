@@ -936,10 +936,11 @@ namespace {
 
 // sub-array offsets of one block's arena (256-byte aligned)
 struct Arena {
-  size_t qa, qo, de, rr, ss, xi, et, ze, p, ya, tv, fx, f[3][4], lud, luf, rp, sa, sn, ss2, total;
+  size_t qa, qo, de, rr, ss, xi, et, ze, p, ya, tv, fx, f[3][4], lud, luf, rp, sa, sn, ss2, va, total;
 };
 
-Arena arena_layout(const Block &b, int nwall) {
+// les: the case needs the single-precision Smagorinsky volume term per point
+Arena arena_layout(const Block &b, int nwall, bool les) {
   Arena A;
   size_t off = 0;
   auto take = [&](size_t n) { size_t o = off; off += ((n * sizeof(double) + 255) / 256) * 256; return o; };
@@ -953,6 +954,7 @@ Arena arena_layout(const Block &b, int nwall) {
   }
   A.lud = take(5 * (size_t)b.nlu); A.luf = take(5 * (size_t)b.nlu); A.rp = take(std::max(nwall, 1));
   A.sa = take(b.nsurf); A.sn = take(3 * (size_t)b.nsurf); A.ss2 = take(3 * (size_t)b.nsurf);
+  A.va = les ? take((N + 1) / 2) : 0;   // N floats
   A.total = off;
   return A;
 }
@@ -1087,7 +1089,7 @@ size_t case_bytes(const HostCase &h) {
   for (int lb = 0; lb < h.nb; lb++) {
     int nwall = 0;
     const Block b = block_shape(h, lb, &nwall);
-    total += acct(arena_layout(b, nwall).total);
+    total += acct(arena_layout(b, nwall, h.ip[TP_I_LES] == 1).total);
     total += acct(sizeof(double) * col_block_doubles(b.L));
     if (lb >= (int)h.grid.size() || h.grid[lb].empty()) continue;
     const double *x = h.grid[lb].data(), *y = x + b.N, *z = y + b.N;
@@ -1116,7 +1118,8 @@ int build_case(HostCase &h) {
   for (int lb = 0; lb < h.nb; lb++) {
     int nwall = 0;
     Block b = block_shape(h, lb, &nwall);
-    const Arena A = arena_layout(b, nwall);
+    const bool les = h.ip[TP_I_LES] == 1;
+    const Arena A = arena_layout(b, nwall, les);
     char *base = nullptr;
     if ((rc = dalloc(h.mem, (void **)&base, A.total))) return rc;
     HIPCHK(hipMemset(base, 0, A.total));
@@ -1129,6 +1132,7 @@ int build_case(HostCase &h) {
     }
     b.lud = P(A.lud); b.luf = P(A.luf); b.rpex = P(A.rp);
     b.sarea = P(A.sa); b.snor = P(A.sn); b.sst = P(A.ss2);
+    b.varr = les ? reinterpret_cast<float *>(base + A.va) : nullptr;
     std::vector<double> lud(5 * b.nlu), luf(5 * b.nlu);
     const int nl = b.nlu;
     for (int d = 0; d < 3; d++) {
@@ -1482,6 +1486,18 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
   }
   std::memcpy(h.ip, iparams, sizeof(h.ip));
   std::memcpy(h.dp, dparams, sizeof(h.dp));
+  // block size: the line solves address a field with 32-bit byte offsets
+  // from its base, so a block holds fewer than 2^29 points
+  for (int b = 0; b < nb; b++) {
+    int64_t np = 1;
+    for (int d = 0; d < 3; d++) {
+      if (dims[3 * b + d] < 1) return fail(TP_E_LAYOUT, "block " + std::to_string(b) + ": fewer than 2 points in a direction");
+      np *= (int64_t)dims[3 * b + d] + 1;
+    }
+    if (8 * np >= (int64_t)1 << 32)
+      return fail(TP_E_SIZE, "block " + std::to_string(b) + " has " + std::to_string(np) +
+                                 " points; a block holds fewer than 2^29 (536870912)");
+  }
   // validate the topology: line lengths the stencils and closures need, and
   // neighbours that exist and match
   for (int b = 0; b < nb; b++) {
@@ -1663,6 +1679,13 @@ int32_t tp_setup_geometry(tp_context *c) {
       const double *x = h.grid[lb].data(), *y = x + b.N, *z = y + b.N;
       std::vector<double> yaco(b.N);
       HIPCHK(hipMemcpy(yaco.data(), b.yaco, sizeof(double) * b.N, hipMemcpyDeviceToHost));
+      if (b.varr) {
+        // Smagorinsky volume term (-1/yaco)^1.5 in single precision, as the
+        // reference keeps it (real(4) varr, src/MainVar3D.f90:105; Main3D.f90:664)
+        std::vector<float> va(b.N);
+        for (int l = 0; l < b.N; l++) va[l] = (float)std::pow(-1 / yaco[l], 1.5);
+        HIPCHK(hipMemcpy(b.varr, va.data(), sizeof(float) * b.N, hipMemcpyHostToDevice));
+      }
       // wall flags in the reference's face order
       {
         int64_t cnt;

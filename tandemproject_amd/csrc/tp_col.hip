@@ -572,7 +572,9 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
         const int row = idx / L;
         const unsigned o = 8u * (base0 + (unsigned)idx);
         const double x = tile[row * pitch + idx - row * L];
-        st(dst, o, add ? ld_stream(&at(ob, o)) + x : x);
+        const double y = add ? ld_stream(&at(ob, o)) + x : x;
+        st(dst, o, y);
+        if (KIND == 1 && cpy) st(cpy, o, y);
       }
       __syncthreads();   // the tile is read before the next rows overwrite it
     }

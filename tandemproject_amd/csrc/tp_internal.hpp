@@ -66,6 +66,9 @@ struct Block {
   int nsponge;
   double *qa, *qo, *de, *rr, *ss, *xim, *etm, *zem, *p, *yaco, *tv;
   double *fx;                 // [15][N] scratch: gradients of u,v,w,T, then the 15 flux components
+  float *varr;                // [N] LES cases: (-1/yaco)^1.5 in single precision (the
+                              // reference's real(4) work array, src/Main3D.f90:664), formed
+                              // once from the metrics on the host; null otherwise
   double *drva[3], *drvb[3], *cm[3];
   double *recv[3];            // [RCV_SLOTS][6][F]: closure sums per field of a merged launch
   double *lud, *luf;          // [5][nlu]: l1, l2, u1, u2, u3 for deriv / filter

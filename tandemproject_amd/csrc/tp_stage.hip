@@ -345,7 +345,10 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
       if (c.les == 1) {
         double d1 = txx * txx + tyy * tyy + tzz * tzz + (hzz + txy) * (hzz + txy) + (hyy + tzx) * (hyy + tzx) +
                     (hxx + tyz) * (hxx + tyz);
-        double varr = (double)(float)pow(-1 / ya[l], 1.5);   // single-precision work array
+        // the single-precision work array (-1/yaco)^1.5, formed at set-up with
+        // the host's pow: yaco does not change, and a one-ulp difference of the
+        // device pow() could flip its rounding to single precision
+        double varr = (double)b.varr[l];
         double r3 = qv[0] * (c.smago1 * c.smago1) * varr * sqrt(2 * d1);
         r1 = r1 + r3 * ya[l];
         r2 = r2 + ph.tgamm1prndtli * r3 * ya[l];

@@ -1,0 +1,57 @@
+"""The oracle's pins must fail when the oracle is wrong.
+
+Each case plants one plausible mistake (a sign, a constant, a swapped index,
+a wrong weight) in a copy of oracle/tp_oracle.c, builds it into a temporary
+directory and runs the stage pins (tests/test_oracle_stage_pins.py) against
+that copy in a child process: at least one pin must fail.  CPU only.
+"""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "oracle", "tp_oracle.c")
+
+# (what the mistake is, text of the oracle, the mistaken text)
+MUTATIONS = {
+    "energy flux frame term sign": ("r1 = d5 * s1 - pl * (umf[0] * x1", "r1 = d5 * s1 + pl * (umf[0] * x1"),
+    "dilatation factor": ("d5 = two_three * (txx + tyy + tzz);\n          double ntxx = r1 * (2 * txx - d5), ntyy",
+                          "d5 = 0.5 * (txx + tyy + tzz);\n          double ntxx = r1 * (2 * txx - d5), ntyy"),
+    "Prandtl number": ("double r2 = gamm1prndtli * r1, d5;", "double r2 = 1.0 / (gamm1 * 0.72) * r1, d5;"),
+    "stress transposed": ("double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);\n          txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;\n        }",
+                          "double ntxy = r1 * (txy + txy), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);\n          txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;\n        }"),
+    "viscous time-step limit": ("if (c->nviscous == 1) ra1 = half / fv;", "if (c->nviscous == 1) ra1 = 0.4 / fv;"),
+    "ramp period": ("double s = sin(0.05 * PI * (double)(n - (int)sc[ORC_S_NDT]));",
+                    "double s = sin(0.055 * PI * (double)(n - (int)sc[ORC_S_NDT]));"),
+    "incoming wave index": ("if (ra0 * (vv + e.ao) > zero) cha[3] = zero;\n              if (ra0 * (vv - e.ao) > zero) cha[4] = zero;",
+                            "if (ra0 * (vv + e.ao) > zero) cha[4] = zero;\n              if (ra0 * (vv - e.ao) > zero) cha[3] = zero;"),
+    "wall reflection sign": ("cha[3 + ipp] = cha[4 - ipp] + two * ra0", "cha[3 + ipp] = -cha[4 - ipp] + two * ra0"),
+    "junction velocity members": ("for (int m = 1; m <= 3; m++) r[m] = half * (v[va][m] + v[vb][m]);",
+                                  "for (int m = 1; m <= 3; m++) r[m] = 0.25 * (v[0][m] + v[1][m] + v[2][m] + v[3][m]);"),
+    "isentropic exponent": ("vit[2] * vit[2]), hamm1);", "vit[2] * vit[2]), ham);"),
+    "RK stage fraction": ("double dtk = dt / (nkrk - nk + 1);", "double dtk = dt / (nkrk - nk + 2);"),
+    "wall temperature factor": ("double ra0 = ham * hamm1 * c->wtemp;", "double ra0 = hamm1 * c->wtemp;"),
+    "averaging weight": ("double r2 = one / rr[l];\n              rr[N + l] = r2;\n              for (int v = 0; v < 5; v++)\n                qa[(size_t)v * N + l] = r2 * ((rr[l] - one)",
+                         "double r2 = 0.4;\n              rr[N + l] = r2;\n              for (int v = 0; v < 5; v++)\n                qa[(size_t)v * N + l] = r2 * ((rr[l] - one)"),
+    "GCBC face weight": ("cnt[l] = cnt[l] + one; wgt[ll] = res; lst[ll] = l;", "cnt[l] = cnt[l] + one; wgt[ll] = 0.5 * res; lst[ll] = l;"),
+}
+
+
+@pytest.mark.parametrize("name", list(MUTATIONS))
+def test_planted_mistake_fails_a_pin(name, tmp_path):
+    good, bad = MUTATIONS[name]
+    src = open(SRC).read()
+    assert src.count(good) >= 1, "the oracle text this case edits has changed"
+    mut = tmp_path / "tp_oracle.c"
+    mut.write_text(src.replace(good, bad, 1))
+    lib = tmp_path / "liboracle.so"
+    subprocess.check_call(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c11",
+                           "-I" + os.path.join(ROOT, "oracle"), "-shared", "-o", str(lib), str(mut), "-lm"])
+    env = dict(os.environ, ORACLE_LIB=str(lib), OMP_NUM_THREADS="2")
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider",
+                        os.path.join(ROOT, "tests", "test_oracle_stage_pins.py")],
+                       capture_output=True, text=True, env=env, cwd=ROOT, timeout=300)
+    tail = r.stdout.strip().splitlines()[-1] if r.stdout.strip() else r.stderr[-500:]
+    assert r.returncode != 0 and "failed" in tail, f"no pin caught '{name}': {tail}"
