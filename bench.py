@@ -23,93 +23,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from tandemproject_amd import api  # noqa: E402
-from tandemproject_amd import grid as G  # noqa: E402
-from tandemproject_amd import layout as L  # noqa: E402
+from tandemproject_amd import workloads as W  # noqa: E402
 
 METRIC = "FP64 Mpoint-RK-steps/s + %HBM peak; ensemble cases/h and p99 latency"
 HBM_PEAK_GBS = 8000.0
 
 
 def workload(scale=1.0):
-    lx = [int(round(140 * scale)), int(round(240 * scale)), int(round(140 * scale))]
-    ly = [int(round(200 * scale)), int(round(200 * scale))]
-    lz = [95 if scale >= 1.0 else 31]
-    lay = L.h_layout(lx, ly, lz, npcz=[2], nviscous=1)
-    blocks = G.h_grid_blocks(lx, ly, lz, xu=7.0, xd=10.0, yb=7.0, yt=7.0, span=0.2,
-                             thick=0.12, wall_spacing=4e-3)
-    ss = G.subblock_grids(lay, blocks)
-    # inflow: synthetic isotropic turbulence from 4096 seeded random Fourier
-    # modes (von Karman spectrum, Tu = 2.5 %, integral length 0.1 chord),
-    # frozen in a periodic box of 4 chords convected with u_inf, imposed
-    # through the inflow sponge in the band |y| < 2
-    params = dict(reoo=4e5 / 0.3, tempoo=273.0, amach1=0.3, amach2=0.0, amach3=0.0, wtemp=1.0,
-                  cfl=1.0, tmax=1e9, timf=15.0, fltk=0.85, fltkbc=0.85, dto=-1.0, domlen=10.0,
-                  szth1=2.0, szth2=2.0, szxt=4.0, szco=12.0, span=0.2, wlew=0.11, wlea=0.015,
-                  nkrk=4, nviscous=1, nsmf=0, les=0, nextrabc=0, nextgcic=0,
-                  inflow=1, nmodes=4096, seed=2, ntz=64, tgusto=0.0, slit=4.0, tu=0.025, lturb=0.1,
-                  gustw=2.0)
-    return lay, ss, params, dict(lxibk=lx, letbk=ly, lzebk=lz)
-
-
-def aerofoil_params(**over):
-    """Deck of a single-aerofoil case on the bench's H-grid domain (grid
-    x in [-7, 10], |y| <= 7; inflow sponge x < -5)."""
-    p = dict(reoo=4e5 / 0.3, tempoo=273.0, amach1=0.3, amach2=0.0, amach3=0.0, wtemp=1.0,
-             cfl=1.0, tmax=1e9, timf=15.0, fltk=0.85, fltkbc=0.85, dto=-1.0, domlen=10.0,
-             szth1=2.0, szth2=2.0, szxt=4.0, szco=12.0, span=0.2, wlew=0.11, wlea=0.015,
-             nkrk=4, nviscous=1, nsmf=0, les=0, nextrabc=0, nextgcic=0)
-    p.update(over)
-    return p
-
-
-def drawn_case(rng, npts_lo, npts_hi, nz_lo, nz_hi):
-    """Block layout and grid of a single-aerofoil case with about N points, N
-    log-uniform in [npts_lo, npts_hi], nz uniform in [nz_lo, nz_hi]; the
-    in-plane blocks keep the proportions of the 20 M-point shape
-    (lxibk = [a, 1.7a, a], letbk = [1.43a, 1.43a])."""
-    n = float(np.exp(rng.uniform(np.log(npts_lo), np.log(npts_hi))))
-    nz = int(rng.integers(nz_lo, nz_hi + 1))
-    a = max(16, int(round(np.sqrt(n / nz / 10.6))))
-    lx = [a, int(round(1.7 * a)), a]
-    ly = [int(round(1.43 * a))] * 2
-    lz = [nz - 1]
-    lay = L.h_layout(lx, ly, lz, nviscous=1)
-    blocks = G.h_grid_blocks(lx, ly, lz, xu=7.0, xd=10.0, yb=7.0, yt=7.0, span=0.2, thick=0.12,
-                             wall_spacing=4e-3)
-    return lay, G.subblock_grids(lay, blocks)
-
-
-def ensemble_cases(n, seed=4):
-    """Batched-ensemble cases: single-aerofoil gust response, 0.15-2 M points,
-    nz 13-32 (the spanwise-periodic closure spans 13 points, so the 8-12
-    points of the quasi-3D range are raised to 13), AoA U(0, 8 deg),
-    M U(0.2, 0.5), Re_c = 4e5, sinusoidal transverse gust with amplitude
-    U(1 %, 5 %) of |u_inf| and reduced frequency U(0.5, 8)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        lay, ss = drawn_case(rng, 1.5e5, 2.0e6, 13, 32)
-        aoa = np.radians(rng.uniform(0.0, 8.0))
-        mach = rng.uniform(0.2, 0.5)
-        p = aerofoil_params(reoo=4e5 / mach, amach1=mach * np.cos(aoa), amach2=mach * np.sin(aoa),
-                            inflow=2, ntz=32, tu=rng.uniform(0.01, 0.05), gustk=rng.uniform(0.5, 8.0),
-                            gustw=2.0)
-        out.append((lay, ss, p))
-    return out
-
-
-def served_cases(n, seed=5):
-    """Served turbulence-response cases: single aerofoil, 0.2-20 M points, nz
-    16-96, M = 0.3, Re_c = 4e5, turbulent inflow of 1024 seeded Fourier modes
-    with Tu U(1 %, 5 %) and integral length U(0.05, 0.2) chords."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(n):
-        lay, ss = drawn_case(rng, 2.0e5, 2.0e7, 16, 96)
-        p = aerofoil_params(inflow=1, nmodes=1024, seed=100 + i, ntz=32, tgusto=0.0, slit=4.0,
-                            tu=rng.uniform(0.01, 0.05), lturb=rng.uniform(0.05, 0.2), gustw=2.0)
-        out.append((lay, ss, p))
-    return out
+    return W.naca0012_20m(scale)
 
 
 def run_plain(lay, ss, p, nsteps):
@@ -152,7 +73,7 @@ def ensemble_bench(ncase, nsteps, nplain):
     measured on a seeded sample of the same cases).  Both are timed from the
     cases' grids on the host to their final states on the host; grid
     generation is outside."""
-    cases = ensemble_cases(ncase)
+    cases = W.ensemble_cases(ncase)
     pts = [lay.npoints() for lay, _, _ in cases]
     log(f"ensemble: {ncase} cases, {sum(pts) / 1e6:.1f} M points, {nsteps} steps each")
     srv = api.Server(budget_bytes=0, max_active=ncase, graph=True)
@@ -203,7 +124,7 @@ def serving_bench(nreq, nsteps, load, fill):
     to solution (arrival to final state on the host) and cases/h, against
     the plain path serving the same stream one case at a time (first come,
     first served, each case's plain time measured alone)."""
-    cases = served_cases(nreq)
+    cases = W.served_cases(nreq)
     pts = [lay.npoints() for lay, _, _ in cases]
     log(f"serving: {nreq} requests, {sum(pts) / 1e6:.1f} M points, {nsteps} steps each")
     # service time of each request alone on the plain path

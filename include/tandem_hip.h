@@ -101,6 +101,14 @@ int32_t tp_add_case(tp_context *ctx, int32_t nblocks, const int32_t *dims, const
                     const int32_t *ncd, const int32_t *nsz, int32_t njct, const int32_t *jct,
                     const int32_t *jvel, const int32_t *iparams, const double *dparams,
                     int32_t *case_id);
+/* the checks tp_add_case makes of a case, without a context (and so
+ * without a GPU): boundary codes, neighbours and matching faces, line
+ * lengths the closures need (src/Main3D.f90:92-178), junction members, and
+ * block sizes -- a block holds fewer than 2^29 points (the line solves
+ * address a field with 32-bit byte offsets).  No counterpart in the
+ * reference, which reads its deck and aborts on a bad one. */
+int32_t tp_check_case(int32_t nblocks, const int32_t *dims, const int32_t *nbc, const int32_t *ncd,
+                      int32_t njct, const int32_t *jct, const int32_t *jvel, const int32_t *iparams);
 /* allocate device state of the cases added so far, compute the scheme
  * coefficients and LU factors (src/Main3D.f90:293-332), build the launch
  * tables.  A context may be finalized with no case (cases then come through
@@ -135,11 +143,59 @@ int32_t tp_deriv(tp_context *ctx, int32_t nn, int32_t nz, int32_t m);
 /* filte(nn,nz): src/Subroutines3D.f90:145-206 (rr(:,nz) += filter increment) */
 int32_t tp_filte(tp_context *ctx, int32_t nn, int32_t nz);
 
+/* deriv(nn,nz,m) on one field held by the caller in device memory:
+ * dst = d src / d index_nn over the block of (lxi+1) x (let+1) x (lze+1)
+ * points in the reference's layout (xi fastest), the pentadiagonal compact
+ * scheme of src/Subroutines3D.f90:84-141 with the closures of the line ends
+ * nbc2[0], nbc2[1]: 10, 20, 25 or 30 (one-sided rows) or both 45 (periodic,
+ * the closure sums of mpigo, src/Subroutines3D.f90:16-80, with n45 = 0,
+ * formed from the other end of each line).  src and dst must not overlap.
+ * Lines of at most 512 points (longer ones: a context and tp_deriv).  The
+ * solve is enqueued on stream (a hipStream_t; NULL: the null stream); the
+ * library allocates its tables in device memory for the call and returns
+ * once the solve has run. */
+int32_t tp_field_deriv(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc2, int32_t nn,
+                       const double *src, double *dst, void *stream);
+
 /* ---- coarse entry points ---- */
 /* filtering of qa and qo = qa: src/Main3D.f90:554-561 */
 int32_t tp_filter_step(tp_context *ctx);
 /* one Runge-Kutta stage nk: src/Main3D.f90:567-958 */
 int32_t tp_stage(tp_context *ctx, int32_t nk);
+
+/* ---- the sections of a stage as drop-ins ----
+ * tp_stage(ctx, nk) is these nine calls in this order; called one by one they
+ * compute the same state bit for bit (the stresses then pass through tv at
+ * every point, which the fused stage keeps in registers).  Each acts on all
+ * sub-blocks of all cases, like the inline blocks of src/Main3D.f90 it
+ * replaces on every MPI rank. */
+/* stage start: frame velocity (movef, src/Main3D.f90:571-572), primitives
+ * de(:,1:5), p, ss(:,1) = viscosity (:576-583), and at nk = 1 the time step
+ * size (:587-622) */
+int32_t tp_primitives(tp_context *ctx, int32_t nk);
+/* viscous stresses and heat fluxes into tv(:,1:9) = txx tyy tzz txy tyz tzx
+ * hxx hyy hzz at every point (src/Main3D.f90:626-693; gradients of de(:,2:5)
+ * from the compact derivative); nothing for inviscid cases */
+int32_t tp_viscous(tp_context *ctx);
+/* at nk = 1 the surface sample (src/Main3D.f90:695-706), then the
+ * metric-weighted Euler and viscous fluxes (from tv) and their compact
+ * derivatives into de(:,1:5), the end values kept in drva (:708-768) */
+int32_t tp_flux_divergence(tp_context *ctx, int32_t nk);
+/* sponge relaxation towards the rest state or the inflow target
+ * (spongego, src/SingleTurbulence3D.f90:332-374) */
+int32_t tp_sponge(tp_context *ctx);
+/* characteristic boundary and interface conditions on de: preparation,
+ * interface exchange and application (src/Main3D.f90:778-869) */
+int32_t tp_gcbc(tp_context *ctx);
+/* frame velocity for the update and qa = qo - dtk yaco de
+ * (src/Main3D.f90:873-881) */
+int32_t tp_rk_update(tp_context *ctx, int32_t nk);
+/* wall temperature and velocity (src/Main3D.f90:884-895) */
+int32_t tp_wall(tp_context *ctx);
+/* corner junctions (junction, src/SingleTurbulence3D.f90:378-443) */
+int32_t tp_junction(tp_context *ctx);
+/* interface averaging of qa (src/Main3D.f90:897-952) */
+int32_t tp_interface_avg(tp_context *ctx);
 /* nsteps whole time steps (filter + nkrk stages + n, t advance): src/Main3D.f90:552-965 */
 int32_t tp_step(tp_context *ctx, int32_t nsteps);
 /* the same, replaying one step captured in a HIP graph */

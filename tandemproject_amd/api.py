@@ -37,7 +37,13 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_server_stats", "tp_server_set_fill", "tp_stats_start", "tp_stats_forces",
            "tp_stats_wall_count", "tp_stats_wall", "tp_set_parents", "tp_get_parents",
            "tp_case_dims", "tp_case_params", "tp_write_restart", "tp_read_restart", "tp_write_grid",
-           "tp_read_grid", "tp_write_plot3d", "tp_get_grid"]
+           "tp_read_grid", "tp_write_plot3d", "tp_get_grid", "tp_check_case",
+           "tp_primitives", "tp_viscous", "tp_flux_divergence", "tp_sponge", "tp_gcbc", "tp_rk_update",
+           "tp_wall", "tp_junction", "tp_interface_avg", "tp_field_deriv"]
+# the sections of a stage, in the order tp_stage runs them (name, takes nk)
+STAGE_PARTS = [("primitives", True), ("viscous", False), ("flux_divergence", True), ("sponge", False),
+               ("gcbc", False), ("rk_update", True), ("wall", False), ("junction", False),
+               ("interface_avg", False)]
 
 _lib = None
 
@@ -68,7 +74,12 @@ def load(path=LIB_PATH):
         "tp_download": [P, i32, i32, i32, pd, i64], "tp_get_scalars": [P, i32, pd],
         "tp_set_scalars": [P, i32, pd], "tp_mpigo": [P, i32, i32, i32],
         "tp_deriv": [P, i32, i32, i32], "tp_filte": [P, i32, i32], "tp_filter_step": [P],
-        "tp_stage": [P, i32], "tp_step": [P, i32], "tp_step_graph": [P, i32], "tp_sync": [P],
+        "tp_stage": [P, i32], "tp_step": [P, i32],
+        "tp_check_case": [i32, pi32, pi32, pi32, i32, pi32, pi32, pi32],
+        "tp_primitives": [P, i32], "tp_viscous": [P], "tp_flux_divergence": [P, i32], "tp_sponge": [P],
+        "tp_gcbc": [P], "tp_rk_update": [P, i32], "tp_wall": [P], "tp_junction": [P],
+        "tp_interface_avg": [P],
+        "tp_field_deriv": [i32, i32, i32, pi32, i32, P, P, P], "tp_step_graph": [P, i32], "tp_sync": [P],
         "tp_profile": [P, i32], "tp_profile_classes": [P, i32],
         "tp_profile_read": [P, i32, pd, ctypes.POINTER(i64), pd, i32],
         "tp_info": [P, ctypes.POINTER(i64)],
@@ -191,6 +202,20 @@ class PinnedBuffer:
 
     def __del__(self):
         self.free()
+
+
+def field_deriv(dims3, nbc2, nn, src_ptr, dst_ptr, stream=None):
+    """tp_field_deriv on device addresses (e.g. torch tensors' data_ptr())."""
+    nbc = np.ascontiguousarray(nbc2, np.int32)
+    _check(load().tp_field_deriv(int(dims3[0]), int(dims3[1]), int(dims3[2]), _ip(nbc), int(nn),
+                                 ctypes.c_void_p(src_ptr), ctypes.c_void_p(dst_ptr), ctypes.c_void_p(stream)))
+
+
+def check_case(layout, params):
+    """tp_check_case: the checks tp_add_case makes, without a context (raises TPError)."""
+    arrs, ipar, _ = case_args(layout, params)
+    d, b, n, _, j, v = arrs
+    _check(load().tp_check_case(layout.nb, _ip(d), _ip(b), _ip(n), layout.njct, _ip(j), _ip(v), _ip(ipar)))
 
 
 class Solver:
@@ -416,6 +441,13 @@ class Solver:
 
     def stage(self, nk):
         _check(load().tp_stage(self.h, nk))
+
+    def stage_part(self, nk, part):
+        """One section of stage nk (a name of STAGE_PARTS): tp_primitives,
+        tp_viscous, ... tp_interface_avg."""
+        takes_nk = dict(STAGE_PARTS)[part]
+        fn = getattr(load(), "tp_" + part)
+        _check(fn(self.h, nk) if takes_nk else fn(self.h))
 
     def step(self, n=1, graph=False):
         if graph:

@@ -790,7 +790,12 @@ void enqueue_filter_step(tp_context *c) {
   run_line(c, 1, 2, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1, F_QO, 0);
 }
 
-void enqueue_stage(tp_context *c, int nk) {
+// ---- the sections of a stage (src/Main3D.f90:567-958), shared by the fused
+// stage and the per-section drop-ins ----
+
+// stage start: frame velocity and inflow weights (:571-572), primitives
+// (:576-583), time step at the first stage (:587-622)
+void enqueue_begin(tp_context *c, int nk) {
   const int ncase = (int)c->linked.size();
   launch_sc_stage(c->d_coef, c->d_sc, ncase, nk, c->stream);
   StageArgs a = base_args(c);
@@ -800,52 +805,102 @@ void enqueue_stage(tp_context *c, int nk) {
     run_points(c, PK_DTRED, a, K_POINT, 0.0);
     launch_sc_dt(c->d_coef, c->d_sc, c->d_part, c->d_wr, ncase, 1, c->stream);
   }
-  if (c->any_visc) {
-    // src/Main3D.f90:627-651: gradients of u, v, w, T (de(:,2:5)) along the
-    // three index directions into fx(:,3f+d) -- the four fields in one launch
-    // per direction
-    const bool dir = direct_ok(c, 0);
-    if (!dir) run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1);
-    run_line(c, 0, 2, F_DE, 1, F_FX, 2, 0, 0, 4, 1, 3, -1, 0, dir);
-    run_line(c, 0, 1, F_DE, 1, F_FX, 1, 0, 0, 4, 1, 3, -1, 0, dir);
-    run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3, -1, 0, dir);
-  }
-  // src/Main3D.f90:653-768: stresses and heat fluxes, then all 15 flux
-  // components, in one pass over the gradients; then
-  // de(:,m) = d/dxi F + d/deta G + d/dzeta H, summed in the reference's order
-  {
-    StageArgs f = a;
-    f.opt = c->keep_stress ? 1 : 0;
-    run_points(c, PK_FLUXALL, f, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 14 + 15));
-  }
-  if (nk == 1 && c->L.surf.n) {
-    // surface forces and pressure statistics, src/Main3D.f90:695-706 (the
-    // kernels return at once for cases that do not sample this step)
-    run_faces(c, c->L.surf, FK_SURF_SAMPLE, a, K_MISC);
-    launch_surf_reduce(c->d_coef, c->d_sc, c->d_wpart, c->d_wwr, ncase, c->stream);
-  }
-  // the five fields m in one launch per step: flux components fx(:,3(m-1)+d)
+}
+
+// src/Main3D.f90:627-651: gradients of u, v, w, T (de(:,2:5)) along the
+// three index directions into fx(:,3f+d) -- the four fields in one launch per
+// direction
+void enqueue_gradients(tp_context *c) {
+  if (!c->any_visc) return;
+  const bool dir = direct_ok(c, 0);
+  if (!dir) run_exchange(c, 0, 1, 0, F_DE, 1, 4, 1);
+  run_line(c, 0, 2, F_DE, 1, F_FX, 2, 0, 0, 4, 1, 3, -1, 0, dir);
+  run_line(c, 0, 1, F_DE, 1, F_FX, 1, 0, 0, 4, 1, 3, -1, 0, dir);
+  run_line(c, 0, 0, F_DE, 1, F_FX, 0, 0, 0, 4, 1, 3, -1, 0, dir);
+}
+
+// surface forces and pressure statistics, src/Main3D.f90:695-706 (the
+// kernels return at once for cases that do not sample this step)
+void enqueue_surface(tp_context *c, int nk) {
+  if (nk != 1 || !c->L.surf.n) return;
+  StageArgs a = base_args(c);
+  a.nk = nk;
+  run_faces(c, c->L.surf, FK_SURF_SAMPLE, a, K_MISC);
+  launch_surf_reduce(c->d_coef, c->d_sc, c->d_wpart, c->d_wwr, (int)c->linked.size(), c->stream);
+}
+
+// de(:,m) = d/dxi F + d/deta G + d/dzeta H of the flux components
+// fx(:,3(m-1)+d), the five fields m in one launch per direction, summed in
+// the reference's order (src/Main3D.f90:721-768)
+void enqueue_flux_lines(tp_context *c) {
   const bool dirf = direct_ok(c, 0);
   if (!dirf) run_exchange(c, 0, 0, 0, F_FX, 0, 5, 3);
   run_line(c, 0, 0, F_FX, 0, F_DE, 0, 1, 0, 5, 3, 1, -1, 0, dirf);
   run_line(c, 0, 1, F_FX, 1, F_DE, 0, 1, 1, 5, 3, 1, -1, 0, dirf);
   run_line(c, 0, 2, F_FX, 2, F_DE, 0, 1, 1, 5, 3, 1, -1, 0, dirf);
-  if (c->L.sponge.n) {
-    Timed t(c, K_MISC, 0.0);
-    StageArgs s = a;
-    s.work = c->L.sponge.d;
-    launch_point(PK_SPONGE, c->L.sponge.n, c->stream, s);
-  }
+}
+
+// sponge with the inflow target, src/SingleTurbulence3D.f90:332-374
+void enqueue_sponge(tp_context *c) {
+  if (!c->L.sponge.n) return;
+  Timed t(c, K_MISC, 0.0);
+  StageArgs s = base_args(c);
+  s.work = c->L.sponge.d;
+  launch_point(PK_SPONGE, c->L.sponge.n, c->stream, s);
+}
+
+// characteristic boundary and interface conditions, src/Main3D.f90:778-869
+void enqueue_gcbc(tp_context *c) {
+  const StageArgs a = base_args(c);
   run_faces(c, c->L.gprep, FK_GCBC_PREP, a, K_GCBC);
   run_faces(c, c->L.gcopy, FK_COPY, a, K_GCBC);
   for (int d = 0; d < 3; d++) run_faces(c, c->L.gapply[d], d == 0 ? FK_GCBC_APPLY_XI : FK_GCBC_APPLY, a, K_GCBC);
-  launch_sc_rk(c->d_coef, c->d_sc, ncase, nk, c->stream);
+}
+
+// frame velocity for the update and the RK update, src/Main3D.f90:873-881
+void enqueue_rk(tp_context *c, int nk) {
+  launch_sc_rk(c->d_coef, c->d_sc, (int)c->linked.size(), nk, c->stream);
+  StageArgs a = base_args(c);
+  a.nk = nk;
   run_points(c, PK_RK, a, K_RK, 128.0);
-  run_faces(c, c->L.wall, FK_WALL, a, K_GCBC);
+}
+
+// wall temperature and velocity, src/Main3D.f90:884-895
+void enqueue_wall(tp_context *c) { run_faces(c, c->L.wall, FK_WALL, base_args(c), K_GCBC); }
+
+// corner junctions, src/SingleTurbulence3D.f90:378-443
+void enqueue_junction(tp_context *c) {
   if (c->njct) launch_junction(c->d_blocks, c->d_jct, c->d_jvel, c->njct, c->maxk, c->stream);
+}
+
+// interface averaging, src/Main3D.f90:897-952
+void enqueue_avg(tp_context *c) {
+  const StageArgs a = base_args(c);
   run_faces(c, c->L.acap, FK_AVG_CAP, a, K_GCBC);
   run_faces(c, c->L.acap, FK_COPY, a, K_GCBC);
   for (int d = 0; d < 3; d++) run_faces(c, c->L.aapply[d], FK_AVG_APPLY, a, K_GCBC);
+}
+
+// one stage, src/Main3D.f90:567-958: the stresses and heat fluxes stay in
+// registers of the pass that forms the fluxes (tv is written only where it
+// is read, or everywhere with TP_OPT_STRESSES)
+void enqueue_stage(tp_context *c, int nk) {
+  enqueue_begin(c, nk);
+  enqueue_gradients(c);
+  {
+    StageArgs f = base_args(c);
+    f.nk = nk;
+    f.opt = c->keep_stress ? 1 : 0;
+    run_points(c, PK_FLUXALL, f, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 14 + 15));
+  }
+  enqueue_surface(c, nk);
+  enqueue_flux_lines(c);
+  enqueue_sponge(c);
+  enqueue_gcbc(c);
+  enqueue_rk(c, nk);
+  enqueue_wall(c);
+  enqueue_junction(c);
+  enqueue_avg(c);
 }
 
 void enqueue_step(tp_context *c) {
@@ -1457,14 +1512,10 @@ int32_t tp_destroy(tp_context *c) {
   return 0;
 }
 
-int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_t *nbc, const int32_t *ncd,
-                    const int32_t *nsz, int32_t njct, const int32_t *jct, const int32_t *jvel,
-                    const int32_t *iparams, const double *dparams, int32_t *case_id) {
-  if (!c) return fail(TP_E_ARG, "null context");
-  if (nb <= 0 || !dims || !nbc || !ncd || !nsz || !iparams || !dparams)
-    return fail(TP_E_ARG, "tp_add_case: missing arrays");
-  if ((int)c->cases.size() >= MAX_CASE) return fail(TP_E_SIZE, "too many cases");
-  if (njct > 0 && (!jct || !jvel)) return fail(TP_E_ARG, "junction arrays missing");
+int32_t tp_check_case(int32_t nb, const int32_t *dims, const int32_t *nbc, const int32_t *ncd, int32_t njct,
+                      const int32_t *jct, const int32_t *jvel, const int32_t *iparams) {
+  if (nb <= 0 || !dims || !nbc || !ncd || !iparams) return fail(TP_E_ARG, "tp_check_case: missing arrays");
+  if (njct < 0 || (njct > 0 && (!jct || !jvel))) return fail(TP_E_ARG, "junction arrays missing");
   for (int j = 0; j < njct; j++)
     for (int q = 0; q < 4; q++) {
       const int lb = jct[12 * j + 3 * q];
@@ -1472,20 +1523,6 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
       if (dims[3 * lb + 2] != dims[3 * jct[12 * j] + 2])
         return fail(TP_E_LAYOUT, "junction members differ in zeta size");
     }
-  HostCase h;
-  h.nb = nb;
-  h.njct = njct;
-  h.dims.assign(dims, dims + 3 * nb);
-  h.nbc.assign(nbc, nbc + 6 * nb);
-  h.ncd.assign(ncd, ncd + 6 * nb);
-  h.nsz.assign(nsz, nsz + 6 * nb);
-  if (njct > 0) {
-    if (!jct || !jvel) return fail(TP_E_ARG, "junction arrays missing");
-    h.jct.assign(jct, jct + 12 * njct);
-    h.jvel.assign(jvel, jvel + 2 * njct);
-  }
-  std::memcpy(h.ip, iparams, sizeof(h.ip));
-  std::memcpy(h.dp, dparams, sizeof(h.dp));
   // block size: the line solves address a field with 32-bit byte offsets
   // from its base, so a block holds fewer than 2^29 points
   for (int b = 0; b < nb; b++) {
@@ -1532,6 +1569,32 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
                                      std::to_string(L) + " points; its faces need >= " + std::to_string(need));
     }
   }
+  if (iparams[TP_I_NKRK] < 1) return fail(TP_E_ARG, "nkrk must be >= 1");
+  return 0;
+}
+
+int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_t *nbc, const int32_t *ncd,
+                    const int32_t *nsz, int32_t njct, const int32_t *jct, const int32_t *jvel,
+                    const int32_t *iparams, const double *dparams, int32_t *case_id) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  if (nb <= 0 || !dims || !nbc || !ncd || !nsz || !iparams || !dparams)
+    return fail(TP_E_ARG, "tp_add_case: missing arrays");
+  if ((int)c->cases.size() >= MAX_CASE) return fail(TP_E_SIZE, "too many cases");
+  if (int rc = tp_check_case(nb, dims, nbc, ncd, njct, jct, jvel, iparams)) return rc;
+  HostCase h;
+  h.nb = nb;
+  h.njct = njct;
+  h.dims.assign(dims, dims + 3 * nb);
+  h.nbc.assign(nbc, nbc + 6 * nb);
+  h.ncd.assign(ncd, ncd + 6 * nb);
+  h.nsz.assign(nsz, nsz + 6 * nb);
+  if (njct > 0) {
+    if (!jct || !jvel) return fail(TP_E_ARG, "junction arrays missing");
+    h.jct.assign(jct, jct + 12 * njct);
+    h.jvel.assign(jvel, jvel + 2 * njct);
+  }
+  std::memcpy(h.ip, iparams, sizeof(h.ip));
+  std::memcpy(h.dp, dparams, sizeof(h.dp));
   if (c->nkrk && c->nkrk != iparams[TP_I_NKRK])
     return fail(TP_E_ARG, "all cases of a context must use the same number of RK stages");
   if (iparams[TP_I_NKRK] < 1) return fail(TP_E_ARG, "nkrk must be >= 1");
@@ -1896,6 +1959,80 @@ int32_t tp_stage(tp_context *c, int32_t nk) {
   enqueue_stage(c, nk);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// ---- the sections of a stage as drop-ins (all sub-blocks of all cases) ----
+namespace {
+int stage_ok(tp_context *c, int nk) {
+  if (int rc = check_ready(c)) return rc;
+  if (nk < 1 || nk > std::max(c->nkrk, 1)) return fail(TP_E_ARG, "bad stage " + std::to_string(nk));
+  return 0;
+}
+int launched() {
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+int32_t tp_primitives(tp_context *c, int32_t nk) {
+  if (int rc = stage_ok(c, nk)) return rc;
+  if (!c->linked.empty()) enqueue_begin(c, nk);
+  return launched();
+}
+
+int32_t tp_viscous(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  if (c->linked.empty() || !c->any_visc) return 0;
+  enqueue_gradients(c);
+  run_points(c, PK_STRESS, base_args(c), K_FLUX, 8.0 * (27 + 9));
+  return launched();
+}
+
+int32_t tp_flux_divergence(tp_context *c, int32_t nk) {
+  if (int rc = stage_ok(c, nk)) return rc;
+  if (c->linked.empty()) return 0;
+  enqueue_surface(c, nk);
+  StageArgs f = base_args(c);
+  f.nk = nk;
+  run_points(c, PK_FLUX_TV, f, K_FLUX, 8.0 * (c->any_visc ? 27 + 15 : 14 + 15));
+  enqueue_flux_lines(c);
+  return launched();
+}
+
+int32_t tp_sponge(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  if (!c->linked.empty()) enqueue_sponge(c);
+  return launched();
+}
+
+int32_t tp_gcbc(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  if (!c->linked.empty()) enqueue_gcbc(c);
+  return launched();
+}
+
+int32_t tp_rk_update(tp_context *c, int32_t nk) {
+  if (int rc = stage_ok(c, nk)) return rc;
+  if (!c->linked.empty()) enqueue_rk(c, nk);
+  return launched();
+}
+
+int32_t tp_wall(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  if (!c->linked.empty()) enqueue_wall(c);
+  return launched();
+}
+
+int32_t tp_junction(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  if (!c->linked.empty()) enqueue_junction(c);
+  return launched();
+}
+
+int32_t tp_interface_avg(tp_context *c) {
+  if (int rc = check_ready(c)) return rc;
+  if (!c->linked.empty()) enqueue_avg(c);
+  return launched();
 }
 
 int32_t tp_step(tp_context *c, int32_t nsteps) {
@@ -2326,6 +2463,141 @@ int32_t tp_case_params(tp_context *c, int32_t cs, int32_t *iparams, double *dpar
   if (ci < 0) return fail(TP_E_ARG, "bad case");
   if (iparams) std::memcpy(iparams, c->cases[ci].ip, sizeof(c->cases[ci].ip));
   if (dparams) std::memcpy(dparams, c->cases[ci].dp, sizeof(c->cases[ci].dp));
+  return 0;
+}
+
+// ---- the derivative of one field held by the caller ----
+namespace {
+
+// device bytes of the work area of tp_field_deriv: block view, coefficients,
+// chunked-solver table, work list (each 256-byte aligned)
+struct FieldWs {
+  size_t blk, coef, tab, work, total;
+};
+FieldWs field_ws(int L, int P, int nwork) {
+  FieldWs w;
+  size_t off = 0;
+  auto take = [&](size_t n) { size_t o = off; off += (n + 255) / 256 * 256; return o; };
+  w.blk = take(sizeof(Block));
+  w.coef = take(sizeof(CaseCoef));
+  w.tab = take(sizeof(double) * col_table_doubles(L, P));
+  w.work = take(sizeof(Work) * (size_t)nwork);
+  w.total = off;
+  return w;
+}
+
+// everything tp_field_deriv launches with, built on the host: the block
+// view of the caller's arrays, the derivative's coefficients, the table of
+// P chunks and the work list; the launch needs the work area's address
+struct FieldCall {
+  int d = 0, L = 0, P = 0, C = 0, nwork = 0;
+  bool periodic = false;
+  Block blk;
+  CaseCoef coef;
+  std::vector<double> tab;
+  std::vector<Work> work;
+  FieldWs ws;
+};
+
+int field_prepare(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc2, int32_t nn, const double *src,
+                  double *dst, FieldCall &f) {
+  if (nn < 1 || nn > 3 || !nbc2 || !src || !dst) return fail(TP_E_ARG, "tp_field_deriv: bad arguments");
+  const int d = nn - 1;
+  const int dims[3] = {lxi, let, lze};
+  int64_t np = 1;
+  for (int e = 0; e < 3; e++) {
+    if (dims[e] < 1) return fail(TP_E_LAYOUT, "tp_field_deriv: fewer than 2 points in a direction");
+    np *= (int64_t)dims[e] + 1;
+  }
+  if (8 * np >= (int64_t)1 << 32) return fail(TP_E_SIZE, "tp_field_deriv: a block holds fewer than 2^29 points");
+  for (int s = 0; s < 2; s++) {
+    const int c = nbc2[s];
+    if (c != 10 && c != 20 && c != 25 && c != 30 && c != 45)
+      return fail(TP_E_ARG, "tp_field_deriv: end codes are 10, 20, 25, 30 (one-sided) or 45 (periodic)");
+  }
+  if ((nbc2[0] == 45) != (nbc2[1] == 45)) return fail(TP_E_ARG, "tp_field_deriv: a periodic line has both ends 45");
+  f.d = d;
+  f.L = dims[d] + 1;
+  f.periodic = nbc2[0] == 45;
+  if (f.L < (f.periodic ? LMD + 2 : 7))
+    return fail(TP_E_LAYOUT, "tp_field_deriv: lines of " + std::to_string(f.L) + " points are too short");
+  f.P = col_default(f.L);
+  if (!f.P) return fail(TP_E_SIZE, "tp_field_deriv: lines longer than 512 points need a context (tp_deriv)");
+  f.C = col_capacity((f.L + f.P - 1) / f.P);
+  std::memset(&f.blk, 0, sizeof(Block));
+  Block &b = f.blk;
+  b.nx = lxi + 1; b.ny = let + 1; b.nz = lze + 1;
+  b.N = (int)np;
+  b.L[0] = b.nx; b.L[1] = b.ny; b.L[2] = b.nz;
+  b.F[0] = b.ny * b.nz; b.F[1] = b.nz * b.nx; b.F[2] = b.nx * b.ny;
+  for (int e = 0; e < 3; e++)
+    for (int s = 0; s < 2; s++) { b.nbc[e][s] = 10; b.ncd[e][s] = -1; }
+  for (int s = 0; s < 2; s++) {
+    b.nbc[d][s] = nbc2[s];
+    b.ndf[d][s] = f.periodic ? 1 : 0;
+    b.ncd[d][s] = f.periodic ? 0 : -1;
+  }
+  b.cs = 0;
+  b.rr = const_cast<double *>(src);   // read as rr(:,1) ...
+  b.de = dst;                         // ... written as de(:,1)
+  // coefficients: the derivative's closure rows, src/Main3D.f90:71-73, 309-312
+  std::memset(&f.coef, 0, sizeof(CaseCoef));
+  std::memcpy(f.coef.abc, kAbc, sizeof(kAbc));
+  closure_rows(0.85 * kPi, f.coef);
+  // LU factors of this direction (src/Main3D.f90:316-332) and their chunk table
+  double albed[2][3][5];
+  const double inner[3][5] = {{0, 0, 1, kAlpha, kBeta}, {0, kAlpha, 1, kAlpha, kBeta}, {kBeta, kAlpha, 1, kAlpha, kBeta}};
+  std::memcpy(albed[0], kBnd, sizeof(kBnd));
+  std::memcpy(albed[1], inner, sizeof(inner));
+  std::vector<double> lu(5 * (size_t)f.L);
+  double *l1 = &lu[0], *l2 = l1 + f.L, *u1 = l2 + f.L, *u2 = u1 + f.L, *u3 = u2 + f.L;
+  const int end = f.periodic ? 1 : 0;
+  band_lu(f.L, albed[end], albed[end], kAlpha, kBeta, l1, l2, u1, u2, u3);
+  f.tab = col_tables(f.L, f.P, l1, l2, u1, u2, u3);
+  const int64_t nlines = np / f.L;
+  f.work.clear();
+  for (int64_t q = 0; q < nlines; q += 64) f.work.push_back({0, (int)q, (int)std::min<int64_t>(64, nlines - q), d});
+  f.nwork = (int)f.work.size();
+  f.ws = field_ws(f.L, f.P, f.nwork);
+  return 0;
+}
+
+// copy the call's tables into the work area ws and launch the solve on st
+int field_launch(FieldCall &f, char *ws, hipStream_t st) {
+  f.blk.col[f.d][0] = reinterpret_cast<const double *>(ws + f.ws.tab);
+  HIPCHK(hipMemcpyAsync(ws + f.ws.blk, &f.blk, sizeof(Block), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ws + f.ws.coef, &f.coef, sizeof(CaseCoef), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ws + f.ws.tab, f.tab.data(), sizeof(double) * f.tab.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ws + f.ws.work, f.work.data(), sizeof(Work) * f.work.size(), hipMemcpyHostToDevice, st));
+  LineArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.blocks = reinterpret_cast<const Block *>(ws + f.ws.blk);
+  a.work = reinterpret_cast<const Work *>(ws + f.ws.work);
+  a.coef = reinterpret_cast<const CaseCoef *>(ws + f.ws.coef);
+  a.in_arr = F_RR; a.in_col = 0; a.out_arr = F_DE; a.out_col = 0;
+  a.m = 0; a.acc = 0; a.nlp = 65; a.copy_arr = -1;
+  a.direct = f.periodic ? 1 : 0;   // periodic: closure sums from the other end of the same lines
+  a.n45 = 0;
+  launch_col(f.d, 0, f.C, f.P, f.nwork, 1, st, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+int32_t tp_field_deriv(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc2, int32_t nn, const double *src,
+                       double *dst, void *stream) {
+  FieldCall f;
+  if (int rc = field_prepare(lxi, let, lze, nbc2, nn, src, dst, f)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  char *ws = nullptr;
+  HIPCHK(hipMalloc(&ws, f.ws.total));
+  int rc = field_launch(f, ws, st);
+  // the work area lives until the solve has run
+  const hipError_t e = hipStreamSynchronize(st);
+  (void)hipFree(ws);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(TP_E_HIP, std::string("tp_field_deriv: ") + hipGetErrorString(e));
   return 0;
 }
 

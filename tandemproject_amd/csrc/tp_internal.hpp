@@ -278,7 +278,7 @@ void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, c
 int set_error(int code, const std::string &msg);
 
 enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPYF = 9, PK_INIT, PK_METRICS,
-                   PK_FLUXALL = 14 };
+                   PK_FLUXALL = 14, PK_STRESS = 15, PK_FLUX_TV = 16 };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,
                   FK_SURF_SETUP, FK_SURF_SAMPLE, FK_GCBC_APPLY_XI };
 

@@ -297,6 +297,14 @@ __device__ inline void flux_put(double *fx, int N, int l, int m, double r1, doub
   st_stream(&fx[(size_t)(3 * (m - 1) + 2) * N + l], r3);
 }
 
+// MODE: FUSED forms stresses and fluxes from the gradients in one pass (the
+// stage's path); STRESS forms only the stresses and heat fluxes and writes
+// them to tv at every point (the viscous drop-in, Main3D.f90:626-693);
+// FROM_TV forms the fluxes from the stresses in tv (the flux drop-in,
+// :708-768).  STRESS followed by FROM_TV gives the fluxes of FUSED bit for bit.
+enum FluxMode { FM_FUSED = 0, FM_STRESS = 1, FM_FROM_TV = 2 };
+
+template <int MODE>
 __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
@@ -318,16 +326,20 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
     // the 12 gradients loaded with the rest, before the primitives are
     // formed, so that all of the point's loads are in flight at once
     double gr[4][3];
-    if (visc) {
+    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (visc && MODE != FM_FROM_TV) {
 #pragma unroll
       for (int f = 0; f < 4; f++)
 #pragma unroll
         for (int m = 0; m < 3; m++) gr[f][m] = ld_stream(&fx[(size_t)(3 * f + m) * N + l]);
     }
+    if (visc && MODE == FM_FROM_TV) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) t[k] = ld_stream(&tv[(size_t)k * N + l]);
+    }
     const Prim pr = primitives(c, ph, qv[0], qv[1], qv[2], qv[3], qv[4]);
     const double u = pr.d2, v = pr.d3, ww = pr.d4;
-    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (visc) {
+    if (visc && MODE != FM_FROM_TV) {
       double cb[4][3];
 #pragma unroll
       for (int f = 0; f < 4; f++) {
@@ -367,11 +379,12 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
       hyy = r2 * sy + u * txy + v * tyy + ww * tyz;
       hzz = r2 * sz + u * tzx + v * tyz + ww * tzz;
       t[0] = txx; t[1] = tyy; t[2] = tzz; t[3] = txy; t[4] = tyz; t[5] = tzx; t[6] = hxx; t[7] = hyy; t[8] = hzz;
-      if (a.opt || (wall_row >= 0 && (l / b.nx) % b.ny == wall_row)) {
+      if (MODE == FM_STRESS || a.opt || (wall_row >= 0 && (l / b.nx) % b.ny == wall_row)) {
 #pragma unroll
         for (int k = 0; k < 9; k++) tv[(size_t)k * N + l] = t[k];
       }
     }
+    if (MODE == FM_STRESS) continue;
     const double q1 = u + u0, q2 = v + u1, q3 = ww + u2;
     const double s1 = x1 * q1 + x2 * q2 + x3 * q3;
     const double s2 = e1 * q1 + e2 * q2 + e3 * q3;
@@ -1053,7 +1066,9 @@ void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
     case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;
     case 10: LAUNCH(k_initialo, nwg, 256, st, a); break;
     case 11: LAUNCH(k_metrics, nwg, 256, st, a); break;
-    case 14: LAUNCH(k_flux_all, nwg, 256, st, a); break;
+    case 14: LAUNCH(k_flux_all<FM_FUSED>, nwg, 256, st, a); break;
+    case 15: LAUNCH(k_flux_all<FM_STRESS>, nwg, 256, st, a); break;
+    case 16: LAUNCH(k_flux_all<FM_FROM_TV>, nwg, 256, st, a); break;
   }
 }
 void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a) {

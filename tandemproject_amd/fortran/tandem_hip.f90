@@ -147,6 +147,61 @@ module tandem_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nk
     end function
+    ! the sections of a stage (src/Main3D.f90:567-958) as drop-ins, in tp_stage's order
+    integer(c_int32_t) function tp_primitives(ctx, nk) bind(C, name="tp_primitives")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+    end function
+    integer(c_int32_t) function tp_viscous(ctx) bind(C, name="tp_viscous")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int32_t) function tp_flux_divergence(ctx, nk) bind(C, name="tp_flux_divergence")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+    end function
+    integer(c_int32_t) function tp_sponge(ctx) bind(C, name="tp_sponge")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int32_t) function tp_gcbc(ctx) bind(C, name="tp_gcbc")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int32_t) function tp_rk_update(ctx, nk) bind(C, name="tp_rk_update")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+    end function
+    integer(c_int32_t) function tp_wall(ctx) bind(C, name="tp_wall")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int32_t) function tp_junction(ctx) bind(C, name="tp_junction")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int32_t) function tp_interface_avg(ctx) bind(C, name="tp_interface_avg")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+    end function
+    ! deriv on one field in device memory (src, dst: device addresses)
+    integer(c_int32_t) function tp_field_deriv(lxi, let, lze, nbc2, nn, src, dst, stream) &
+        bind(C, name="tp_field_deriv")
+      import :: c_ptr, c_int32_t
+      integer(c_int32_t), value :: lxi, let, lze, nn
+      integer(c_int32_t), intent(in) :: nbc2(2)
+      type(c_ptr), value :: src, dst, stream
+    end function
+    ! the checks of tp_add_case without a context
+    integer(c_int32_t) function tp_check_case(nblocks, dims, nbc, ncd, njct, jct, jvel, iparams) &
+        bind(C, name="tp_check_case")
+      import :: c_int32_t
+      integer(c_int32_t), value :: nblocks, njct
+      integer(c_int32_t), intent(in) :: dims(*), nbc(*), ncd(*), jct(*), jvel(*), iparams(*)
+    end function
     integer(c_int32_t) function tp_step(ctx, nsteps) bind(C, name="tp_step")
       import :: c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -87,6 +87,62 @@ def h_grid_blocks(lxibk, letbk, lzebk, xu=7.0, xd=10.0, yb=7.0, yt=7.0, span=0.2
     return blocks
 
 
+def tandem_grid_blocks(lxibk, letbk, lzebk, spacing=3.0, xu=7.0, xd=10.0, ym=1.0, yb=7.0, yt=7.0, span=0.2,
+                       thick=0.12, blend=0.04, wall_spacing=2e-3):
+    """Grid of each parent block of two aerofoils in tandem (bkx = 6, bky = 4,
+    bkz = 1): columns upstream [-xu, 0], chord 1 [0, 1], the gap in two
+    columns [1, spacing] (clustered at the trailing and the leading edge),
+    chord 2 [spacing, spacing + 1], wake [spacing + 1, spacing + 1 + xd];
+    rows [-yb, -ym], [-ym, surface], [surface, ym], [ym, yt], the surface the
+    aerofoil over the chord columns and y = 0 elsewhere, clustered at the
+    surface (first spacing wall_spacing) and at y = +-ym from the far rows.
+    Returns (x, y, z) arrays shaped (nz, ny, nx) per parent block, row-major
+    over the columns as the layout numbers them."""
+    if len(lxibk) != 6 or len(letbk) != 4 or len(lzebk) != 1:
+        raise ValueError("the tandem grid has 6 x 4 x 1 blocks")
+    x1, x2 = 0.0, spacing
+    c1 = x1 + _chord(lxibk[1])
+    c2 = x2 + _chord(lxibk[4])
+    d1 = c1[1] - c1[0]
+    d2 = c2[1] - c2[0]
+    mid = 0.5 * (1.0 + x2)
+    xup = -xu * _stretch(lxibk[0], d1 / xu)[::-1]
+    xup[-1] = 0.0
+    ga = 1.0 + (mid - 1.0) * _stretch(lxibk[2], d1 / (mid - 1.0))
+    gb = x2 - (x2 - mid) * _stretch(lxibk[3], d2 / (x2 - mid))[::-1]
+    ga[-1] = gb[0] = mid
+    wk = x2 + 1.0 + xd * _stretch(lxibk[5], d2 / xd)
+    xcols = [xup, c1, ga, gb, c2, wk]
+    chord = [None, x1, None, None, x2, None]
+    en = [_stretch(letbk[1], wall_spacing / ym), _stretch(letbk[2], wall_spacing / ym)]   # surface -> ym
+    last = [en[0][-1] - en[0][-2], en[1][-1] - en[1][-2]]
+    ef = [_stretch(letbk[0], ym * last[0] / (yb - ym)), _stretch(letbk[3], ym * last[1] / (yt - ym))]   # ym -> far
+    nz = lzebk[0] + 1
+    zs = -0.5 * span + span * np.arange(nz) / lzebk[0]
+    blocks = []
+    for j in range(4):
+        for i in range(6):
+            x = xcols[i]
+            ts = naca_thickness(x - chord[i], thick, blend) if chord[i] is not None else np.zeros_like(x)
+            if j == 0:
+                y1 = -ym - (yb - ym) * ef[0][::-1]
+                y2 = np.broadcast_to(y1[:, None], (y1.size, x.size))
+            elif j == 1:
+                e = en[0][::-1]   # far side (y = -ym, e = 1) to the surface (e = 0)
+                y2 = -ts[None, :] + (-ym + ts[None, :]) * e[:, None]
+            elif j == 2:
+                y2 = ts[None, :] + (ym - ts[None, :]) * en[1][:, None]
+            else:
+                y1 = ym + (yt - ym) * ef[1]
+                y2 = np.broadcast_to(y1[:, None], (y1.size, x.size))
+            ny, nx = y2.shape
+            X = np.broadcast_to(x[None, None, :], (nz, ny, nx)).copy()
+            Y = np.broadcast_to(y2[None, :, :], (nz, ny, nx)).copy()
+            Z = np.broadcast_to(zs[:, None, None], (nz, ny, nx)).copy()
+            blocks.append((X, Y, Z))
+    return blocks
+
+
 def subblock_grids(layout, blocks):
     """Slice parent-block grids into the layout's sub-blocks; returns a list of
     flat ss arrays (x, y, z concatenated, Fortran order within each)."""

@@ -47,8 +47,14 @@ class Layout:
         return int(np.prod(d[b]))
 
 
-def _domdcomp(mb, bkx, bky, mbk, nviscous):
-    """Boundary codes and neighbour blocks of block mb (SingleTurbulence3D.f90:121-164)."""
+def _domdcomp(mb, bkx, bky, mbk, nviscous, walls):
+    """Boundary codes and neighbour blocks of block mb (SingleTurbulence3D.f90:121-164).
+
+    The reference's 3x2 layout has its one aerofoil between rows 0 and 1 of
+    column 1 (walls = [(0, 1)]); a wall (r, c) here puts an aerofoil surface
+    between rows r and r + 1 of column c: the eta end of block (r, c) and the
+    eta start of block (r + 1, c) are the wall (20 + 5 nviscous), each with
+    the block across the aerofoil as neighbour."""
     bcw = 20 + 5 * nviscous
     ms = [0, 0, 0]; me = [0, 0, 0]; nbcs = [0, 0, 0]; nbce = [0, 0, 0]
     ll = mb % bkx
@@ -66,20 +72,32 @@ def _domdcomp(mb, bkx, bky, mbk, nviscous):
         ms[1], me[1], nbcs[1], nbce[1] = mb - bkx, mb + bkx, 30, 30
     nbcs[2] = nbce[2] = 45
     ms[2] = me[2] = mb
-    if mb == 1:
-        nbce[1] = bcw
-    if mb == 4:
-        nbcs[1] = bcw
+    row = mb // bkx
+    for (r, c) in walls:
+        if ll == c and row == r:
+            nbce[1] = bcw
+        if ll == c and row == r + 1:
+            nbcs[1] = bcw
     return ms, me, nbcs, nbce
 
 
-def h_layout(lxibk, letbk, lzebk, npcx=None, npcy=None, npcz=None, nviscous=1):
+def h_layout(lxibk, letbk, lzebk, npcx=None, npcy=None, npcz=None, nviscous=1, walls=None):
     """The reference's bkx x bky x bkz H-type block layout split into sub-blocks.
 
     lxibk/letbk/lzebk are the per-column/row/plane last indices as in
     inputp.dat lines 1-3; npcx/npcy/npcz the sub-block counts (lines 6-8).
+    walls: aerofoil surfaces as (row, column) pairs, each between rows row
+    and row + 1 of its column (default: the reference's 3x2 layout, one
+    aerofoil between the two rows of the middle column).
     """
     bkx, bky, bkz = len(lxibk), len(letbk), len(lzebk)
+    if walls is None:
+        walls = [(0, 1)]
+    if bkz != 1:
+        raise ValueError("one block plane in zeta (split it with npcz)")
+    for (r, c) in walls:
+        if not (0 <= r < bky - 1 and 0 < c < bkx - 1):
+            raise ValueError(f"wall {(r, c)} must lie between two rows of an inner column")
     npcx = list(npcx or [1] * bkx)
     npcy = list(npcy or [1] * bky)
     npcz = list(npcz or [1] * bkz)
@@ -106,7 +124,7 @@ def h_layout(lxibk, letbk, lzebk, npcx=None, npcy=None, npcz=None, nviscous=1):
     start = np.zeros((nranks, 3), np.int32)
     for myid in range(nranks):
         mb = int(np.searchsorted(mo, myid, side="right") - 1)
-        ms, me, nbcs, nbce = _domdcomp(mb, bkx, bky, mbk, nviscous)
+        ms, me, nbcs, nbce = _domdcomp(mb, bkx, bky, mbk, nviscous, walls)
         rel = myid - mo[mb]
         ip = rel % npc[mb, 0]
         jp = (rel // npc[mb, 0]) % npc[mb, 1]
@@ -151,27 +169,44 @@ def h_layout(lxibk, letbk, lzebk, npcx=None, npcy=None, npcz=None, nviscous=1):
             for s in range(2):
                 if nbc[r, nn, s] in (10, 20, 25):
                     ncd[r, nn, s] = -1 if nbc[r, nn, s] == 10 else ncd[r, nn, s]
-    # junctions (SingleTurbulence3D.f90:378-443): defined for the 3x2 layout
+    # junctions (SingleTurbulence3D.f90:378-443): the corner lines where an
+    # aerofoil's leading or trailing edge meets the blocks beside it, per
+    # spanwise sub-block: members (ip, jp, block) bottom-left, bottom-right,
+    # top-left, top-right; the velocity is averaged over the two members
+    # whose faces are the wall (right ones at a leading edge, left ones at a
+    # trailing edge)
+    wallset = set(walls)
+    corners = []
+    for (r, c) in sorted(walls, key=lambda w: (w[0], w[1])):
+        if (r, c - 1) not in wallset:
+            corners.append((r, c - 1, 1))   # leading edge: walls on the right
+        if (r, c + 1) not in wallset:
+            corners.append((r, c, 0))       # trailing edge: walls on the left
     jct = []; jvel = []
-    njun = (bkx - 1) * (bky - 1)
-    if bkx >= 2 and bky >= 2:
-        if njun > 2:
-            raise ValueError("corner junction averaging is defined for at most 2 junctions (3x2 layout)")
-        for kpl in range(int(npc[0, 2])):
-            for nn in range(njun):
-                njct = nn + nn // (bkx - 1)
-                mem = []
-                for (ip, jp, mm) in ((1, 1, njct), (0, 1, njct + 1), (1, 0, njct + bkx), (0, 0, njct + bkx + 1)):
-                    r = mo[mm] + kpl * npc[mm, 1] * npc[mm, 0] + jp * (npc[mm, 1] - 1) * npc[mm, 0] + ip * (npc[mm, 0] - 1)
-                    mem.append((int(r), ip, jp))
-                jct.append(mem)
-                c = 1 - nn
-                jvel.append((c, 2 + c))
+    for kpl in range(int(npc[0, 2])):
+        for (r, cl, le) in corners:
+            mem = []
+            base = r * bkx + cl
+            for (ip, jp, mm) in ((1, 1, base), (0, 1, base + 1), (1, 0, base + bkx), (0, 0, base + bkx + 1)):
+                rr = mo[mm] + kpl * npc[mm, 1] * npc[mm, 0] + jp * (npc[mm, 1] - 1) * npc[mm, 0] + ip * (npc[mm, 0] - 1)
+                mem.append((int(rr), ip, jp))
+            jct.append(mem)
+            jvel.append((1, 3) if le else (0, 2))
     jct = np.array(jct, np.int32).reshape(-1, 4, 3)
     jvel = np.array(jvel, np.int32).reshape(-1, 2)
     return Layout(nb=nranks, dims=dims, nbc=nbc, ncd=ncd, nsz=nsz, jct=jct, jvel=jvel,
                   parent=parent, pos=pos, start=start, bk=(bkx, bky, bkz),
-                  blockdims=bdims.astype(np.int32))
+                  blockdims=bdims.astype(np.int32), extra={"walls": list(walls)})
+
+
+def tandem_layout(lxibk, letbk, lzebk, chord_cols=(1, 4), npcz=None, nviscous=1):
+    """Two aerofoils in tandem on a bkx x bky H layout: each chord column
+    carries an aerofoil between the two middle rows (no counterpart in the
+    reference, whose decks hold one aerofoil; the block codes, neighbours,
+    sub-blocks and junctions follow the same rules as h_layout)."""
+    bky = len(letbk)
+    r = bky // 2 - 1
+    return h_layout(lxibk, letbk, lzebk, npcz=npcz, nviscous=nviscous, walls=[(r, c) for c in chord_cols])
 
 
 def single_block(lxi, let, lze, nbc, ncd=None):

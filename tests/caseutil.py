@@ -12,12 +12,24 @@ BASE_PARAMS = dict(reoo=2.5e4, tempoo=273.0, amach1=0.4, amach2=0.0, amach3=0.0,
 
 
 def aerofoil_case(lxibk=(24, 32, 24), letbk=(20, 20), lzebk=(16,), npcz=None, npcx=None,
-                  nviscous=1, **over):
+                  nviscous=1, span=0.2, **over):
     lay = L.h_layout(list(lxibk), list(letbk), list(lzebk), npcx=npcx, npcz=npcz, nviscous=nviscous)
-    blocks = G.h_grid_blocks(list(lxibk), list(letbk), list(lzebk), xu=4, xd=5, yb=4, yt=4, span=0.2)
+    blocks = G.h_grid_blocks(list(lxibk), list(letbk), list(lzebk), xu=4, xd=5, yb=4, yt=4, span=span)
     ss = G.subblock_grids(lay, blocks)
-    p = dict(BASE_PARAMS); p["nviscous"] = nviscous; p.update(over)
+    p = dict(BASE_PARAMS); p["nviscous"] = nviscous; p["span"] = span; p.update(over)
     return lay, ss, p
+
+
+def c1_case():
+    """The plain CPU-runnable configuration: NACA0012 at AoA 0, M = 0.4,
+    Re_c = 1e4 (reoo = Re_c / M = 2.5e4: the reference scales by the speed
+    of sound), laminar, uniform-flow start, spanwise periodic over 0.1c.
+    The 4 blocks of 96 x 64 x 8 become the reference's 3x2 H layout of the
+    same in-plane size with 13 spanwise points, since the periodic closure
+    reads 12 points beside the face (LMD + 1, src/Subroutines3D.f90:39): 6
+    blocks, 207,662 points."""
+    return aerofoil_case(lxibk=(40, 80, 40), letbk=(48, 48), lzebk=(12,), span=0.1,
+                         amach1=0.4, reoo=2.5e4, nviscous=1)
 
 
 def seeded_field(n_points_shape, seed, nmodes=8, noise=1e-3):

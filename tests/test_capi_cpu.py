@@ -87,3 +87,36 @@ def test_fortran_binding_compiles_links_and_runs(tmp_path):
     assert "binding ok" in out.stdout
     if not torch.cuda.is_available():
         assert "tp_create failed as expected" in out.stdout and "no HIP device" in out.stdout
+
+
+def test_case_checks_without_gpu():
+    """tp_check_case refuses, without a context, what tp_add_case refuses:
+    a block of 2^29 points or more (32-bit byte offsets in the line solves),
+    periodic lines shorter than the 13 points their closure reads, unknown
+    boundary codes; and accepts the reference's 3x2 layout."""
+    from caseutil import BASE_PARAMS
+    from tandemproject_amd import layout as L
+    api.check_case(L.h_layout([24, 32, 24], [20, 20], [31], npcz=[2]), BASE_PARAMS)
+    with pytest.raises(api.TPError, match="2\\^29"):
+        api.check_case(L.single_block(1023, 1023, 511, [[10, 10], [10, 10], [45, 45]]), BASE_PARAMS)
+    api.check_case(L.single_block(1023, 1023, 510, [[10, 10], [10, 10], [45, 45]]), BASE_PARAMS)
+    with pytest.raises(api.TPError, match="its faces need >= 13"):
+        api.check_case(L.single_block(20, 20, 8, [[10, 10], [10, 10], [45, 45]]), BASE_PARAMS)
+    with pytest.raises(api.TPError, match="unknown boundary code"):
+        api.check_case(L.single_block(20, 20, 20, [[10, 11], [10, 10], [45, 45]]), BASE_PARAMS)
+
+
+def test_oracle_refuses_bad_cases():
+    """The oracle answers a case it cannot hold with an error, not an abort:
+    a 9-point periodic line (its closure reads 13 points) and a selector out
+    of range raise OracleError in this process."""
+    import pyoracle as po
+    from caseutil import BASE_PARAMS
+    from tandemproject_amd import layout as L
+    with pytest.raises(po.OracleError, match="has 9 points; its faces need >= 13"):
+        po.Oracle(L.single_block(20, 20, 8, [[10, 10], [10, 10], [45, 45]]), BASE_PARAMS)
+    o = po.Oracle(L.single_block(20, 20, 20, [[10, 10], [10, 10], [45, 45]]), BASE_PARAMS)
+    with pytest.raises(po.OracleError, match="out of range"):
+        o.deriv(4, 1, 1)
+    with pytest.raises(po.OracleError, match="out of range"):
+        o.stage_part(9, "begin")

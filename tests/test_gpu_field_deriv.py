@@ -1,0 +1,71 @@
+"""tp_field_deriv: the reference's deriv (src/Subroutines3D.f90:84-141) on one
+field held by the caller in device memory, against the oracle's deriv on the
+same seeded field, to the per-call bar of 1e-13 relative; every direction,
+one-sided and periodic ends, lines from 13 to 481 points."""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from caseutil import BASE_PARAMS, seeded_field
+from tandemproject_amd import api
+from tandemproject_amd import layout as L
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (lxi, let, lze), nn, end codes of direction nn
+    ((39, 29, 19), 1, (10, 30)),
+    ((39, 29, 19), 2, (25, 10)),
+    ((39, 29, 19), 3, (45, 45)),
+    ((12, 17, 23), 1, (45, 45)),
+    ((239, 12, 14), 1, (20, 10)),
+    ((30, 480, 13), 2, (10, 10)),
+    ((63, 8, 95), 3, (45, 45)),
+]
+
+
+def oracle_deriv(dims, nn, ends, f):
+    nbc = [[10, 10], [10, 10], [10, 10]]
+    nbc[nn - 1] = list(ends)
+    lay = L.single_block(*dims, nbc)
+    o = po.Oracle(lay, BASE_PARAMS)
+    o.array(0, "rr")[:f.size] = f
+    o.mpigo(0, 1, 0)
+    o.deriv(nn, 1, 0)
+    N = f.size
+    return o.array(0, "rr")[(nn - 1) * N:nn * N].copy()
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_field_deriv_matches_oracle(case):
+    import torch
+    dims, nn, ends = CASES[case]
+    print("field deriv", dims, "direction", nn, "ends", ends, flush=True)
+    shape = (dims[2] + 1, dims[1] + 1, dims[0] + 1)
+    f = seeded_field(shape, 31 + case)
+    ref = oracle_deriv(dims, nn, ends, f)
+    src = torch.tensor(f, dtype=torch.float64, device="cuda")
+    dst = torch.full_like(src, float("nan"))
+    torch.cuda.synchronize()
+    api.field_deriv(dims, ends, nn, src.data_ptr(), dst.data_ptr())
+    torch.cuda.synchronize()
+    got = dst.cpu().numpy()
+    err = float(np.abs(got - ref).max() / np.abs(ref).max())
+    print("  max rel diff", err, flush=True)
+    assert np.isfinite(got).all()
+    assert err <= 1e-13
+    # the input is untouched
+    np.testing.assert_array_equal(src.cpu().numpy(), f)
+
+
+def test_field_deriv_refuses_bad_input():
+    import torch
+    x = torch.zeros(8, dtype=torch.float64, device="cuda")
+    with pytest.raises(api.TPError, match="end codes"):
+        api.field_deriv((20, 20, 20), (10, 40), 1, x.data_ptr(), x.data_ptr())
+    with pytest.raises(api.TPError, match="512 points"):
+        api.field_deriv((600, 8, 8), (10, 10), 1, x.data_ptr(), x.data_ptr())
+    with pytest.raises(api.TPError, match="too short"):
+        api.field_deriv((20, 20, 8), (45, 45), 3, x.data_ptr(), x.data_ptr())
