@@ -42,9 +42,13 @@ enum {
  * it ships disabled): a velocity table over the inflow-sponge points sampled
  * at TP_I_NTZ+1 times over one period, filled from
  *   1: TP_I_NMODES seeded random Fourier modes of turbulence (tp_inflow_modes);
- *   2: the single mode of a sinusoidal transverse gust (tp_gust_mode). */
+ *   2: the single mode of a sinusoidal transverse gust (tp_gust_mode);
+ *   3: the reference's synthetic eddies (src/SingleTurbulence3D.f90:262-291),
+ *      given by tp_set_inflow_eddies, the table scaled by TP_D_CFIT.
+ * TP_I_FORCING = 1 adds the reference's localised spanwise-modal body force
+ * (forceup/forcego, src/rpt.f90:728-769; switched off in its decks). */
 enum { TP_I_NKRK = 0, TP_I_NVISCOUS, TP_I_NSMF, TP_I_LES, TP_I_NEXTRABC, TP_I_NEXTGCIC,
-       TP_I_INFLOW, TP_I_NMODES, TP_I_SEED, TP_I_NTZ, TP_I_NPARAM = 16 };
+       TP_I_INFLOW, TP_I_NMODES, TP_I_SEED, TP_I_NTZ, TP_I_FORCING, TP_I_NPARAM = 16 };
 /* double case parameters, dparams[] (inputo.dat, inputp.dat; fltk/fltkbc in units of pi).
  * Inflow: TP_D_TGUSTO start time (tgusto), TP_D_SLIT length of the periodic
  * turbulence box along x (the table's period is slit/u_inf, as slit/uoo(1) in
@@ -52,13 +56,18 @@ enum { TP_I_NKRK = 0, TP_I_NVISCOUS, TP_I_NSMF, TP_I_LES, TP_I_NEXTRABC, TP_I_NE
  * component over |u_inf|), TP_D_LTURB integral length scale, TP_D_GUSTW half
  * width in y of the band of inflow-sponge points that receive the turbulence.
  * Gust (TP_I_INFLOW = 2): TP_D_TU amplitude over |u_inf|, TP_D_GUSTK reduced
- * frequency omega c / (2 |u_inf|) with chord c = 1; TP_D_SLIT is derived. */
+ * frequency omega c / (2 |u_inf|) with chord c = 1; TP_D_SLIT is derived.
+ * Eddies (TP_I_INFLOW = 3): TP_D_CFIT scale of the table (cfit), TP_D_SLIT
+ * the x-period of the eddy train.  Forcing: TP_D_AMFOR amplitude, TP_D_XFOR,
+ * TP_D_YFOR centre, TP_D_RFOR radius, TP_D_TSFOR, TP_D_TEFOR time window
+ * (amfor, xfor, yfor, rfor, tsfor, tefor of src/MainVar3D.f90:167). */
 enum {
   TP_D_REOO = 0, TP_D_TEMPOO, TP_D_AMACH1, TP_D_AMACH2, TP_D_AMACH3, TP_D_WTEMP, TP_D_CFL,
   TP_D_TMAX, TP_D_TIMF, TP_D_FLTK, TP_D_FLTKBC, TP_D_DTO, TP_D_AOA0, TP_D_AOA1, TP_D_TPS,
   TP_D_TP, TP_D_SMAGO1, TP_D_SMAGO2, TP_D_DOMLEN, TP_D_SZTH1, TP_D_SZTH2, TP_D_SZXT,
   TP_D_SZCO, TP_D_SPAN, TP_D_WLEW, TP_D_WLEA, TP_D_TGUSTO, TP_D_SLIT, TP_D_TU, TP_D_LTURB,
-  TP_D_GUSTW, TP_D_GUSTK, TP_D_NPARAM = 32
+  TP_D_GUSTW, TP_D_GUSTK, TP_D_CFIT, TP_D_AMFOR, TP_D_XFOR, TP_D_YFOR, TP_D_RFOR, TP_D_TSFOR,
+  TP_D_TEFOR, TP_D_NPARAM = 48
 };
 /* arrays reachable by tp_upload / tp_download (names of src/MainVar3D.f90:78-102) */
 enum {
@@ -262,6 +271,21 @@ int32_t tp_inflow_modes(int32_t nmodes, uint64_t seed, double tu, double lturb, 
  * slit / u_x is the gust period.  Needs no device.  No counterpart in the
  * reference. */
 int32_t tp_gust_mode(double tu, double kr, const double *uoo3, double *out8, double *slit);
+
+/* ---- the reference's synthetic eddies (TP_I_INFLOW = 3) ----
+ * The eddies of the reference's inflow (inputext, src/SingleTurbulence3D.f90:
+ * 86-115): from its random numbers (randnum.dat, the first 8 nits values in
+ * file order: ran(:,1), ait(:,1), ran(:,2), ait(:,2), ran(:,3), ait(:,3),
+ * yit(:), zit(:)) and deck values deck[18] = span, slit, gaus, cutlb,
+ * domlen, szth1, tl0(0), tl1(0), tlw(0), tl0(1), tl1(1), tlw(1), tam(1:3,0),
+ * tam(1:3,1) (inputp.dat): out10[n*10 + {0..9}] = centre xit, yit, zit,
+ * sit(1:3) (1/size^2), ait(1:3) (amplitudes), type mxc.  Needs no device. */
+int32_t tp_eddy_params(int32_t nits, const double *randnum, const double *deck, double *out10);
+/* the eddies a case with TP_I_INFLOW = 3 builds its inflow table from
+ * (src/SingleTurbulence3D.f90:262-298: each eddy and its images one slit
+ * upstream and one span to either side, convected with u_inf, the table
+ * scaled by TP_D_CFIT); before the case's geometry is set up */
+int32_t tp_set_inflow_eddies(tp_context *ctx, int32_t case_id, int32_t nits, const double *eddies10);
 
 /* ---- the reference's file formats ----
  * The reference writes one restart and one raw grid file per parent block

@@ -14,13 +14,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # copy of it load that copy instead)
 LIB_PATH = os.environ.get("ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 
-I_NKRK, I_NVISCOUS, I_NSMF, I_LES, I_NEXTRABC, I_NEXTGCIC, I_INFLOW, I_NMODES, I_SEED, I_NTZ = range(10)
+I_NKRK, I_NVISCOUS, I_NSMF, I_LES, I_NEXTRABC, I_NEXTGCIC, I_INFLOW, I_NMODES, I_SEED, I_NTZ, I_FORCING = range(11)
 N_IPARAM = 16
 D_NAMES = ["reoo", "tempoo", "amach1", "amach2", "amach3", "wtemp", "cfl", "tmax",
            "timf", "fltk", "fltkbc", "dto", "aoa0", "aoa1", "tps", "tp", "smago1",
            "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea",
-           "tgusto", "slit", "tu", "lturb", "gustw"]
-N_DPARAM = 32
+           "tgusto", "slit", "tu", "lturb", "gustw", "gustk", "cfit", "amfor", "xfor", "yfor", "rfor",
+           "tsfor", "tefor"]
+N_DPARAM = 48
 A_NAMES = ["qa", "qo", "de", "rr", "ss", "xim", "etm", "zem", "p", "yaco", "tv",
            "drva1", "drva2", "drva3", "drvb1", "drvb2", "drvb3", "send1", "send2", "send3",
            "recv1", "recv2", "recv3", "cm1", "cm2", "cm3", "xu", "xl", "yu", "yl",
@@ -84,6 +85,9 @@ def lib():
             [ctypes.c_double] * 5 + [dp, dp, dp]
         L.orc_sponge_list.argtypes = [P, ctypes.c_int, ip, ip]
         L.orc_sponge_list.restype = ctypes.c_int
+        L.orc_set_inflow_eddies.argtypes = [P, ctypes.c_int, dp]
+        L.orc_force_list.argtypes = [P, ctypes.c_int, ip, dp]
+        L.orc_force_list.restype = ctypes.c_int
         L.orc_set_inflow_modes.argtypes = [P, ctypes.c_int, dp]
         L.orc_inflow_weights.restype = ctypes.c_int
         L.orc_inflow_weights.argtypes = [P, ctypes.c_double, ctypes.c_double, ip, dp, dp]
@@ -125,6 +129,7 @@ class Oracle:
         ipar[I_NMODES] = params.get("nmodes", 0)
         ipar[I_SEED] = params.get("seed", 0)
         ipar[I_NTZ] = params.get("ntz", 0)
+        ipar[I_FORCING] = params.get("forcing", 0)
         dpar = np.zeros(N_DPARAM)
         for i, k in enumerate(D_NAMES):
             dpar[i] = params.get(k, 0.0)
@@ -142,6 +147,19 @@ class Oracle:
         m = np.ascontiguousarray(modes, np.float64)
         self._modes = m
         lib().orc_set_inflow_modes(self.h, m.shape[0], _dp(m))
+
+    def set_inflow_eddies(self, eddies):
+        """Synthetic eddies (n, 10) of inflow 3 (before metrics)."""
+        e = np.ascontiguousarray(eddies, np.float64)
+        self._eddies = e
+        lib().orc_set_inflow_eddies(self.h, e.shape[0], _dp(e))
+
+    def force_list(self, b):
+        """(point index, xafor (3, n)) of the forcing points of block b"""
+        n = lib().orc_force_list(self.h, int(b), None, None)
+        l = np.zeros(max(n, 1), np.int32); x = np.zeros(3 * max(n, 1))
+        lib().orc_force_list(self.h, int(b), _ip(l), _dp(x))
+        return l[:n], x[:3 * n].reshape(3, n)
 
     def inflow_weights(self, timo, dtk):
         """Table rows, node times and weights of the time interpolation, or None."""

@@ -118,6 +118,9 @@ typedef struct {
   double xbc_ra0;   /* state of the global ra0 seen by extrabcc */
   int nsurf, sside; /* wall surface points (eta face), wall side (0 j = 0, 1 j = let) */
   double *sarea, *snor, *sst;   /* area(0:lcwall), wnor(0:lcwall,3), stats p_prev, S1, S2 */
+  int lfor;         /* forcing points - 1 */
+  int *lcfor;       /* their point indices */
+  double *xafor;    /* xafor(0:lfor, 3) */
 } oblk;
 
 struct orc {
@@ -128,6 +131,10 @@ struct orc {
   int nkrk, nviscous, nsmf, les, nextrabc, nextgcic;
   int inflow, nmodes, ntz;
   double tgusto, slit, gustw, *modes;
+  int nits;            /* synthetic eddies (inflow 3): xit, yit, zit, sit(3), ait(3), mxc each */
+  double *eddies, cfit;
+  int forcing;         /* body force, src/rpt.f90:728-769 */
+  double amfor, xfor, yfor, rfor, tsfor, tefor;
   double reoo, tempoo, amach1, amach2, amach3, wtemp, cfl, tmax, timf, fltk,
       fltkbc, dto, aoa0, aoa1, tps, tp, smago1, smago2, domlen, szth1, szth2,
       szxt, szco, span, wlew, wlea;
@@ -644,6 +651,10 @@ orc *orc_create(int nb, const int32_t *dims, const int32_t *nbc,
   c->nextrabc = ip[ORC_I_NEXTRABC]; c->nextgcic = ip[ORC_I_NEXTGCIC];
   c->inflow = ip[ORC_I_INFLOW]; c->ntz = ip[ORC_I_NTZ];
   c->tgusto = dp[ORC_D_TGUSTO]; c->slit = dp[ORC_D_SLIT]; c->gustw = dp[ORC_D_GUSTW];
+  c->cfit = dp[ORC_D_CFIT];
+  c->forcing = ip[ORC_I_FORCING];
+  c->amfor = dp[ORC_D_AMFOR]; c->xfor = dp[ORC_D_XFOR]; c->yfor = dp[ORC_D_YFOR];
+  c->rfor = dp[ORC_D_RFOR]; c->tsfor = dp[ORC_D_TSFOR]; c->tefor = dp[ORC_D_TEFOR];
   c->reoo = dp[ORC_D_REOO]; c->tempoo = dp[ORC_D_TEMPOO];
   c->amach1 = dp[ORC_D_AMACH1]; c->amach2 = dp[ORC_D_AMACH2]; c->amach3 = dp[ORC_D_AMACH3];
   c->wtemp = dp[ORC_D_WTEMP]; c->cfl = dp[ORC_D_CFL]; c->tmax = dp[ORC_D_TMAX];
@@ -749,8 +760,10 @@ void orc_destroy(orc *c) {
     free(B->rpex); free(B->sbcc); free(B->lcsz); free(B->asz); free(B->bsz);
     free(B->lctz); free(B->atz); free(B->vito);
     free(B->sarea); free(B->snor); free(B->sst);
+    free(B->lcfor); free(B->xafor);
   }
   free(c->modes);
+  free(c->eddies);
   free(c->b); free(c->jct); free(c->jvel); free(c);
 }
 
@@ -818,6 +831,24 @@ int orc_sponge_list(const orc *c, int b, int32_t *l, int32_t *g) {
     for (int k = 0; k <= B->ltz; k++) g[B->lctz[k]] = k;
   }
   return n;
+}
+
+int orc_force_list(const orc *c, int b, int32_t *l, double *xafor3) {
+  const oblk *B = &c->b[b];
+  const int n = B->lfor + 1;
+  for (int q = 0; q < n; q++) {
+    if (l) l[q] = B->lcfor[q];
+    if (xafor3)
+      for (int m = 0; m < 3; m++) xafor3[m * n + q] = B->xafor[m * n + q];
+  }
+  return n;
+}
+
+void orc_set_inflow_eddies(orc *c, int nits, const double *eddies10) {
+  free(c->eddies);
+  c->nits = nits;
+  c->eddies = dalloc((size_t)10 * nits);
+  memcpy(c->eddies, eddies10, sizeof(double) * 10 * nits);
 }
 
 void orc_set_inflow_modes(orc *c, int nmodes, const double *modes8) {
@@ -1330,10 +1361,10 @@ void orc_metrics(orc *c) {
     }
     free(lst); free(sg); free(lm);
     /* inflow points and their velocity table, src/SingleTurbulence3D.f90:253-292
-     * (the table is filled from the given Fourier modes instead of eddies) */
+     * (inflow 3: the synthetic eddies; 1, 2: the given Fourier modes) */
     free(B->lctz); free(B->atz); free(B->vito);
     B->lctz = NULL; B->atz = NULL; B->vito = NULL; B->ltz = -1;
-    if (c->inflow && c->modes) {
+    if ((c->inflow == 3 && c->eddies) || (c->inflow && c->inflow != 3 && c->modes)) {
       double ra0 = PI / c->szth1;
       int *gl = (int *)malloc(sizeof(int) * (B->lsz + 2));
       int lt = -1;
@@ -1352,6 +1383,47 @@ void orc_metrics(orc *c) {
         double sn = sin(fmin(ra0 * (B->ss[l] + c->domlen), PI / 2));
         B->atz[g] = sn * sn;
       }
+      if (c->inflow == 3) {
+        /* each eddy nn and its images one slit upstream and one span to
+         * either side, convected with uoo to the table times tt(ii):
+         * velocity rr x d at the point, d its offset from the eddy centre,
+         * src/SingleTurbulence3D.f90:268-291; then scaled by cfit (:298) */
+        const int ng = lt + 1;
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+        for (int e = 0; e < ng * nt1; e++) {
+          int g = e % ng, ii = e / ng;
+          int l = B->lcsz[gl[g]];
+          double tt = ii * fctr;
+          const double px = B->ss[l], py = B->ss[N + l], pz = B->ss[2 * (size_t)N + l];
+          double v0 = zero, v1 = zero, v2 = zero;
+          for (int nn = 0; nn < c->nits; nn++) {
+            const double *E = c->eddies + 10 * (size_t)nn;
+            const double mxc = E[9];
+            double ve[3];
+            for (int m = 0; m < 3; m++) ve[m] = (-9 + mxc) * E[3 + m] * E[3 + m];
+            const double ra1 = -36 - 60 * mxc, ra2 = two_three * mxc;
+            for (int i = 0; i <= 1; i++)
+              for (int k = -1; k <= 1; k++) {
+                const double dm0 = E[0] - i * c->slit, dm1 = E[1], dm2 = E[2] - k * c->span;
+                const double rv0 = dm0 + tt * c->uoo[0], rv1 = dm1 + tt * c->uoo[1], rv2 = dm2 + tt * c->uoo[2];
+                const double d1 = px - rv0, d2 = py - rv1, d3 = pz - rv2;
+                const double d4 = d1 * d1 + d2 * d2 + d3 * d3;
+                double d5 = d4 * d4;
+                double r1 = ve[0] * d5, r2 = ve[1] * d5, r3 = ve[2] * d5;
+                d5 = ra1 * d4;
+                r1 = E[6] * d5 * exp(r1) * (1 + ra2 * r1);
+                r2 = E[7] * d5 * exp(r2) * (1 + ra2 * r2);
+                r3 = E[8] * d5 * exp(r3) * (1 + ra2 * r3);
+                v0 = v0 + r2 * d3 - r3 * d2;
+                v1 = v1 + r3 * d1 - r1 * d3;
+                v2 = v2 + r1 * d2 - r2 * d1;
+              }
+          }
+          B->vito[((size_t)0 * nt1 + ii) * ng + g] = c->cfit * v0;
+          B->vito[((size_t)1 * nt1 + ii) * ng + g] = c->cfit * v1;
+          B->vito[((size_t)2 * nt1 + ii) * ng + g] = c->cfit * v2;
+        }
+      } else {
 #pragma omp parallel for num_threads(nthreads()) schedule(static)
       for (int e = 0; e < (lt + 1) * nt1; e++) {
         int g = e % (lt + 1), ii = e / (lt + 1);
@@ -1369,6 +1441,42 @@ void orc_metrics(orc *c) {
         B->vito[((size_t)1 * nt1 + ii) * (lt + 1) + g] = s1;
         B->vito[((size_t)2 * nt1 + ii) * (lt + 1) + g] = s2;
       }
+      }
+    }
+    /* forcing points and coefficients (forceup, src/rpt.f90:728-753): the
+     * points within rfor of (xfor, yfor), three spanwise modes each with a
+     * Gaussian weight ~ exp(-ra0 r^2), ra0 = -ln(1e-4) / rfor^2.  As in the
+     * reference, ra1 holds 2 pi / span only for the first point: the loop
+     * overwrites it with each point's weight, which the next point's modes
+     * then take as their wavenumber. */
+    free(B->lcfor); free(B->xafor);
+    B->lcfor = NULL; B->xafor = NULL; B->lfor = -1;
+    if (c->forcing == 1) {
+      double ra0 = -log(0.0001) / (c->rfor * c->rfor), ra1 = 2 * PI / c->span, ra2 = c->rfor * c->rfor;
+      int ll = -1;
+      int *lst = (int *)malloc(sizeof(int) * (N + 1));
+      double *r2v = dalloc(N);
+      for (int l = 0; l < N; l++) {
+        r2v[l] = (B->ss[l] - c->xfor) * (B->ss[l] - c->xfor) + (B->ss[N + l] - c->yfor) * (B->ss[N + l] - c->yfor);
+        if (r2v[l] - ra2 < 0) lst[++ll] = l;
+      }
+      B->lfor = ll;
+      if (ll >= 0) {
+        B->lcfor = lst;
+        B->xafor = dalloc((size_t)3 * (ll + 1));
+        for (int q = 0; q <= ll; q++) {
+          int l = lst[q];
+          double z = B->ss[2 * (size_t)N + l];
+          double b1 = cos(ra1 * z), b2 = cos(3 * ra1 * z), b3 = cos(4 * ra1 * z);
+          ra1 = half * exp(-ra0 * r2v[l]) / B->yaco[l];
+          B->xafor[q] = ra1 * b1;
+          B->xafor[(ll + 1) + q] = ra1 * b2;
+          B->xafor[2 * (size_t)(ll + 1) + q] = ra1 * b3;
+        }
+      } else {
+        free(lst);
+      }
+      free(r2v);
     }
   }
 }
@@ -1828,6 +1936,27 @@ static void stage_sponge(orc *c, int nk) {
       for (int d = 0; d < 3; d++)
         B->de[(1 + d) * N + l] = B->de[(1 + d) * N + l] + a * (B->qa[(1 + d) * N + l] - B->ss[d * N + l]);
       B->de[4 * N + l] = B->de[4 * N + l] + B->bsz[q] * (B->p[l] - ham * pow(B->rr[l], gam));
+    }
+  }
+  /* body force, forcego (src/rpt.f90:758-769; src/Main3D.f90:776 calls it
+   * right after the sponge) */
+  if (c->forcing == 1) {
+    const double t = sc[ORC_S_TIMO] + sc[ORC_S_DTK];
+    if (t > c->tsfor && t < c->tefor) {
+      const double ra0 = t - c->tsfor;
+      const double ra1 = c->amfor * cos(ra0 * 48.76 * c->amachoo) / 3;
+      const double ra2 = c->amfor * cos(ra0 * 53.6 * c->amachoo) / 3;
+      const double ra3 = c->amfor * cos(ra0 * 53.6 * c->amachoo) / 3;
+      for (int b = 0; b < c->nb; b++) {
+        oblk *B = &c->b[b];
+        const int N = B->N, nf = B->lfor + 1;
+        for (int q = 0; q < nf; q++) {
+          const int l = B->lcfor[q];
+          const double f = ra1 * B->xafor[q] + ra2 * B->xafor[nf + q] + ra3 * B->xafor[2 * (size_t)nf + q];
+          B->de[N + l] = B->de[N + l] + B->qa[l] * f;
+          B->de[2 * N + l] = B->de[2 * N + l] - B->qa[l] * f;
+        }
+      }
     }
   }
 }

@@ -25,7 +25,7 @@ extern "C" {
 /* integer parameters (index into iparams[]) */
 enum {
   ORC_I_NKRK = 0, ORC_I_NVISCOUS, ORC_I_NSMF, ORC_I_LES, ORC_I_NEXTRABC,
-  ORC_I_NEXTGCIC, ORC_I_INFLOW, ORC_I_NMODES, ORC_I_SEED, ORC_I_NTZ, ORC_I_NPARAM = 16
+  ORC_I_NEXTGCIC, ORC_I_INFLOW, ORC_I_NMODES, ORC_I_SEED, ORC_I_NTZ, ORC_I_FORCING, ORC_I_NPARAM = 16
 };
 /* double parameters (index into dparams[]) */
 enum {
@@ -34,7 +34,8 @@ enum {
   ORC_D_DTO, ORC_D_AOA0, ORC_D_AOA1, ORC_D_TPS, ORC_D_TP, ORC_D_SMAGO1,
   ORC_D_SMAGO2, ORC_D_DOMLEN, ORC_D_SZTH1, ORC_D_SZTH2, ORC_D_SZXT,
   ORC_D_SZCO, ORC_D_SPAN, ORC_D_WLEW, ORC_D_WLEA, ORC_D_TGUSTO, ORC_D_SLIT,
-  ORC_D_TU, ORC_D_LTURB, ORC_D_GUSTW, ORC_D_NPARAM = 32
+  ORC_D_TU, ORC_D_LTURB, ORC_D_GUSTW, ORC_D_GUSTK, ORC_D_CFIT, ORC_D_AMFOR, ORC_D_XFOR, ORC_D_YFOR,
+  ORC_D_RFOR, ORC_D_TSFOR, ORC_D_TEFOR, ORC_D_NPARAM = 48
 };
 
 /* per-block arrays reachable through orc_array() */
@@ -74,6 +75,13 @@ const char *orc_last_error(void);
 /* inflow: the Fourier modes (8 doubles each: k, a, phase, 0) the velocity
  * table is built from; call before orc_metrics */
 void orc_set_inflow_modes(orc *c, int nmodes, const double *modes8);
+/* inflow 3: the synthetic eddies the table is built from, 10 doubles each:
+ * centre xit, yit, zit, sit(1:3), ait(1:3), mxc (src/SingleTurbulence3D.f90:
+ * 86-115); call before orc_metrics */
+void orc_set_inflow_eddies(orc *c, int nits, const double *eddies10);
+/* forcing points of block b and their coefficients xafor(:,1:3) (forceup);
+ * returns the count; either pointer may be NULL */
+int orc_force_list(const orc *c, int b, int32_t *l, double *xafor3);
 /* time interpolation of the inflow table at (timo, dtk): table rows il[5],
  * node times tl[5] and weights w[5]; returns 0 before tgusto */
 int orc_inflow_weights(const orc *c, double timo, double dtk, int *il5, double *tl5, double *w5);

@@ -14,11 +14,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtandem_hip.so")
 
-I_NAMES = ["nkrk", "nviscous", "nsmf", "les", "nextrabc", "nextgcic", "inflow", "nmodes", "seed", "ntz"]
+I_NAMES = ["nkrk", "nviscous", "nsmf", "les", "nextrabc", "nextgcic", "inflow", "nmodes", "seed", "ntz", "forcing"]
 D_NAMES = ["reoo", "tempoo", "amach1", "amach2", "amach3", "wtemp", "cfl", "tmax",
            "timf", "fltk", "fltkbc", "dto", "aoa0", "aoa1", "tps", "tp", "smago1",
            "smago2", "domlen", "szth1", "szth2", "szxt", "szco", "span", "wlew", "wlea", "tgusto",
-           "slit", "tu", "lturb", "gustw", "gustk"]
+           "slit", "tu", "lturb", "gustw", "gustk", "cfit", "amfor", "xfor", "yfor", "rfor", "tsfor", "tefor"]
+N_IPARAM, N_DPARAM = 16, 48
 ARRAYS = ["qa", "qo", "de", "rr", "ss", "xim", "etm", "zem", "p", "yaco", "tv",
           "drva1", "drva2", "drva3", "drvb1", "drvb2", "drvb3", "send1", "send2", "send3",
           "recv1", "recv2", "recv3", "cm1", "cm2", "cm3", "xu", "xl", "yu", "yl", "rpex"]
@@ -39,7 +40,8 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_case_dims", "tp_case_params", "tp_write_restart", "tp_read_restart", "tp_write_grid",
            "tp_read_grid", "tp_write_plot3d", "tp_get_grid", "tp_check_case",
            "tp_primitives", "tp_viscous", "tp_flux_divergence", "tp_sponge", "tp_gcbc", "tp_rk_update",
-           "tp_wall", "tp_junction", "tp_interface_avg", "tp_field_deriv"]
+           "tp_wall", "tp_junction", "tp_interface_avg", "tp_field_deriv", "tp_eddy_params",
+           "tp_set_inflow_eddies"]
 # the sections of a stage, in the order tp_stage runs them (name, takes nk)
 STAGE_PARTS = [("primitives", True), ("viscous", False), ("flux_divergence", True), ("sponge", False),
                ("gcbc", False), ("rk_update", True), ("wall", False), ("junction", False),
@@ -79,7 +81,9 @@ def load(path=LIB_PATH):
         "tp_primitives": [P, i32], "tp_viscous": [P], "tp_flux_divergence": [P, i32], "tp_sponge": [P],
         "tp_gcbc": [P], "tp_rk_update": [P, i32], "tp_wall": [P], "tp_junction": [P],
         "tp_interface_avg": [P],
-        "tp_field_deriv": [i32, i32, i32, pi32, i32, P, P, P], "tp_step_graph": [P, i32], "tp_sync": [P],
+        "tp_field_deriv": [i32, i32, i32, pi32, i32, P, P, P],
+        "tp_eddy_params": [i32, pd, pd, pd],
+        "tp_set_inflow_eddies": [P, i32, i32, pd], "tp_step_graph": [P, i32], "tp_sync": [P],
         "tp_profile": [P, i32], "tp_profile_classes": [P, i32],
         "tp_profile_read": [P, i32, pd, ctypes.POINTER(i64), pd, i32],
         "tp_info": [P, ctypes.POINTER(i64)],
@@ -166,10 +170,10 @@ def case_args(layout, params):
     """Arrays of tp_add_case for a block layout and a parameter dict: the
     layout arrays, iparams and dparams (unnamed parameters are 0, nkrk 4,
     nviscous 1)."""
-    ipar = np.zeros(16, np.int32)
+    ipar = np.zeros(N_IPARAM, np.int32)
     for i, k in enumerate(I_NAMES):
         ipar[i] = int(params.get(k, {"nkrk": 4, "nviscous": 1}.get(k, 0)))
-    dpar = np.zeros(32)
+    dpar = np.zeros(N_DPARAM)
     for i, k in enumerate(D_NAMES):
         dpar[i] = float(params.get(k, 0.0))
     arrs = [np.ascontiguousarray(a, np.int32) for a in
@@ -202,6 +206,19 @@ class PinnedBuffer:
 
     def __del__(self):
         self.free()
+
+
+def eddy_params(nits, randnum, deck):
+    """tp_eddy_params: the reference's synthetic eddies, (nits, 10) array of
+    centre (3), 1/size^2 (3), amplitude (3), type.  deck: dict with span,
+    slit, gaus, cutlb, domlen, szth1, tl0, tl1, tlw (2 each), tam (2 x 3)."""
+    r = np.ascontiguousarray(randnum[:8 * nits], np.float64)
+    d = np.array([deck["span"], deck["slit"], deck["gaus"], deck["cutlb"], deck["domlen"], deck["szth1"],
+                  deck["tl0"][0], deck["tl1"][0], deck["tlw"][0], deck["tl0"][1], deck["tl1"][1], deck["tlw"][1],
+                  *deck["tam"][0], *deck["tam"][1]], np.float64)
+    out = np.zeros((nits, 10))
+    _check(load().tp_eddy_params(int(nits), _dp(r), _dp(d), _dp(out)))
+    return out
 
 
 def field_deriv(dims3, nbc2, nn, src_ptr, dst_ptr, stream=None):
@@ -441,6 +458,10 @@ class Solver:
 
     def stage(self, nk):
         _check(load().tp_stage(self.h, nk))
+
+    def set_inflow_eddies(self, case, eddies):
+        e = np.ascontiguousarray(eddies, np.float64)
+        _check(load().tp_set_inflow_eddies(self.h, int(case), e.shape[0], _dp(e)))
 
     def stage_part(self, nk, part):
         """One section of stage nk (a name of STAGE_PARTS): tp_primitives,

@@ -440,6 +440,8 @@ struct HostCase {
   // each sub-block, its first point in the parent, the parents' last indices
   int npar = 0;
   std::vector<int> parent, pstart, pdims;
+  // synthetic eddies of inflow 3 (tp_set_inflow_eddies), 10 doubles each
+  std::vector<double> eddies;
 };
 
 struct DevList {
@@ -458,7 +460,7 @@ struct ColGroup {
 };
 
 struct ListSet {
-  DevList points, sponge, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
+  DevList points, sponge, force, lines[3], exch, gprep, gapply[3], gcopy, wall, acap, aapply[3], normals;
   DevList surf;                     // wall-surface points, 256 per item
   std::vector<ColGroup> cg[3][2];   // line-solve launches per direction and kind
   std::vector<int> wr;   // first point-work item of each linked case, then the end
@@ -842,11 +844,19 @@ void enqueue_flux_lines(tp_context *c) {
 
 // sponge with the inflow target, src/SingleTurbulence3D.f90:332-374
 void enqueue_sponge(tp_context *c) {
-  if (!c->L.sponge.n) return;
-  Timed t(c, K_MISC, 0.0);
   StageArgs s = base_args(c);
-  s.work = c->L.sponge.d;
-  launch_point(PK_SPONGE, c->L.sponge.n, c->stream, s);
+  if (c->L.sponge.n) {
+    Timed t(c, K_MISC, 0.0);
+    s.work = c->L.sponge.d;
+    launch_point(PK_SPONGE, c->L.sponge.n, c->stream, s);
+  }
+  // the body force, forcego (src/rpt.f90:758-769), which src/Main3D.f90:776
+  // applies right after the sponge
+  if (c->L.force.n) {
+    Timed t(c, K_MISC, 0.0);
+    s.work = c->L.force.d;
+    launch_point(PK_FORCE, c->L.force.n, c->stream, s);
+  }
 }
 
 // characteristic boundary and interface conditions, src/Main3D.f90:778-869
@@ -1137,6 +1147,19 @@ void inflow_select(const HostCase &h, const double *x, const double *y, const st
 
 int inflow_nmodes(const HostCase &h) { return h.coef.inflow == 2 ? 1 : h.ip[TP_I_NMODES]; }
 
+// forcing points of a block and their squared distance from the forcing
+// centre (forceup, src/rpt.f90:731-740): the points within rfor of (xfor, yfor)
+void force_select(const HostCase &h, const Block &b, const double *x, const double *y, std::vector<int> &lst,
+                  std::vector<double> &r2) {
+  lst.clear(); r2.clear();
+  if (h.ip[TP_I_FORCING] != 1) return;
+  const double xf = h.dp[TP_D_XFOR], yf = h.dp[TP_D_YFOR], rf = h.dp[TP_D_RFOR], ra2 = rf * rf;
+  for (int l = 0; l < b.N; l++) {
+    const double r = (x[l] - xf) * (x[l] - xf) + (y[l] - yf) * (y[l] - yf);
+    if (r - ra2 < 0) { lst.push_back(l); r2.push_back(r); }
+  }
+}
+
 // device bytes a case takes once admitted: block arenas, chunked-solver
 // tables, and its sponge and inflow arrays (from the host grid)
 size_t case_bytes(const HostCase &h) {
@@ -1152,6 +1175,12 @@ size_t case_bytes(const HostCase &h) {
     std::vector<double> sg, lm, atz;
     sponge_select(h, lb, b, x, y, z, lst, sg, lm);
     if (!lst.empty()) total += acct(4 * lst.size()) + 2 * acct(8 * lst.size());
+    {
+      std::vector<int> fl;
+      std::vector<double> fr;
+      force_select(h, b, x, y, fl, fr);
+      if (!fl.empty()) total += acct(4 * fl.size()) + acct(24 * fl.size());
+    }
     if (h.coef.inflow) {
       inflow_select(h, x, y, lst, gix, pts, atz);
       total += acct(4 * std::max<size_t>(gix.size(), 1));
@@ -1235,7 +1264,7 @@ int build_case(HostCase &h) {
 
 // launch tables over the global blocks gs (sorted by case)
 int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
-  std::vector<Work> pts, sp, lw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm, sf;
+  std::vector<Work> pts, sp, fo, lw[3], ex, gp, ga[3], gc, wl, ac, aa[3], nm, sf;
   L.wr.assign(c->linked.size() + 1, 0);
   L.wwr.assign(c->linked.size() + 1, 0);
   int slot = -1;
@@ -1250,6 +1279,7 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
     const int CH = 256;
     for (int a0 = 0; a0 < b.N; a0 += CH) pts.push_back({g, a0, std::min(CH, b.N - a0), 0});
     for (int a0 = 0; a0 < b.nsponge; a0 += 256) sp.push_back({g, a0, std::min(256, b.nsponge - a0), 0});
+    for (int a0 = 0; a0 < b.nforce; a0 += 256) fo.push_back({g, a0, std::min(256, b.nforce - a0), 0});
     for (int d = 0; d < 3; d++) {
       const int nlines = b.N / b.L[d], nl = c->nlp[d] - 1;
       for (int q = 0; q < nlines; q += nl) lw[d].push_back({g, q, std::min(nl, nlines - q), d});
@@ -1288,6 +1318,7 @@ int make_lists(tp_context *c, const std::vector<int> &gs, Mem &m, ListSet &L) {
   if ((rc = upload_list(m, L.surf, sf))) return rc;
   if ((rc = upload_list(m, L.points, pts))) return rc;
   if ((rc = upload_list(m, L.sponge, sp))) return rc;
+  if ((rc = upload_list(m, L.force, fo))) return rc;
   if ((rc = upload_list(m, L.exch, ex))) return rc;
   if ((rc = upload_list(m, L.gprep, gp))) return rc;
   if ((rc = upload_list(m, L.gcopy, gc))) return rc;
@@ -1479,6 +1510,69 @@ int32_t tp_inflow_modes(int32_t nmodes, uint64_t seed, double tu, double lturb, 
 }
 
 
+int32_t tp_eddy_params(int32_t nits, const double *randnum, const double *deck, double *out10) {
+  if (nits < 1 || !randnum || !deck || !out10) return fail(TP_E_ARG, "tp_eddy_params: bad arguments");
+  // deck: span, slit, gaus, cutlb, domlen, szth1, then tl0, tl1, tlw of the
+  // two eddy types, then tam(1:3) of each
+  const double span = deck[0], slit = deck[1], gaus = deck[2], cutlb = deck[3], domlen = deck[4],
+               szth1 = deck[5];
+  const double tl0[2] = {deck[6], deck[9]}, tl1[2] = {deck[7], deck[10]}, tlw[2] = {deck[8], deck[11]};
+  const double tam[2][3] = {{deck[12], deck[13], deck[14]}, {deck[15], deck[16], deck[17]}};
+  const double tlb = std::max(std::max(tl0[0], tl0[1]), std::max(tl1[0], tl1[1]));
+  // randnum.dat in file order: ran(:,1), ait(:,1), ran(:,2), ait(:,2),
+  // ran(:,3), ait(:,3), yit(:), zit(:), src/SingleTurbulence3D.f90:88-93
+  const size_t n = (size_t)nits;
+  std::vector<double> ran(3 * n), ait(3 * n), sit(3 * n), xit(n), yit(n), zit(n);
+  std::vector<int> mxc(n);
+  for (int m = 0; m < 3; m++)
+    for (size_t q = 0; q < n; q++) {
+      ran[m * n + q] = randnum[(2 * m) * n + q];
+      ait[m * n + q] = randnum[(2 * m + 1) * n + q];
+    }
+  for (size_t q = 0; q < n; q++) { yit[q] = randnum[6 * n + q]; zit[q] = randnum[7 * n + q]; }
+  // eddy type, sizes and amplitudes (:95-102)
+  for (size_t q = 0; q < n; q++) mxc[q] = ran[q] <= gaus ? 0 : 1;
+  double res = std::pow(2.0 * (2.0 - cutlb) * tlb * span * slit / nits, 1.0 / 3.0);
+  for (int m = 0; m < 3; m++)
+    for (size_t q = 0; q < n; q++) {
+      const int t = mxc[q];
+      double &r = ran[m * n + q];
+      r = tl0[t] + (tl1[t] - tl0[t]) * std::pow(r, tlw[t]);
+      sit[m * n + q] = 1.0 / std::pow(r, 2.0);
+      ait[m * n + q] = res * tam[t][m] * (2.0 * ait[m * n + q] - 1.0) * std::pow(sit[m * n + q], 1.5);
+    }
+  // centres: a train along x, each eddy spaced by the sizes of its
+  // neighbours, over one slit upstream of the inflow sponge edge (:103-107)
+  double sr = 0;
+  for (size_t q = 0; q < n; q++) sr = sr + ran[q];
+  const double denxit = slit / (2.0 * sr);
+  xit[0] = -domlen + 0.5 * (szth1 + slit) - denxit * ran[0];
+  for (size_t q = 1; q < n; q++) xit[q] = xit[q - 1] - denxit * (ran[q - 1] + ran[q]);
+  for (size_t q = 0; q < n; q++) {
+    res = 3.0 * span - 2.0 * std::max(std::max(ran[q], ran[n + q]), ran[2 * n + q]);
+    yit[q] = (1.0 - cutlb) * ran[n + q] * (2.0 * yit[q] - 1.0);
+    zit[q] = std::min(span, res) * (zit[q] - 0.5);
+  }
+  for (size_t q = 0; q < n; q++) {
+    double *o = out10 + 10 * q;
+    o[0] = xit[q]; o[1] = yit[q]; o[2] = zit[q];
+    for (int m = 0; m < 3; m++) { o[3 + m] = sit[m * n + q]; o[6 + m] = ait[m * n + q]; }
+    o[9] = mxc[q];
+  }
+  return 0;
+}
+
+int32_t tp_set_inflow_eddies(tp_context *c, int32_t cs, int32_t nits, const double *eddies10) {
+  if (!c) return fail(TP_E_ARG, "null context");
+  const int ci = case_index(c, cs);
+  if (ci < 0 || nits < 1 || !eddies10) return fail(TP_E_ARG, "tp_set_inflow_eddies: bad arguments");
+  HostCase &h = c->cases[ci];
+  if (h.coef.inflow != 3) return fail(TP_E_STATE, "tp_set_inflow_eddies: the case's TP_I_INFLOW is not 3");
+  if (h.geom) return fail(TP_E_STATE, "tp_set_inflow_eddies: the case's geometry is set up already");
+  h.eddies.assign(eddies10, eddies10 + 10 * (size_t)nits);
+  return 0;
+}
+
 const char *tp_last_error(void) { return g_err.c_str(); }
 
 int32_t tp_create(tp_context **out) {
@@ -1638,7 +1732,12 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
   k.les = iparams[TP_I_LES]; k.nextgcic = iparams[TP_I_NEXTGCIC];
   k.span = dparams[TP_D_SPAN];
   k.inflow = iparams[TP_I_INFLOW];
-  if (k.inflow < 0 || k.inflow > 2) return fail(TP_E_ARG, "TP_I_INFLOW must be 0, 1 or 2");
+  if (k.inflow < 0 || k.inflow > 3) return fail(TP_E_ARG, "TP_I_INFLOW must be 0, 1, 2 or 3");
+  k.forcing = iparams[TP_I_FORCING];
+  if (k.forcing != 0 && k.forcing != 1) return fail(TP_E_ARG, "TP_I_FORCING must be 0 or 1");
+  if (k.forcing == 1 && !(dparams[TP_D_RFOR] > 0 && dparams[TP_D_SPAN] > 0))
+    return fail(TP_E_ARG, "forcing needs TP_D_RFOR > 0 and TP_D_SPAN > 0");
+  k.amfor = dparams[TP_D_AMFOR]; k.tsfor = dparams[TP_D_TSFOR]; k.tefor = dparams[TP_D_TEFOR];
   if (k.inflow) {
     k.ntz = iparams[TP_I_NTZ];
     if (k.ntz < 4) return fail(TP_E_ARG, "inflow table needs TP_I_NTZ >= 4 time intervals");
@@ -1648,6 +1747,8 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
       if (iparams[TP_I_NMODES] < 1) return fail(TP_E_ARG, "inflow needs TP_I_NMODES >= 1");
       if (!(slit > 0) || !(dparams[TP_D_LTURB] > 0))
         return fail(TP_E_ARG, "turbulent inflow needs TP_D_SLIT > 0 and TP_D_LTURB > 0");
+    } else if (k.inflow == 3) {
+      if (!(slit > 0)) return fail(TP_E_ARG, "eddy inflow needs TP_D_SLIT > 0");
     } else {
       if (!(dparams[TP_D_GUSTK] > 0)) return fail(TP_E_ARG, "gust inflow needs TP_D_GUSTK > 0");
       double m8[8];
@@ -1703,6 +1804,8 @@ int32_t tp_setup_geometry(tp_context *c) {
       if ((int)h.grid.size() <= lb || h.grid[lb].empty())
         return fail(TP_E_STATE, "grid of case " + std::to_string(h.id) + " block " + std::to_string(lb) +
                                     " not set");
+    if (h.coef.inflow == 3 && h.eddies.empty())
+      return fail(TP_E_STATE, "case " + std::to_string(h.id) + ": eddy inflow needs tp_set_inflow_eddies");
     todo.push_back((int)i);
     for (int lb = 0; lb < h.nb; lb++) gs.push_back(h.block0 + lb);
   }
@@ -1798,6 +1901,32 @@ int32_t tp_setup_geometry(tp_context *c) {
         if ((rc = upload_vec(h.mem, &b.bsz, bs))) return rc;
       }
       b.nsponge = (int)lst.size();
+      // forcing coefficients (forceup, src/rpt.f90:741-751): three spanwise
+      // modes with a Gaussian weight; as in the reference, ra1 holds 2 pi /
+      // span only for the first point, the loop overwriting it with each
+      // point's weight, which the next point's modes take as wavenumber
+      {
+        std::vector<int> fl;
+        std::vector<double> fr;
+        force_select(h, b, x, y, fl, fr);
+        b.nforce = (int)fl.size();
+        if (b.nforce) {
+          const double rf = h.dp[TP_D_RFOR];
+          const double ra0 = -std::log(0.0001) / (rf * rf);
+          double ra1 = 2 * kPi / h.dp[TP_D_SPAN];
+          const size_t nf = fl.size();
+          std::vector<double> xa(3 * nf);
+          for (size_t q = 0; q < nf; q++) {
+            const int l = fl[q];
+            const double zz = z[l];
+            const double b1 = std::cos(ra1 * zz), b2 = std::cos(3 * ra1 * zz), b3 = std::cos(4 * ra1 * zz);
+            ra1 = 0.5 * std::exp(-ra0 * fr[q]) / yaco[l];
+            xa[q] = ra1 * b1; xa[nf + q] = ra1 * b2; xa[2 * nf + q] = ra1 * b3;
+          }
+          if ((rc = upload_vec(h.mem, &b.lcfor, fl))) return rc;
+          if ((rc = upload_vec(h.mem, &b.xafor, xa))) return rc;
+        }
+      }
       if (h.coef.inflow) {
         std::vector<int> gix, pts;
         std::vector<double> atz, pos;
@@ -1810,6 +1939,24 @@ int32_t tp_setup_geometry(tp_context *c) {
             pos[q] = x[pts[q]]; pos[b.ngust + q] = y[pts[q]]; pos[2 * (size_t)b.ngust + q] = z[pts[q]];
           }
           if ((rc = upload_vec(h.mem, &b.atz, atz))) return rc;
+          if (h.coef.inflow == 3) {
+            // the reference's synthetic eddies, src/SingleTurbulence3D.f90:262-298
+            if ((rc = dalloc(h.mem, (void **)&b.vito, sizeof(double) * 3 * (h.coef.ntz + 1) * (size_t)b.ngust)))
+              return rc;
+            Mem tmp;
+            double *ded = nullptr, *dpos = nullptr;
+            if ((rc = upload_vec(tmp, &ded, h.eddies)) || (rc = upload_vec(tmp, &dpos, pos))) {
+              tmp.release();
+              return rc;
+            }
+            launch_eddy_table(ded, (int)(h.eddies.size() / 10), dpos, b.ngust, h.coef.ntz, h.coef.fctr, h.coef.uoo,
+                              h.dp[TP_D_SLIT], h.dp[TP_D_SPAN], h.dp[TP_D_CFIT], b.vito, c->stream);
+            const hipError_t e = hipStreamSynchronize(c->stream);
+            tmp.release();
+            if (e != hipSuccess) return fail(TP_E_HIP, std::string("eddy table: ") + hipGetErrorString(e));
+            HIPCHK(hipGetLastError());
+            continue;   // what follows builds the Fourier-mode table of inflow 1 and 2
+          }
           const int nm = inflow_nmodes(h);
           std::vector<double> modes(8 * (size_t)nm);
           if (h.coef.inflow == 2) {

@@ -51,6 +51,9 @@ struct CaseCoef {
   int inflow, ntz;
   double tgusto, period, fctr;   // table period slit/uoo(1) and spacing slit/(ntz uoo(1))
   double span;                   // spanwise extent (force coefficients' reference area per chord)
+  // body force (forcego, src/rpt.f90:758-769)
+  int forcing, pad2;
+  double amfor, tsfor, tefor;
 };
 
 // ---- device view of one sub-block ----
@@ -79,6 +82,9 @@ struct Block {
   int *gix;                   // per sponge point: its inflow index, -1 if none
   double *atz;                // per inflow point: ramp sin^2 of the reference
   double *vito;               // [3][ntz+1][ngust] inflow velocity table
+  int nforce;                 // forcing points (forceup, src/rpt.f90:728-753)
+  int *lcfor;                 // their point indices
+  double *xafor;              // [3][nforce] modal coefficients
   // wall surface (the eta face that is a wall): points ll = k*nx + i
   int nsurf, sside;           // wall points (0: none), side (0: j = 0, 1: j = ny-1)
   double *sarea;              // [nsurf] cell areas (src/rpt.f90:774-830, wallArea)
@@ -255,6 +261,8 @@ struct StageArgs {
 // launchers (tp_line.hip, tp_col.hip, tp_stage.hip)
 void launch_inflow_table(const double *modes, const double *rot, int nmodes, const double *pos, int ng, int ntz,
                          double fctr, const double *uoo, double *vito, hipStream_t st);
+void launch_eddy_table(const double *eddies, int nits, const double *pos, int ng, int ntz, double fctr,
+                       const double *uoo, double slit, double span, double cfit, double *vito, hipStream_t st);
 // chunked line solves: P waves per 64 lines, chunk capacity C points per lane
 int col_capacity(int chunk);
 size_t col_lds_bytes(int d, int P);
@@ -277,7 +285,7 @@ void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, c
 // host: record the message of a failure (tp_last_error) and return its code
 int set_error(int code, const std::string &msg);
 
-enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_SPONGE = 6, PK_RK, PK_COPYF = 9, PK_INIT, PK_METRICS,
+enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_FORCE = 5, PK_SPONGE = 6, PK_RK, PK_COPYF = 9, PK_INIT, PK_METRICS,
                    PK_FLUXALL = 14, PK_STRESS = 15, PK_FLUX_TV = 16 };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,
                   FK_SURF_SETUP, FK_SURF_SAMPLE, FK_GCBC_APPLY_XI };

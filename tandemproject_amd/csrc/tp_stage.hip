@@ -467,6 +467,77 @@ __global__ void __launch_bounds__(256) k_sponge(StageArgs a) {
   }
 }
 
+// ---- body force, forcego (src/rpt.f90:758-769): between tsfor and tefor,
+// three spanwise modes with temporal cosines added to the x-momentum
+// residual and taken from the y-momentum residual at the forcing points ----
+__global__ void __launch_bounds__(256) k_force(StageArgs a) {
+  WORK_POINTS
+  const CaseCoef &c = a.coef[b.cs];
+  const double *sc = a.sc + b.cs * S_STRIDE;
+  const double t = sc[S_TIMO] + sc[S_DTK];
+  if (!(c.forcing == 1 && t > c.tsfor && t < c.tefor)) return;
+  const double ra0 = t - c.tsfor;
+  const double ra1 = c.amfor * cos(ra0 * 48.76 * c.amachoo) / 3;
+  const double ra2 = c.amfor * cos(ra0 * 53.6 * c.amachoo) / 3;
+  const double ra3 = c.amfor * cos(ra0 * 53.6 * c.amachoo) / 3;
+  const int nf = b.nforce;
+  for (int q = w.a + threadIdx.x; q < w.a + w.n; q += blockDim.x) {
+    const int l = b.lcfor[q];
+    const double f = ra1 * b.xafor[q] + ra2 * b.xafor[nf + q] + ra3 * b.xafor[2 * nf + q];
+    b.de[N + l] = b.de[N + l] + b.qa[l] * f;
+    b.de[2 * N + l] = b.de[2 * N + l] - b.qa[l] * f;
+  }
+}
+
+// ---- inflow velocity table from the reference's synthetic eddies
+// (src/SingleTurbulence3D.f90:268-291): each eddy nn and its images one slit
+// upstream and one span to either side, convected with uoo to the table
+// time tt(ii), induce rr x d at the point (d its offset from the centre,
+// rr_m = ait_m (ra1 r^2) exp(ve_m r^4) (1 + ra2 ve_m r^4)); the sum, in the
+// reference's order, is scaled by cfit.  One thread per (point, time). ----
+__global__ void __launch_bounds__(256) k_eddy_table(const double *eddies, int nits, const double *pos, int ng,
+                                                    int ntz, double fctr, double u0, double u1, double u2, double slit,
+                                                    double span, double cfit, double *vito) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nt1 = ntz + 1;
+  if (e >= ng * nt1) return;
+  const int g = e % ng, ii = e / ng;
+  const double px = pos[g], py = pos[ng + g], pz = pos[2 * (size_t)ng + g];
+  const double tt = ii * fctr;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  for (int nn = 0; nn < nits; nn++) {
+    const double *E = eddies + 10 * (size_t)nn;
+    const double mxc = E[9];
+    const double ve0 = (-9 + mxc) * E[3] * E[3], ve1 = (-9 + mxc) * E[4] * E[4], ve2 = (-9 + mxc) * E[5] * E[5];
+    const double ra1 = -36 - 60 * mxc, ra2 = (2.0 / 3.0) * mxc;
+    for (int i = 0; i <= 1; i++)
+      for (int k = -1; k <= 1; k++) {
+        const double rv0 = (E[0] - i * slit) + tt * u0, rv1 = E[1] + tt * u1, rv2 = (E[2] - k * span) + tt * u2;
+        const double d1 = px - rv0, d2 = py - rv1, d3 = pz - rv2;
+        const double d4 = d1 * d1 + d2 * d2 + d3 * d3;
+        double d5 = d4 * d4;
+        double r1 = ve0 * d5, r2 = ve1 * d5, r3 = ve2 * d5;
+        d5 = ra1 * d4;
+        r1 = E[6] * d5 * exp(r1) * (1 + ra2 * r1);
+        r2 = E[7] * d5 * exp(r2) * (1 + ra2 * r2);
+        r3 = E[8] * d5 * exp(r3) * (1 + ra2 * r3);
+        v0 = v0 + r2 * d3 - r3 * d2;
+        v1 = v1 + r3 * d1 - r1 * d3;
+        v2 = v2 + r1 * d2 - r2 * d1;
+      }
+  }
+  vito[((size_t)0 * nt1 + ii) * ng + g] = cfit * v0;
+  vito[((size_t)1 * nt1 + ii) * ng + g] = cfit * v1;
+  vito[((size_t)2 * nt1 + ii) * ng + g] = cfit * v2;
+}
+
+void launch_eddy_table(const double *eddies, int nits, const double *pos, int ng, int ntz, double fctr,
+                       const double *uoo, double slit, double span, double cfit, double *vito, hipStream_t st) {
+  const int tot = ng * (ntz + 1);
+  hipLaunchKernelGGL(k_eddy_table, dim3((tot + 255) / 256), dim3(256), 0, st, eddies, nits, pos, ng, ntz, fctr,
+                     uoo[0], uoo[1], uoo[2], slit, span, cfit, vito);
+}
+
 // ---- inflow velocity table: frozen turbulence convected with uoo, sampled
 // at the inflow points at the table times tt(i) = i*fctr (the reference fills
 // vito(ll, i, :) the same way from its eddies, SingleTurbulence3D.f90:262-292) ----
@@ -1061,6 +1132,7 @@ void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
   switch (which) {
     case 0: LAUNCH(k_primitives, nwg, 256, st, a); break;
     case 1: LAUNCH(k_dt_reduce, nwg, 256, st, a); break;
+    case 5: LAUNCH(k_force, nwg, 256, st, a); break;
     case 6: LAUNCH(k_sponge, nwg, 256, st, a); break;
     case 7: LAUNCH(k_rk, nwg, 256, st, a); break;
     case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;

@@ -20,8 +20,8 @@ module tandem_hip
   ! integer case parameters, iparams(1:16) (0-based slot + 1)
   integer, parameter :: TP_I_NKRK = 1, TP_I_NVISCOUS = 2, TP_I_NSMF = 3, TP_I_LES = 4, &
                         TP_I_NEXTRABC = 5, TP_I_NEXTGCIC = 6, TP_I_INFLOW = 7, TP_I_NMODES = 8, &
-                        TP_I_SEED = 9, TP_I_NTZ = 10, TP_I_NPARAM = 16
-  ! double case parameters, dparams(1:32)
+                        TP_I_SEED = 9, TP_I_NTZ = 10, TP_I_FORCING = 11, TP_I_NPARAM = 16
+  ! double case parameters, dparams(1:48)
   integer, parameter :: TP_D_REOO = 1, TP_D_TEMPOO = 2, TP_D_AMACH1 = 3, TP_D_AMACH2 = 4, &
                         TP_D_AMACH3 = 5, TP_D_WTEMP = 6, TP_D_CFL = 7, TP_D_TMAX = 8, &
                         TP_D_TIMF = 9, TP_D_FLTK = 10, TP_D_FLTKBC = 11, TP_D_DTO = 12, &
@@ -30,7 +30,8 @@ module tandem_hip
                         TP_D_SZTH2 = 21, TP_D_SZXT = 22, TP_D_SZCO = 23, TP_D_SPAN = 24, &
                         TP_D_WLEW = 25, TP_D_WLEA = 26, TP_D_TGUSTO = 27, TP_D_SLIT = 28, &
                         TP_D_TU = 29, TP_D_LTURB = 30, TP_D_GUSTW = 31, TP_D_GUSTK = 32, &
-                        TP_D_NPARAM = 32
+                        TP_D_CFIT = 33, TP_D_AMFOR = 34, TP_D_XFOR = 35, TP_D_YFOR = 36, &
+                        TP_D_RFOR = 37, TP_D_TSFOR = 38, TP_D_TEFOR = 39, TP_D_NPARAM = 48
   ! array ids of tp_upload / tp_download (names of src/MainVar3D.f90:78-102)
   integer(c_int32_t), parameter :: TP_QA = 0, TP_QO = 1, TP_DE = 2, TP_RR = 3, TP_SS = 4, &
                                    TP_XIM = 5, TP_ETM = 6, TP_ZEM = 7, TP_P = 8, TP_YACO = 9, &
@@ -194,6 +195,20 @@ module tandem_hip
       integer(c_int32_t), value :: lxi, let, lze, nn
       integer(c_int32_t), intent(in) :: nbc2(2)
       type(c_ptr), value :: src, dst, stream
+    end function
+    ! the reference's synthetic eddies (inflow 3)
+    integer(c_int32_t) function tp_eddy_params(nits, randnum, deck, out10) bind(C, name="tp_eddy_params")
+      import :: c_int32_t, c_double
+      integer(c_int32_t), value :: nits
+      real(c_double), intent(in) :: randnum(*), deck(18)
+      real(c_double), intent(out) :: out10(*)
+    end function
+    integer(c_int32_t) function tp_set_inflow_eddies(ctx, case_id, nits, eddies10) &
+        bind(C, name="tp_set_inflow_eddies")
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: case_id, nits
+      real(c_double), intent(in) :: eddies10(*)
     end function
     ! the checks of tp_add_case without a context
     integer(c_int32_t) function tp_check_case(nblocks, dims, nbc, ncd, njct, jct, jvel, iparams) &

@@ -7,7 +7,7 @@
 !   run_case <case file> <result file>
 ! case file (stream, native byte order): nb, njct, dims(3 nb), nbc(6 nb),
 ! ncd(6 nb), nsz(6 nb), jct(12 njct), jvel(2 njct), iparams(16) as int32;
-! dparams(32) as real64; per sub-block x, y, z (real64); nsteps, exact lines
+! dparams(48) as real64; per sub-block x, y, z (real64); nsteps, exact lines
 ! (int32).  Result file: qa(0:lmx, 1:5) of each sub-block in order (real64).
 program run_case
   use iso_c_binding, only: c_ptr, c_int32_t, c_int64_t, c_double
@@ -17,7 +17,7 @@ program run_case
   type(c_ptr) :: ctx
   integer(c_int32_t) :: nb, njct, cid, nsteps, exact, ip(16)
   integer(c_int32_t), allocatable :: dims(:), nbc(:), ncd(:), nsz(:), jct(:), jvel(:)
-  real(c_double) :: dp(32)
+  real(c_double) :: dp(48)
   real(c_double), allocatable :: xyz(:), qa(:)
   integer :: b, n, u
 

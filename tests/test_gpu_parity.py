@@ -505,3 +505,83 @@ def test_profile_classes():
     for k in ("deriv", "filter", "point", "flux", "gcbc", "rk"):
         assert got[k][1] > 0, k
     assert got["deriv"][1] == 4 * 6
+
+
+def eddy_inputs(p, nits=24):
+    """The reference's synthetic eddies from its own random numbers
+    (tests/golden/randnum_head.json) on this case's domain, with non-zero
+    amplitudes (the shipped deck's are zero), and the band half-width the
+    reference derives, tlb (2.5 - cutlb) (src/SingleTurbulence3D.f90:255)."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "randnum_head.json")))
+    d = dict(ref["deck"])
+    d.update(span=p["span"], slit=2.0, domlen=p["domlen"], szth1=p["szth1"],
+             tam=[[0.05, 0.05, 0.05], [0.03, 0.03, 0.03]])
+    eddies = api.eddy_params(nits, np.array(ref["randnum"]), d)
+    tlb = max(max(d["tl0"]), max(d["tl1"]))
+    return eddies, dict(inflow=3, ntz=16, tgusto=0.0, slit=2.0, cfit=d["cfit"], gustw=tlb * (2.5 - d["cutlb"]))
+
+
+@LINE_MODES
+def test_inflow_eddies_steps(exact):
+    """Inflow from the reference's synthetic eddies (TP_I_INFLOW = 3): the
+    device table, interpolation and sponge target against the oracle over 40
+    steps, and the eddies do change the flow."""
+    say("inflow eddies, 40 steps,", "sequential" if exact else "chunked")
+    lay, ss, p = aerofoil_case()
+    eddies, over = eddy_inputs(p)
+    p = dict(p); p.update(over)
+    o = po.Oracle(lay, p)
+    o.set_inflow_eddies(eddies)
+    s = api.Solver([(lay, p)], exact_lines=exact)
+    s.set_inflow_eddies(0, eddies)
+    plain = api.Solver([(lay, dict(p, inflow=0))], exact_lines=exact)
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b]); s.set_grid(0, b, ss[b]); plain.set_grid(0, b, ss[b])
+    o.metrics(); s.setup_geometry(); plain.setup_geometry()
+    assert sum(o.array(b, "atz").size for b in range(lay.nb)) > 0
+    o.initialo(); s.initialo(0); plain.initialo(0)
+    o.step(40); s.step(40); plain.step(40)
+    s.sync()
+    compare_state(o, s, lay, 1e-10, acoustic=not exact)
+    moved = max(float(np.abs(s.download(0, b, "qa") - plain.download(0, b, "qa")).max()) for b in range(lay.nb))
+    say("  max change of the state by the eddies", moved)
+    assert moved > 1e-7
+
+
+@LINE_MODES
+def test_forcing_steps(exact):
+    """The reference's body force (forceup/forcego, src/rpt.f90:728-769)
+    switched on behind the aerofoil: 40 steps against the oracle."""
+    say("forcing, 40 steps,", "sequential" if exact else "chunked")
+    lay, ss, p = aerofoil_case()
+    p = dict(p)
+    p.update(forcing=1, amfor=0.05, xfor=1.6, yfor=0.1, rfor=0.5, tsfor=0.0, tefor=100.0)
+    o, s = run_steps(lay, ss, p, 40, exact=exact)
+    compare_state(o, s, lay, 1e-10, acoustic=not exact)
+    ref = api.Solver([(lay, dict(p, forcing=0))], exact_lines=exact)
+    for b in range(lay.nb):
+        ref.set_grid(0, b, ss[b])
+    ref.setup_geometry(); ref.initialo(0); ref.step(40); ref.sync()
+    moved = max(float(np.abs(s.download(0, b, "qa") - ref.download(0, b, "qa")).max()) for b in range(lay.nb))
+    say("  max change of the state by the force", moved)
+    assert moved > 1e-7
+
+
+@LINE_MODES
+def test_pitching_frame_steps(exact):
+    """Pitching aerofoil (movef/pitch, src/Subroutineso.f90:222-259): the
+    frame velocity rotates from aoa0 to aoa1 over tp from tps; 100 steps
+    across the whole pitch window against the oracle."""
+    say("pitching frame, 100 steps,", "sequential" if exact else "chunked")
+    lay, ss, p = aerofoil_case()
+    p = dict(p)
+    p.update(tps=0.01, tp=0.04, aoa0=0.0, aoa1=6.0)
+    o, s = run_steps(lay, ss, p, 100, exact=exact)
+    so, sg = o.scalars(), s.scalars(0)
+    say("  time", so["timo"], "angle of attack", so["aoa"], sg["aoa"])
+    assert so["timo"] > p["tps"] + p["tp"]          # the whole window was crossed
+    assert abs(sg["aoa"] - 6.0) < 1e-12 and abs(so["aoa"] - sg["aoa"]) < 1e-12
+    assert abs(sg["umf2"]) > 0.01                  # the frame turned
+    compare_state(o, s, lay, 1e-10, acoustic=not exact)
