@@ -119,6 +119,39 @@ def ensemble_bench(ncase, nsteps, nplain):
                         "reduced frequency 0.5-8), 0.15-2 M points, nz 13-32, seed 4"}
 
 
+def tandem_bench(nsteps, warmup, tune):
+    """Batching within one case: the tandem configuration's 24 uneven blocks
+    (192 x 96 to 512 x 192 in-plane, nz = 64, 47.2 M points) in shared
+    launches, line-solve shapes autotuned, steps replayed from the captured
+    step graph."""
+    lay, ss, params, dims = W.tandem_45m()
+    npts = lay.npoints()
+    log(f"tandem: {lay.nb} blocks, {npts / 1e6:.2f} M points")
+    s = api.Solver([(lay, params)])
+    for b in range(lay.nb):
+        s.set_grid(0, b, ss[b])
+    s.setup_geometry()
+    tuned = s.autotune(5) if tune else None
+    s.initialo(0)
+    s.step(warmup, graph=True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(nsteps, graph=True)
+    s.sync()
+    dt = time.perf_counter() - t0
+    finite = all(bool(np.isfinite(s.download(0, b, "qa")).all()) for b in (0, lay.nb - 1))
+    s.close()
+    out = {"value": round(npts * nsteps / dt / 1e6, 2), "unit": "Mpoint-steps/s", "steps": nsteps,
+           "ms_per_step": round(dt / nsteps * 1e3, 3), "points": npts, "blocks": lay.nb, "graph": True,
+           "finite": finite,
+           "line_chunks": {f"{'xez'[d]}-{k}": c for (d, k), (c, _) in tuned.items()} if tuned else None,
+           "workload": "two NACA0012 in tandem 3c apart, 24 uneven blocks, in-plane 192x96 to 512x192 "
+                       f"(lxibk={dims['lxibk']}, letbk={dims['letbk']}), nz={dims['lzebk'][0] + 1}, M=0.3, "
+                       "Re_c=2e5, viscous, turbulent inflow (2048 modes)"}
+    log(f"tandem: {out['value']} Mpoint-steps/s ({out['ms_per_step']} ms per step)")
+    return out
+
+
 def serving_bench(nreq, nsteps, load, fill):
     """Poisson stream of served cases through the request loop: p50/p99 time
     to solution (arrival to final state on the host) and cases/h, against
@@ -139,9 +172,9 @@ def serving_bench(nreq, nsteps, load, fill):
     arr = np.cumsum(np.random.default_rng(55).exponential(1.0 / rate, nreq))
     arr -= arr[0]
     srv = api.Server(budget_bytes=0, max_active=64, graph=True, fill_points=fill)
-    t0 = srv.clock() + 0.05
-    reqs = [srv.submit(lay, p, ss, nsteps, t0 + a) for (lay, ss, p), a in zip(cases, arr)]
-    srv.run(max_steps=10)
+    # a client thread submits each request at its arrival into the running
+    # loop (the arrival is the submission call)
+    reqs = srv.run_stream([(lay, p, ss, nsteps, a) for (lay, ss, p), a in zip(cases, arr)], max_steps=10)
     st = srv.stats()
     diff = 0.0
     for r, qa in zip(reqs, qa_plain):
@@ -264,6 +297,8 @@ def main():
                     help="offered load as a multiple of the plain path's capacity")
     ap.add_argument("--serve-fill", type=float, default=24e6,
                     help="admission stops while the cases on the device hold this many points")
+    ap.add_argument("--tandem-steps", type=int, default=10,
+                    help="steps of the tandem (24 uneven blocks, 47 M points) measurement (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="after the run, one plain device-to-device copy of MB MiB (a reference kernel "
                          "of known traffic for scaling hardware byte counters)")
@@ -357,6 +392,7 @@ def main():
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
     s.close()
+    tandem = tandem_bench(args.tandem_steps, 3, not args.no_tune) if args.tandem_steps > 0 else None
     ens = ensemble_bench(args.ensemble_cases, args.ensemble_steps, args.ensemble_plain) \
         if args.ensemble_cases > 0 else None
     if ens:
@@ -381,7 +417,8 @@ def main():
                    "points": npts, "sub_blocks": lay.nb, "graph": bool(args.graph),
                    "line_chunks": line_chunks,
                    "ms_per_step_by_class": breakdown},
-        "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph_replay, "ensemble": ens, "serving": srv,
+        "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph_replay, "tandem": tandem, "ensemble": ens,
+        "serving": srv,
     }
     print(json.dumps(out))
 

@@ -234,6 +234,7 @@ int32_t tp_sync(tp_context *ctx);
  *   steps. */
 enum { TP_OPT_EXACT_LINES = 0, TP_OPT_CHUNKS = 1, TP_OPT_STRESSES = 2 };
 int32_t tp_set_option(tp_context *ctx, int32_t option, int64_t value);
+int32_t tp_get_option(tp_context *ctx, int32_t option, int64_t *value);
 
 /* Time every usable line-solver configuration (sequential kernel and P =
  * 1..16 chunks) on the context's own blocks, reps launches each, and keep
@@ -382,19 +383,33 @@ int32_t tp_remove_case(tp_context *ctx, int32_t case_id);
 /* the same for n cases at once (one rebuild of the launch tables); nothing
  * is removed if any id is unknown */
 int32_t tp_remove_cases(tp_context *ctx, int32_t n, const int32_t *case_ids);
+/* move n cases that are set up on the device (geometry done) from their
+ * contexts srcs[i] into dst, on the same device, without copying or
+ * recomputing anything: each keeps its device memory, state and scalars and
+ * gets the handle new_ids[i] in dst (one rebuild of dst's launch tables).
+ * A host can thus set cases up in contexts of their own on other host
+ * threads and streams while dst steps, and join them between steps. */
+int32_t tp_adopt_cases(tp_context *dst, int32_t n, tp_context *const *srcs, const int32_t *case_ids,
+                       int32_t *new_ids);
 /* out[0] state (0 pending, 1 on the device, 2 with geometry), out[1] points,
  * out[2] sub-blocks, out[3] device bytes held */
 int32_t tp_case_info(tp_context *ctx, int32_t case_id, int64_t *out4);
 
 /* ---- request loop (no counterpart in the reference) ----
  * A server owns a context.  Requests -- whole cases with their grids and a
- * step count -- queue on the host with an arrival time.  Each iteration
- *   1. admits arrived requests in arrival order while a slot is free, the
+ * step count -- queue on the host with an arrival time.  A staging thread
+ * sets arrived requests up, in arrival order, in contexts of their own
+ * (device memory, metrics, sponge, inflow table, the uniform initial state)
+ * while their device bytes fit the HBM budget beside the cases on the device
+ * and fewer than tp_server_set_staging requests wait set up; this runs on
+ * its own streams, beside the stepping.  Each iteration
+ *   1. admits set-up requests in arrival order while a slot is free and the
  *      cases on the device hold fewer points than the fill target
- *      (tp_server_set_fill) and the case's device bytes (tp_case_bytes) fit
- *      the HBM budget (a request that does not fit the budget is passed over
- *      for later, smaller ones), sets them up on the device and gives them
- *      the uniform initial state;
+ *      (tp_server_set_fill), joining them to the cases on the device
+ *      (tp_adopt_cases: no copy, no recomputation); an idle device first
+ *      waits until the requests that have arrived are set up, or until those
+ *      set up fill the slots, the fill target or the staging window, so that
+ *      a burst runs as one batch;
  *   2. advances every case on the device by the same number of steps (at
  *      most max_steps, at most what any of them still needs) in shared
  *      launches (captured step graph when use_graph);
@@ -408,15 +423,25 @@ typedef struct tp_server tp_server;
 int32_t tp_server_create(int64_t budget_bytes, int32_t max_active, int32_t use_graph, tp_server **srv);
 int32_t tp_server_destroy(tp_server *srv);
 tp_context *tp_server_context(tp_server *srv);   /* for tp_set_option / tp_autotune */
+/* requests set up ahead of their admission (default: max_active): each
+ * request is set up in a context of its own (allocation, metrics, sponge, inflow table,
+ * initial state) on a host thread and stream of its own while the cases on
+ * the device step, and joins them between steps (tp_adopt_cases).  Staged
+ * cases hold their device bytes within the budget. */
+int32_t tp_server_set_staging(tp_server *srv, int32_t ahead);
 /* admit no further request while the cases on the device hold at least
  * `points` grid points (one case is always admitted; 0: no limit, the
  * default).  Enough points to fill the GPU keep its throughput while large
  * cases finish one after another instead of sharing it. */
 int32_t tp_server_set_fill(tp_server *srv, int64_t points);
+/* message of the last failed server call on the calling thread */
 const char *tp_server_error(tp_server *srv);
 double tp_server_clock(tp_server *srv);
 /* queue a case (arguments of tp_add_case, then xyz[b] = grid of sub-block b);
- * t_arrival < 0: now.  Fails with TP_E_SIZE if the case alone exceeds the budget. */
+ * t_arrival < 0: the time of the call.  Fails with TP_E_SIZE if the case
+ * alone exceeds the budget.  May be called from another host thread while
+ * tp_server_run or tp_server_iterate runs (a client submitting into a
+ * running loop); the other server calls belong to one thread. */
 int32_t tp_server_submit(tp_server *srv, int32_t nblocks, const int32_t *dims, const int32_t *nbc,
                          const int32_t *ncd, const int32_t *nsz, int32_t njct, const int32_t *jct,
                          const int32_t *jvel, const int32_t *iparams, const double *dparams,
@@ -424,7 +449,8 @@ int32_t tp_server_submit(tp_server *srv, int32_t nblocks, const int32_t *dims, c
 /* one iteration; status6 = on the device, queued (arrived), not yet arrived,
  * finished, admitted in this iteration, steps made */
 int32_t tp_server_iterate(tp_server *srv, int32_t max_steps, int32_t *status6);
-/* iterate until every request has finished, idling until arrivals */
+/* iterate until every request submitted so far has finished, idling until
+ * arrivals */
 int32_t tp_server_run(tp_server *srv, int32_t max_steps);
 /* info6 = state (0 queued, 1 on the device, 2 finished, 3 failed), arrival,
  * admission and finish times, steps made, device bytes; qa (may be NULL):
@@ -439,6 +465,10 @@ int64_t tp_server_result_count(tp_server *srv, int32_t req);
  * device, admissions that joined running cases, time spent stepping, in
  * admissions, in retirements, in step-graph captures, number of captures */
 int32_t tp_server_stats(tp_server *srv, double *out16);
+/* out6 = time spent setting requests up (staging thread), requests set up,
+ * set-ups that failed, most requests set up at once, iterations stepped
+ * eagerly because a set-up held the step-graph capture off, staging window */
+int32_t tp_server_stats2(tp_server *srv, double *out6);
 
 /* ---- measurement ---- */
 /* record HIP events around every launch (on the context's stream) */

@@ -91,6 +91,9 @@ def lib():
         L.orc_set_inflow_modes.argtypes = [P, ctypes.c_int, dp]
         L.orc_inflow_weights.restype = ctypes.c_int
         L.orc_inflow_weights.argtypes = [P, ctypes.c_double, ctypes.c_double, ip, dp, dp]
+        L.orc_pow15.argtypes = [ctypes.c_double]
+        L.orc_pow15.restype = ctypes.c_double
+        L.orc_sincos.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -257,6 +260,18 @@ class Oracle:
         lib().orc_coefficients(self.h, _dp(pbci), _dp(pbco), _dp(pbcot), _dp(albef),
                                _dp(fabc), _dp(sbci))
         return dict(pbci=pbci, pbco=pbco, pbcot=pbcot, albef=albef, fabc=fabc, sbci=sbci)
+
+
+def pow15(x):
+    """x**1.5 correctly rounded (the oracle's viscosity law)."""
+    return lib().orc_pow15(float(x))
+
+
+def sincos(x):
+    """(sin x, cos x) correctly rounded (the oracle's per-stage scalars)."""
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().orc_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
 
 
 def penta(n, albes, albee, alpha, beta):

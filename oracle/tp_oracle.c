@@ -7,7 +7,7 @@
  * evaluation order, the reference Fortran (all citations relative to the
  * reference tree):
  *
- *   function            restates                                   pinned by (tests/test_oracle_pins.py)
+ *   function            restates                                   pinned by
  *   ------------------  -----------------------------------------  -----------------------------
  *   orc_penta           src/Subroutineso.f90:15-67                 golden vectors of the compiled reference
  *   orc_fcint           src/Subroutineso.f90:91-103                golden vectors + interior transfer function
@@ -21,27 +21,51 @@
  *                                                                  modified wavenumber
  *   filte               src/Subroutines3D.f90:145-206              constants invariant for every closure,
  *                                                                  interior transfer function
- *   eleme/xtq2r/xtr2q   src/Subroutines3D.f90:235-324              X_R * X_L = I
+ *   eleme/xtq2r/xtr2q   src/Subroutines3D.f90:235-324              X_R * X_L = I, X_L A X_R diagonal
  *   metrics, normals    src/Main3D.f90:365-439                     exact metrics of affine grids, yaco = 1/J
- *   stage body          src/Main3D.f90:567-958                     uniform flow preserved on a periodic grid;
- *                                                                  otherwise parity unpinned (see below)
- *   spongeup/spongego   src/SingleTurbulence3D.f90:209-374         parity unpinned
- *   inflow table        src/SingleTurbulence3D.f90:253-292 (the    tests/test_inflow_cpu.py: periodic over one
- *     (Fourier modes)     eddy table's layout, filled from modes)  period; one gust mode gives the closed-form
+ *   stage sections (orc_stage_part; tests/test_oracle_stage_pins.py, every pin checked to fail on a
+ *   planted mistake in tests/test_oracle_mutations.py):
+ *     BEGIN             src/Main3D.f90:571-622                     CFL and viscous limits of a Cartesian grid
+ *                                                                  in closed form; sin^2 ramp symmetry
+ *     VISCOUS + FLUX    src/Main3D.f90:626-768                     analytic divergence of the Euler and
+ *                                                                  Navier-Stokes fluxes of linear fields
+ *                                                                  (moving frame too), quadratic shear,
+ *                                                                  heat flux of a linear temperature
+ *     SPONGE            src/SingleTurbulence3D.f90:332-374         isentropic target with total enthalpy
+ *                                                                  1/(gamma-1); relaxation linear in the
+ *                                                                  departure; forcing (rpt.f90:758-769) in
+ *                                                                  its window, closed-form coefficients
+ *     GCBC              src/Main3D.f90:778-869                     incoming amplitudes zero after a far-field
+ *                                                                  face; R+ = R- after a reflecting wall
+ *     RK                src/Main3D.f90:873-881                     4th-order Taylor polynomial on y' = ly
+ *     WALL              src/Main3D.f90:884-895                     no-slip, gamma p / rho = T_wall, slots kept
+ *     JUNCTION          src/SingleTurbulence3D.f90:378-443         copies agree: mean rho, E of four, momentum
+ *                                                                  of the two wall blocks
+ *     AVG               src/Main3D.f90:897-952                     interfaces continuous, two copies averaged
+ *   inflow table        src/SingleTurbulence3D.f90:253-292         tests/test_inflow_cpu.py: periodic over one
+ *     (Fourier modes)                                              period; one gust mode gives the closed-form
  *                                                                  travelling sinusoid A cos(theta - 2 pi it/ntz)
+ *   inflow table        src/SingleTurbulence3D.f90:262-298         tests/test_eddy_forcing_cpu.py: one isotropic
+ *     (eddies)                                                     eddy: u = f(r) a x d, perpendicular to d and
+ *                                                                  a, convected with u_inf, scaled by cfit
  *   orc_inflow_weights  src/SingleTurbulence3D.f90:336-356         tests/test_inflow_cpu.py: exact for quartics
  *                                                                  in the node times, wrapped nodes included
- *   inflow target       src/SingleTurbulence3D.f90:360-372         parity unpinned (isentropic density and
- *     (spongego)                                                   momentum from the interpolated velocity)
- *   junction            src/SingleTurbulence3D.f90:378-443         parity unpinned
+ *   forceup             src/rpt.f90:728-753                        tests/test_eddy_forcing_cpu.py: points within
+ *                                                                  rfor, Gaussian weight 1e-4 at rfor, the
+ *                                                                  reference's reuse of ra1 as wavenumber
  *   initialo            src/SingleTurbulence3D.f90:179-205         uniform-flow test (trivial state)
+ *   orc_sincos          sin, cos of src/Subroutineso.f90:192-259,  tests/test_rounding_cpu.py: correct rounding
+ *                       Main3D.f90:610, rpt.f90:761-763            against 300-bit mpmath values
+ *   orc_pow15 (x**1.5)  src/Main3D.f90:582                         tests/test_rounding_cpu.py: correct rounding
+ *                                                                  decided in exact integer arithmetic; libm
+ *                                                                  pow within one unit in the last place
  *
- * The composed ordering of the stage (sponge -> GCBC prep -> exchange ->
- * GCBC apply -> RK -> wall -> junction -> interface averaging), the sbcc
- * spreading, the wall slot flags and the dt ramp are restated from
- * src/Main3D.f90:567-958; the reference program itself cannot be built here
- * (it needs an MPI Fortran module), so these are "parity unpinned": the HIP
- * path is compared with this restatement, which nothing independent checks.
+ * Parity unpinned (nothing outside this file fixes it): the order in which
+ * orc_stage composes its sections (src/Main3D.f90:567-958, the reference's
+ * statement order), the sponge mask geometry (SingleTurbulence3D.f90:209-249,
+ * a profile the deck defines) and the wall-slot flags (extrabcc, :447-464).
+ * The reference program itself cannot be built here (it needs an MPI
+ * Fortran module).
  *
  * Layout: Fortran order, see tp_oracle.h.  All arithmetic is FP64 and is
  * compiled with -ffp-contract=off so that no fused multiply-add changes the
@@ -428,6 +452,78 @@ void orc_sbcco(double fltk, double *pbci, double *pbco, double *pbcot) {
     }
 }
 
+/* ---- sin and cos rounded correctly ----
+ * The per-stage scalars (moving frame, time-step ramp, body force) follow
+ * the reference's sin and cos, whose last bit its math library leaves open;
+ * the oracle fixes the correctly rounded values.  Method: x - k pi/2 with
+ * pi/2 split into three doubles (Cody and Waite) in double-double arithmetic
+ * (Dekker's two-sum and an fma two-product), then the Taylor series of sin
+ * and cos to the 1/29! term, summed by Horner's rule in double-double with
+ * the inverse factorials as double-double constants; the leading double of
+ * the normalised sum is the result.  |x| >= 2^20 (and NaN) is left to libm. */
+typedef struct { double h, l; } ddbl;
+static ddbl dd_fast(double a, double b) { double s = a + b; ddbl r = {s, b - (s - a)}; return r; }
+static ddbl dd_sum(double a, double b) {
+  double s = a + b, bb = s - a;
+  ddbl r = {s, (a - (s - bb)) + (b - bb)};
+  return r;
+}
+static ddbl dd_add(ddbl a, ddbl b) { ddbl s = dd_sum(a.h, b.h); return dd_fast(s.h, s.l + (a.l + b.l)); }
+static ddbl dd_mul(ddbl a, ddbl b) {
+  double p = a.h * b.h, e = fma(a.h, b.h, -p);
+  return dd_fast(p, e + (a.h * b.l + a.l * b.h));
+}
+/* 1/n!, n = 0..29, as double-double */
+static const double INV_FACT[30][2] = {
+    {1.0, 0.0}, {1.0, 0.0}, {0.5, 0.0}, {0.16666666666666666, 9.25185853854297e-18},
+    {0.041666666666666664, 2.3129646346357427e-18}, {0.008333333333333333, 1.1564823173178714e-19},
+    {0.001388888888888889, -5.300543954373577e-20}, {0.0001984126984126984, 1.7209558293420705e-22},
+    {2.48015873015873e-05, 2.1511947866775882e-23}, {2.7557319223985893e-06, -1.858393274046472e-22},
+    {2.755731922398589e-07, 2.3767714622250297e-23}, {2.505210838544172e-08, -1.448814070935912e-24},
+    {2.08767569878681e-09, -1.20734505911326e-25}, {1.6059043836821613e-10, 1.2585294588752098e-26},
+    {1.1470745597729725e-11, 2.0655512752830745e-28}, {7.647163731819816e-13, 7.03872877733453e-30},
+    {4.779477332387385e-14, 4.399205485834081e-31}, {2.8114572543455206e-15, 1.6508842730861433e-31},
+    {1.5619206968586225e-16, 1.1910679660273754e-32}, {8.22063524662433e-18, 2.2141894119604265e-34},
+    {4.110317623312165e-19, 1.4412973378659527e-36}, {1.9572941063391263e-20, -1.3643503830087908e-36},
+    {8.896791392450574e-22, -7.911402614872376e-38}, {3.868170170630684e-23, -8.843177655482344e-40},
+    {1.6117375710961184e-24, -3.6846573564509766e-41}, {6.446950284384474e-26, -1.9330404233703465e-42},
+    {2.4795962632247976e-27, -1.2953730964765229e-43}, {9.183689863795546e-29, 1.4303150396787322e-45},
+    {3.279889237069838e-30, 1.5117542744029879e-46}, {1.1309962886447716e-31, 1.0498015412959506e-47}};
+static ddbl inv_fact(int n, int neg) {
+  ddbl r = {neg ? -INV_FACT[n][0] : INV_FACT[n][0], neg ? -INV_FACT[n][1] : INV_FACT[n][1]};
+  return r;
+}
+void orc_sincos(double x, double *sn, double *cs) {
+  if (!(fabs(x) < 0x1p20)) { *sn = sin(x); *cs = cos(x); return; }
+  if (x == 0.0) { *sn = x; *cs = 1.0; return; }
+  const double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
+  const double k = nearbyint(x * 0.6366197723675814);
+  const double p = k * P1, pe = fma(k, P1, -p);
+  ddbl r = dd_sum(x, -p);
+  ddbl t = {-pe, 0.0};
+  r = dd_add(r, t);
+  const double q = k * P2, qe = fma(k, P2, -q);
+  t.h = -q; t.l = -qe;
+  r = dd_add(r, t);
+  t.h = -(k * P3); t.l = 0.0;
+  r = dd_add(r, t);
+  const ddbl z = dd_mul(r, r);
+  ddbl ps = inv_fact(29, 1), pc = inv_fact(28, 0);   /* n = 14: -z^14/29!, +z^14/28! */
+  for (int n = 13; n >= 0; n--) {
+    ps = dd_add(dd_mul(ps, z), inv_fact(2 * n + 1, n & 1));
+    pc = dd_add(dd_mul(pc, z), inv_fact(2 * n, n & 1));
+  }
+  const double S = dd_mul(r, ps).h, C = pc.h;
+  switch (((long long)k) & 3) {
+    case 0: *sn = S; *cs = C; break;
+    case 1: *sn = C; *cs = -S; break;
+    case 2: *sn = -S; *cs = -C; break;
+    default: *sn = -C; *cs = S; break;
+  }
+}
+static double crsin(double x) { double s, c; orc_sincos(x, &s, &c); return s; }
+static double crcos(double x) { double s, c; orc_sincos(x, &s, &c); return c; }
+
 /* src/Subroutineso.f90:192-259 */
 void orc_movef(int nsmf, double timf, double timo, double dtko, double dtk,
                const double *uoo, double tp, double tps, double aoa0,
@@ -436,30 +532,30 @@ void orc_movef(int nsmf, double timf, double timo, double dtko, double dtk,
   if (nsmf == 0) {
     double ra0 = PI / timf;
     double ra1 = ra0 * fmin(timo, timf), ra2 = ra0 * fmin(timo + dtko, timf);
-    double fctr = one - cos(ra1), dfdt = ra0 * sin(ra2);
+    double fctr = one - crcos(ra1), dfdt = ra0 * crsin(ra2);
     double progmf = half * (fctr + dtk * dfdt);
     for (int d = 0; d < 3; d++) umf[d] = progmf * uoo[d];
-    fctr = sin(ra1);
-    dfdt = ra0 * cos(ra2);
+    fctr = crsin(ra1);
+    dfdt = ra0 * crcos(ra2);
     progmf = half * ra0 * (fctr + dtk * dfdt);
     for (int d = 0; d < 3; d++) dudtmf[d] = progmf * uoo[d];
     if (tp > 0 && timo + dtk > tps) {
       double daoa = (aoa1 - aoa0) * PI / 180, raoa0 = aoa0 * PI / 180;
       double ra3 = timo - tps, dtsdt = PI / tp;
       double ts = fmin(dtsdt * ra3, PI), tsdt = fmin(dtsdt * (ra3 + dtko), PI);
-      double aoat = raoa0 + half * daoa * (1 - cos(ts));
-      double aoatdt = raoa0 + half * daoa * (1 - cos(tsdt));
+      double aoat = raoa0 + half * daoa * (1 - crcos(ts));
+      double aoatdt = raoa0 + half * daoa * (1 - crcos(tsdt));
       double f = half * daoa * dtsdt;
-      double ut[3] = {amachoo * cos(aoat), amachoo * sin(aoat), zero};
-      double du[3] = {-(amachoo * f * sin(aoatdt) * sin(tsdt)),
-                      amachoo * f * cos(aoatdt) * sin(tsdt), zero};
+      double ut[3] = {amachoo * crcos(aoat), amachoo * crsin(aoat), zero};
+      double du[3] = {-(amachoo * f * crsin(aoatdt) * crsin(tsdt)),
+                      amachoo * f * crcos(aoatdt) * crsin(tsdt), zero};
       for (int d = 0; d < 3; d++) umf[d] = ut[d] + dtk * du[d];
-      double dd[3] = {-(amachoo * f * sin(aoat) * sin(ts)),
-                      amachoo * f * cos(aoat) * sin(ts), zero};
-      double st = sin(tsdt);
+      double dd[3] = {-(amachoo * f * crsin(aoat) * crsin(ts)),
+                      amachoo * f * crcos(aoat) * crsin(ts), zero};
+      double st = crsin(tsdt);
       double d2[3] = {
-          -(amachoo * f * (cos(aoatdt) * f * (st * st) + dtsdt * sin(aoatdt) * cos(tsdt))),
-          -(amachoo * f * (sin(aoatdt) * f * (st * st) - dtsdt * cos(aoatdt) * cos(tsdt))),
+          -(amachoo * f * (crcos(aoatdt) * f * (st * st) + dtsdt * crsin(aoatdt) * crcos(tsdt))),
+          -(amachoo * f * (crsin(aoatdt) * f * (st * st) - dtsdt * crcos(aoatdt) * crcos(tsdt))),
           zero};
       for (int d = 0; d < 3; d++) dudtmf[d] = dd[d] + dtk * d2[d];
       if (aoa) *aoa = atan(umf[1] / umf[0]) * 180 / PI;
@@ -1662,6 +1758,26 @@ int orc_stats_wall(const orc *c, int b, double *mean_p, double *rms_p, double *a
 
 /* stage start (frame velocity for the stage, src/Main3D.f90:571-572), primitives
  * (:576-583) and the time step size at the first stage (:587-622) */
+/* x**1.5 (the viscosity law, src/Main3D.f90:582) rounded correctly: the
+ * reference leaves the last bit to its compiler's pow, which differs between
+ * libraries, so the oracle fixes the correctly rounded value.  y = x sqrt(x)
+ * is within two units in the last place of x^1.5; the residual x^3 - y^2 is
+ * formed from error-free products (fma), its leading difference exact by
+ * Sterbenz's lemma, and one Newton step y + (x^3 - y^2) / 2y lands on the
+ * nearest double unless x^1.5 lies within ~2^-50 units of a rounding midpoint.
+ * A perfect square x gives sqrt(x) exactly, and x sqrt(x) is then already
+ * rounded once.  Outside [2^-300, 2^300] (and for x <= 0, NaN) it is pow's. */
+double orc_pow15(double x) {
+  if (!(x > 0x1p-300 && x < 0x1p300)) return pow(x, 1.5);
+  double s = sqrt(x), y = x * s;
+  if (fma(s, s, -x) == 0.0) return y;
+  double P = y * y, p = fma(y, y, -P);  /* y^2 = P + p */
+  double Q = x * x, q = fma(x, x, -Q);  /* x^2 = Q + q */
+  double S = x * Q, e = fma(x, Q, -S);  /* x Q = S + e */
+  double r = (S - P) + ((e - p) + x * q);
+  return y + r / (y + y);
+}
+
 static void stage_begin(orc *c, int nk) {
   double *sc = c->sc;
   const int nkrk = c->nkrk;
@@ -1687,7 +1803,7 @@ static void stage_begin(orc *c, int nk) {
       double d5 = gam * p * d1;
       de[l] = d1; de[N + l] = d2; de[2 * N + l] = d3; de[3 * N + l] = d4; de[4 * N + l] = d5;
       B->p[l] = p;
-      B->ss[l] = c->srefp1dre * pow(d5, 1.5) / (d5 + c->srefoo);
+      B->ss[l] = c->srefp1dre * orc_pow15(d5) / (d5 + c->srefoo);
     }
   }
   /* time step size, src/Main3D.f90:587-610 */
@@ -1725,7 +1841,7 @@ static void stage_begin(orc *c, int nk) {
         sc[ORC_S_DTE] = c->dto;
       }
     }
-    double s = sin(0.05 * PI * (double)(n - (int)sc[ORC_S_NDT]));
+    double s = crsin(0.05 * PI * (double)(n - (int)sc[ORC_S_NDT]));
     sc[ORC_S_DT] = sc[ORC_S_DTS] + (sc[ORC_S_DTE] - sc[ORC_S_DTS]) * (s * s);
     dt = sc[ORC_S_DT];
   }
@@ -1944,9 +2060,9 @@ static void stage_sponge(orc *c, int nk) {
     const double t = sc[ORC_S_TIMO] + sc[ORC_S_DTK];
     if (t > c->tsfor && t < c->tefor) {
       const double ra0 = t - c->tsfor;
-      const double ra1 = c->amfor * cos(ra0 * 48.76 * c->amachoo) / 3;
-      const double ra2 = c->amfor * cos(ra0 * 53.6 * c->amachoo) / 3;
-      const double ra3 = c->amfor * cos(ra0 * 53.6 * c->amachoo) / 3;
+      const double ra1 = c->amfor * crcos(ra0 * 48.76 * c->amachoo) / 3;
+      const double ra2 = c->amfor * crcos(ra0 * 53.6 * c->amachoo) / 3;
+      const double ra3 = c->amfor * crcos(ra0 * 53.6 * c->amachoo) / 3;
       for (int b = 0; b < c->nb; b++) {
         oblk *B = &c->b[b];
         const int N = B->N, nf = B->lfor + 1;

@@ -137,6 +137,11 @@ enum { ORC_P_BEGIN = 0, ORC_P_VISCOUS, ORC_P_FLUX, ORC_P_SPONGE, ORC_P_GCBC, ORC
 int orc_stage_part(orc *c, int nk, int part);
 void orc_step(orc *c, int nsteps);
 
+/* x**1.5 of the viscosity law, correctly rounded */
+double orc_pow15(double x);
+/* sin and cos, correctly rounded (the per-stage scalars) */
+void orc_sincos(double x, double *sn, double *cs);
+
 /* standalone numerics (setup routines) */
 void orc_penta(int n, const double *albes, const double *albee, double alpha,
                double beta, double *xu, double *xl);

@@ -6,8 +6,11 @@ src/Main3D.f90:552-965) on device-resident state.  The HIP library is the
 only compute path: importing this module on a machine where the library
 cannot be loaded, or creating a ``Solver`` without a GPU, raises.
 """
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -40,7 +43,8 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_case_dims", "tp_case_params", "tp_write_restart", "tp_read_restart", "tp_write_grid",
            "tp_read_grid", "tp_write_plot3d", "tp_get_grid", "tp_check_case",
            "tp_primitives", "tp_viscous", "tp_flux_divergence", "tp_sponge", "tp_gcbc", "tp_rk_update",
-           "tp_wall", "tp_junction", "tp_interface_avg", "tp_field_deriv", "tp_eddy_params",
+           "tp_wall", "tp_junction", "tp_interface_avg", "tp_field_deriv", "tp_adopt_cases", "tp_get_option", "tp_server_set_staging",
+           "tp_server_stats2", "tp_eddy_params",
            "tp_set_inflow_eddies"]
 # the sections of a stage, in the order tp_stage runs them (name, takes nk)
 STAGE_PARTS = [("primitives", True), ("viscous", False), ("flux_divergence", True), ("sponge", False),
@@ -62,6 +66,21 @@ def load(path=LIB_PATH):
     if not os.path.exists(path):
         raise TPError(f"{path} missing: build it with `python -m tandemproject_amd.build` "
                       "(there is no CPU fallback)")
+    # One HIP runtime per process.  PyTorch ships its own libamdhip64 and
+    # libhsa-runtime64 and asks for them by the names libamdhip64.so and
+    # libhsa-runtime64.so; this library asks for the sonames .so.7 / .so.1.
+    # Without care a process that loads this library and then imports torch
+    # maps two HIP and two HSA runtimes, and the second finds no GPU.  The
+    # system runtime (the one this library is built against) is therefore
+    # loaded under torch's names first, so that a later `import torch` binds
+    # to it.  A process that imported torch before has torch's runtime
+    # loaded; this library then binds to that one (same soname).
+    if "torch" not in sys.modules:
+        for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+            try:
+                ctypes.CDLL(name, mode=ctypes.RTLD_GLOBAL)
+            except OSError:
+                pass
     L = ctypes.CDLL(path)
     P = ctypes.c_void_p
     i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -83,7 +102,10 @@ def load(path=LIB_PATH):
         "tp_interface_avg": [P],
         "tp_field_deriv": [i32, i32, i32, pi32, i32, P, P, P],
         "tp_eddy_params": [i32, pd, pd, pd],
-        "tp_set_inflow_eddies": [P, i32, i32, pd], "tp_step_graph": [P, i32], "tp_sync": [P],
+        "tp_set_inflow_eddies": [P, i32, i32, pd],
+        "tp_adopt_cases": [P, i32, ctypes.POINTER(P), pi32, pi32],
+        "tp_get_option": [P, i32, ctypes.POINTER(i64)],
+        "tp_server_set_staging": [P, i32], "tp_server_stats2": [P, pd], "tp_step_graph": [P, i32], "tp_sync": [P],
         "tp_profile": [P, i32], "tp_profile_classes": [P, i32],
         "tp_profile_read": [P, i32, pd, ctypes.POINTER(i64), pd, i32],
         "tp_info": [P, ctypes.POINTER(i64)],
@@ -181,6 +203,72 @@ def case_args(layout, params):
     return arrs, ipar, dpar
 
 
+def _hip():
+    """The HIP runtime this library is bound to (by its soname)."""
+    load()
+    h = ctypes.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    h.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+    h.hipHostFree.argtypes = [ctypes.c_void_p]
+    return h
+
+
+def device_memory():
+    """(free, total) device bytes of the current device (hipMemGetInfo)."""
+    fr, tot = ctypes.c_size_t(), ctypes.c_size_t()
+    if _hip().hipMemGetInfo(ctypes.byref(fr), ctypes.byref(tot)) != 0:
+        raise TPError("hipMemGetInfo failed")
+    return fr.value, tot.value
+
+
+def device_synchronize():
+    if _hip().hipDeviceSynchronize() != 0:
+        raise TPError("hipDeviceSynchronize failed")
+
+
+class DeviceArray:
+    """n float64 values of device memory on the current device, for the
+    context-free entry points (tp_field_deriv*) that take caller-owned
+    device pointers.  Copies are synchronous (hipMemcpy)."""
+
+    def __init__(self, n):
+        self._hip = _hip()
+        self.n = int(n)
+        self.ptr = ctypes.c_void_p()
+        if self._hip.hipMalloc(ctypes.byref(self.ptr), 8 * max(self.n, 1)) != 0:
+            self.ptr = None
+            raise TPError("hipMalloc failed")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64).reshape(-1)
+        if a.size != self.n:
+            raise TPError(f"expected {self.n} values, got {a.size}")
+        if self._hip.hipMemcpy(self.ptr, a.ctypes.data, 8 * self.n, 1) != 0:   # host to device
+            raise TPError("hipMemcpy to the device failed")
+        return self
+
+    def download(self):
+        out = np.empty(self.n)
+        if self._hip.hipMemcpy(out.ctypes.data, self.ptr, 8 * self.n, 2) != 0:   # device to host
+            raise TPError("hipMemcpy to the host failed")
+        return out
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
 class PinnedBuffer:
     """n float64 values of page-locked host memory (hipHostMalloc).  Device
     downloads into it run at full link speed; a freshly allocated pageable
@@ -190,7 +278,7 @@ class PinnedBuffer:
     def __init__(self, n):
         self._p = None
         self.array = None
-        self._hip = ctypes.CDLL("libamdhip64.so")
+        self._hip = _hip()
         self._p = ctypes.c_void_p()
         m = max(int(n), 1)
         if self._hip.hipHostMalloc(ctypes.byref(self._p), ctypes.c_size_t(8 * m), ctypes.c_uint(0)) != 0:
@@ -438,7 +526,7 @@ class Solver:
         w = self.stats_wall(case, block)
         if w is None:
             return None
-        dp = np.zeros(len(D_NAMES))
+        dp = np.zeros(N_DPARAM)   # tp_case_params writes all TP_D_NPARAM values
         _check(load().tp_case_params(self.h, case, None, _dp(dp)))
         m2 = sum(dp[D_NAMES.index(k)] ** 2 for k in ("amach1", "amach2", "amach3"))
         return (w[0] - 1.0 / 1.4) * 2.0 / m2, w[1] * 2.0 / m2
@@ -458,6 +546,18 @@ class Solver:
 
     def stage(self, nk):
         _check(load().tp_stage(self.h, nk))
+
+    def adopt(self, others):
+        """Move cases set up in other Solvers into this one: others = list of
+        (solver, case handle); returns the new handles here."""
+        n = len(others)
+        srcs = (ctypes.c_void_p * n)(*[o.h.value for o, _ in others])
+        ids = np.array([c for _, c in others], np.int32)
+        new = np.zeros(n, np.int32)
+        _check(load().tp_adopt_cases(self.h, n, srcs, _ip(ids), _ip(new)))
+        for (o, c), nc in zip(others, new):
+            self.layouts[int(nc)] = o.layouts.pop(c, None) if hasattr(o, "layouts") else None
+        return [int(x) for x in new]
 
     def set_inflow_eddies(self, case, eddies):
         e = np.ascontiguousarray(eddies, np.float64)
@@ -502,6 +602,18 @@ class Solver:
         return dict(points=out[0], blocks=out[1], cases=out[2], device_bytes=out[3])
 
 
+# servers still open when the interpreter exits are closed before it tears
+# its modules down: each joins its staging thread and frees its contexts while
+# the HIP runtime is whole
+_OPEN_SERVERS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_servers():
+    for s in list(_OPEN_SERVERS):
+        s.close()
+
+
 class Server:
     """Request loop over one context (tp_server_*): cases queue with an arrival
     time, are admitted while their device bytes fit the HBM budget and a slot
@@ -518,6 +630,7 @@ class Server:
             raise TPError(L.tp_last_error().decode())
         self.ctx = ctypes.c_void_p(L.tp_server_context(self.h))
         self.layouts = {}
+        _OPEN_SERVERS.add(self)
         if fill_points:
             self._check(L.tp_server_set_fill(self.h, int(fill_points)))
 
@@ -563,6 +676,39 @@ class Server:
     def run(self, max_steps=10):
         self._check(load().tp_server_run(self.h, int(max_steps)))
 
+    def run_stream(self, items, max_steps=10):
+        """Serve a stream: a client thread submits items[i] = (layout,
+        params, grids, nsteps, t) at t seconds after the call (each arrival
+        is the time of its submission) while this thread runs the loop, until
+        every request has finished.  Returns the request ids in item order."""
+        import threading
+        import time
+        ids, err = [None] * len(items), []
+        t0 = self.clock()
+
+        def client():
+            try:
+                for i, (lay, params, grids, nsteps, t) in enumerate(items):
+                    wait = t0 + t - self.clock()
+                    if wait > 0:
+                        time.sleep(wait)
+                    ids[i] = self.submit(lay, params, grids, nsteps)
+            except Exception as e:   # reported by the loop's thread
+                err.append(e)
+
+        th = threading.Thread(target=client, daemon=True)
+        th.start()
+        try:
+            while th.is_alive():
+                self.run(max_steps)     # returns when nothing submitted is left open
+                time.sleep(0.001)
+            self.run(max_steps)         # the last submissions
+        finally:
+            th.join()
+        if err:
+            raise err[0]
+        return ids
+
     def result(self, req, with_state=True):
         """(info dict, list of per-block qa arrays or None)."""
         info = np.zeros(6)
@@ -583,10 +729,19 @@ class Server:
             off += m
         return d, parts
 
+    def set_staging(self, ahead):
+        """Requests set up ahead of admission on the staging thread (>= 1)."""
+        self._check(load().tp_server_set_staging(self.h, int(ahead)))
+
     def stats(self):
         o = np.zeros(16)
         self._check(load().tp_server_stats(self.h, _dp(o)))
         keys = ("finished", "failed", "p50_s", "p99_s", "mean_s", "max_s", "wall_s", "cases_per_hour",
                 "peak_bytes", "peak_cases", "joined_running", "stepping_s", "admit_s", "retire_s",
                 "capture_s", "captures")
-        return dict(zip(keys, (float(x) for x in o)))
+        out = dict(zip(keys, (float(x) for x in o)))
+        o2 = np.zeros(6)
+        self._check(load().tp_server_stats2(self.h, _dp(o2)))
+        out.update(zip(("stage_s", "staged", "stage_failed", "staged_peak", "eager_iterations", "staging_window"),
+                       (float(x) for x in o2)))
+        return out

@@ -35,8 +35,10 @@ static int fail(int code, const std::string &msg) {
 #define HIPCHK(x)                                                                          \
   do {                                                                                     \
     hipError_t e_ = (x);                                                                   \
-    if (e_ != hipSuccess)                                                                  \
+    if (e_ != hipSuccess) {                                                                \
+      (void)hipGetLastError(); /* reported here: not again by the next call */            \
       return fail(TP_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));               \
+    }                                                                                      \
   } while (0)
 
 namespace tp {
@@ -1575,11 +1577,33 @@ int32_t tp_set_inflow_eddies(tp_context *c, int32_t cs, int32_t nits, const doub
 
 const char *tp_last_error(void) { return g_err.c_str(); }
 
+namespace {
+// the HIP runtime the process has loaded must be no older than the one the
+// library is built against: a process can hold another runtime first (an
+// application's bundled copy answers to the same soname), and this library
+// is not run on an older one
+int runtime_check() {
+  int v = 0;
+  if (hipRuntimeGetVersion(&v) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(TP_E_HIP, "hipRuntimeGetVersion failed");
+  }
+  if (v / 100000 < HIP_VERSION / 100000)
+    return fail(TP_E_HIP, "the process has loaded HIP runtime " + std::to_string(v / 10000000) + "." +
+                              std::to_string(v / 100000 % 100) + ", older than the " +
+                              std::to_string(HIP_VERSION_MAJOR) + "." + std::to_string(HIP_VERSION_MINOR) +
+                              " this library is built for: load the library before any other HIP user "
+                              "(in Python, tandemproject_amd.api.load() before `import torch`)");
+  return 0;
+}
+}  // namespace
+
 int32_t tp_create(tp_context **out) {
   if (!out) return fail(TP_E_ARG, "null output pointer");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(TP_E_HIP, "no HIP device available: the MI355X time stepper has no CPU path");
+  if (int rc = runtime_check()) return rc;
   tp_context *c = new tp_context();
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return fail(TP_E_HIP, "hipGetDevice failed"); }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -2198,7 +2222,11 @@ int32_t tp_step_graph(tp_context *c, int32_t nsteps) {
     enqueue_step(c);
     hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
     c->prof = prof;
-    if (e != hipSuccess) return fail(TP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      c->graph = nullptr;
+      return fail(TP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
     HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
   }
   for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
@@ -2234,6 +2262,16 @@ int32_t tp_set_option(tp_context *c, int32_t option, int64_t value) {
         for (int &v : b) v = -1;
     if (!c->finalized) return 0;
     return regroup(c);
+  }
+  return fail(TP_E_ARG, "unknown option " + std::to_string(option));
+}
+
+int32_t tp_get_option(tp_context *c, int32_t option, int64_t *value) {
+  if (!c || !value) return fail(TP_E_ARG, "null argument");
+  switch (option) {
+    case TP_OPT_EXACT_LINES: *value = c->exact_lines ? 1 : 0; return 0;
+    case TP_OPT_CHUNKS: *value = c->col_fixed; return 0;
+    case TP_OPT_STRESSES: *value = c->keep_stress ? 1 : 0; return 0;
   }
   return fail(TP_E_ARG, "unknown option " + std::to_string(option));
 }
@@ -2460,6 +2498,49 @@ int32_t tp_remove_cases(tp_context *c, int32_t n, const int32_t *ids) {
 }
 
 int32_t tp_remove_case(tp_context *c, int32_t cs) { return tp_remove_cases(c, 1, &cs); }
+
+int32_t tp_adopt_cases(tp_context *dst, int32_t n, tp_context *const *srcs, const int32_t *case_ids,
+                       int32_t *new_ids) {
+  if (int rc = check_ready(dst)) return rc;
+  if (n < 0 || (n > 0 && (!srcs || !case_ids || !new_ids))) return fail(TP_E_ARG, "tp_adopt_cases: bad arguments");
+  // every case must be set up on the device of dst, in a context other than dst
+  for (int i = 0; i < n; i++) {
+    tp_context *src = srcs[i];
+    if (!src || src == dst) return fail(TP_E_ARG, "tp_adopt_cases: a source context is missing or is dst");
+    if (src->device != dst->device) return fail(TP_E_ARG, "tp_adopt_cases: contexts on different devices");
+    const int ci = case_index(src, case_ids[i]);
+    if (ci < 0) return fail(TP_E_ARG, "tp_adopt_cases: bad case " + std::to_string(case_ids[i]));
+    const HostCase &h = src->cases[ci];
+    if (!h.built || !h.geom || h.slot < 0)
+      return fail(TP_E_STATE, "tp_adopt_cases: case " + std::to_string(case_ids[i]) + " is not set up on the device");
+    if (dst->nkrk && h.ip[TP_I_NKRK] != dst->nkrk)
+      return fail(TP_E_ARG, "all cases of a context must use the same number of RK stages");
+    for (int j = 0; j < i; j++)
+      if (srcs[j] == src && case_ids[j] == case_ids[i]) return fail(TP_E_ARG, "tp_adopt_cases: a case named twice");
+  }
+  HIPCHK(hipStreamSynchronize(dst->stream));
+  std::vector<tp_context *> touched;
+  for (int i = 0; i < n; i++) {
+    tp_context *src = srcs[i];
+    HIPCHK(hipStreamSynchronize(src->stream));
+    const int ci = case_index(src, case_ids[i]);
+    HostCase h = std::move(src->cases[ci]);
+    // the scalars of the case travel with it (n, t, dt, frame velocity, ...)
+    HIPCHK(hipMemcpy(h.sc, src->d_sc + (size_t)h.slot * S_STRIDE, sizeof(double) * S_STRIDE, hipMemcpyDeviceToHost));
+    h.sc_saved = true;
+    src->cases.erase(src->cases.begin() + ci);
+    if (src->cases.empty()) src->nkrk = 0;
+    if (std::find(touched.begin(), touched.end(), src) == touched.end()) touched.push_back(src);
+    h.id = dst->next_id++;
+    h.slot = -1;
+    new_ids[i] = h.id;
+    dst->nkrk = h.ip[TP_I_NKRK];
+    dst->cases.push_back(std::move(h));
+  }
+  for (tp_context *src : touched)
+    if (int rc = link(src)) return rc;
+  return link(dst);
+}
 
 int32_t tp_case_info(tp_context *c, int32_t cs, int64_t *out4) {
   if (!c || !out4) return fail(TP_E_ARG, "null argument");
@@ -2736,6 +2817,7 @@ int32_t tp_field_deriv(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc
                        double *dst, void *stream) {
   FieldCall f;
   if (int rc = field_prepare(lxi, let, lze, nbc2, nn, src, dst, f)) return rc;
+  if (int rc = runtime_check()) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   char *ws = nullptr;
   HIPCHK(hipMalloc(&ws, f.ws.total));

@@ -39,6 +39,87 @@ __device__ inline Phys phys() {
   const int N = b.N;                                                           \
   (void)N;
 
+// ---- sin and cos rounded correctly, for the per-stage scalars (moving
+// frame, time-step ramp, body force): the reference leaves their last bit to
+// its math library, and a one-unit difference in the frame acceleration
+// grows over a run.  x - k pi/2 with pi/2 split into three doubles (Cody and
+// Waite), in double-double arithmetic (two-sum, fma two-product); Taylor
+// series of sin and cos to the 1/29! term by Horner's rule in double-double;
+// the normalised sum's leading double is the result.  |x| >= 2^20: libm. ----
+struct DD {
+  double h, l;
+};
+__device__ inline DD dd_fast(double a, double b) {
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+__device__ inline DD dd_sum(double a, double b) {
+  const double s = a + b, bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ inline DD dd_add(DD a, DD b) {
+  const DD s = dd_sum(a.h, b.h);
+  return dd_fast(s.h, s.l + (a.l + b.l));
+}
+__device__ inline DD dd_mul(DD a, DD b) {
+  const double p = a.h * b.h, e = fma(a.h, b.h, -p);
+  return dd_fast(p, e + (a.h * b.l + a.l * b.h));
+}
+// 1/n! as double-double, n = 0..29
+__constant__ double kInvFact[30][2] = {
+    {1.0, 0.0}, {1.0, 0.0}, {0.5, 0.0}, {0.16666666666666666, 9.25185853854297e-18},
+    {0.041666666666666664, 2.3129646346357427e-18}, {0.008333333333333333, 1.1564823173178714e-19},
+    {0.001388888888888889, -5.300543954373577e-20}, {0.0001984126984126984, 1.7209558293420705e-22},
+    {2.48015873015873e-05, 2.1511947866775882e-23}, {2.7557319223985893e-06, -1.858393274046472e-22},
+    {2.755731922398589e-07, 2.3767714622250297e-23}, {2.505210838544172e-08, -1.448814070935912e-24},
+    {2.08767569878681e-09, -1.20734505911326e-25}, {1.6059043836821613e-10, 1.2585294588752098e-26},
+    {1.1470745597729725e-11, 2.0655512752830745e-28}, {7.647163731819816e-13, 7.03872877733453e-30},
+    {4.779477332387385e-14, 4.399205485834081e-31}, {2.8114572543455206e-15, 1.6508842730861433e-31},
+    {1.5619206968586225e-16, 1.1910679660273754e-32}, {8.22063524662433e-18, 2.2141894119604265e-34},
+    {4.110317623312165e-19, 1.4412973378659527e-36}, {1.9572941063391263e-20, -1.3643503830087908e-36},
+    {8.896791392450574e-22, -7.911402614872376e-38}, {3.868170170630684e-23, -8.843177655482344e-40},
+    {1.6117375710961184e-24, -3.6846573564509766e-41}, {6.446950284384474e-26, -1.9330404233703465e-42},
+    {2.4795962632247976e-27, -1.2953730964765229e-43}, {9.183689863795546e-29, 1.4303150396787322e-45},
+    {3.279889237069838e-30, 1.5117542744029879e-46}, {1.1309962886447716e-31, 1.0498015412959506e-47}};
+__device__ inline DD inv_fact(int n, bool neg) {
+  return neg ? DD{-kInvFact[n][0], -kInvFact[n][1]} : DD{kInvFact[n][0], kInvFact[n][1]};
+}
+__device__ void cr_sincos(double x, double *sn, double *cs) {
+  if (!(fabs(x) < 0x1p20)) { *sn = sin(x); *cs = cos(x); return; }
+  if (x == 0.0) { *sn = x; *cs = 1.0; return; }
+  const double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
+  const double k = rint(x * 0.6366197723675814);
+  const double p = k * P1, pe = fma(k, P1, -p);
+  DD r = dd_sum(x, -p);
+  r = dd_add(r, DD{-pe, 0.0});
+  const double q = k * P2, qe = fma(k, P2, -q);
+  r = dd_add(r, DD{-q, -qe});
+  r = dd_add(r, DD{-(k * P3), 0.0});
+  const DD z = dd_mul(r, r);
+  DD ps = inv_fact(29, true), pc = inv_fact(28, false);
+  for (int n = 13; n >= 0; n--) {
+    ps = dd_add(dd_mul(ps, z), inv_fact(2 * n + 1, n & 1));
+    pc = dd_add(dd_mul(pc, z), inv_fact(2 * n, n & 1));
+  }
+  const double S = dd_mul(r, ps).h, C = pc.h;
+  switch (((long long)k) & 3) {
+    case 0: *sn = S; *cs = C; break;
+    case 1: *sn = C; *cs = -S; break;
+    case 2: *sn = -S; *cs = -C; break;
+    default: *sn = -C; *cs = S; break;
+  }
+}
+__device__ inline double cr_sin(double x) {
+  double s, c;
+  cr_sincos(x, &s, &c);
+  return s;
+}
+__device__ inline double cr_cos(double x) {
+  double s, c;
+  cr_sincos(x, &s, &c);
+  return c;
+}
+
 // ---- moving frame, Subroutineso.f90:192-259 ----
 __device__ void movef_dev(const CaseCoef &c, double *sc, double dtko, double dtk) {
   double *umf = sc + S_UMF1, *dudt = sc + S_DUDT1;
@@ -46,27 +127,27 @@ __device__ void movef_dev(const CaseCoef &c, double *sc, double dtko, double dtk
   if (c.nsmf == 0) {
     double ra0 = PI_D / c.timf;
     double ra1 = ra0 * fmin(timo, c.timf), ra2 = ra0 * fmin(timo + dtko, c.timf);
-    double fctr = 1.0 - cos(ra1), dfdt = ra0 * sin(ra2);
+    double fctr = 1.0 - cr_cos(ra1), dfdt = ra0 * cr_sin(ra2);
     double progmf = 0.5 * (fctr + dtk * dfdt);
     for (int d = 0; d < 3; d++) umf[d] = progmf * c.uoo[d];
-    fctr = sin(ra1);
-    dfdt = ra0 * cos(ra2);
+    fctr = cr_sin(ra1);
+    dfdt = ra0 * cr_cos(ra2);
     progmf = 0.5 * ra0 * (fctr + dtk * dfdt);
     for (int d = 0; d < 3; d++) dudt[d] = progmf * c.uoo[d];
     if (c.tp > 0 && timo + dtk > c.tps) {
       double daoa = (c.aoa1 - c.aoa0) * PI_D / 180, raoa0 = c.aoa0 * PI_D / 180;
       double ra3 = timo - c.tps, dtsdt = PI_D / c.tp;
       double ts = fmin(dtsdt * ra3, PI_D), tsdt = fmin(dtsdt * (ra3 + dtko), PI_D);
-      double aoat = raoa0 + 0.5 * daoa * (1 - cos(ts));
-      double aoatdt = raoa0 + 0.5 * daoa * (1 - cos(tsdt));
+      double aoat = raoa0 + 0.5 * daoa * (1 - cr_cos(ts));
+      double aoatdt = raoa0 + 0.5 * daoa * (1 - cr_cos(tsdt));
       double f = 0.5 * daoa * dtsdt, am = c.amachoo;
-      double ut0 = am * cos(aoat), ut1 = am * sin(aoat);
-      double du0 = -(am * f * sin(aoatdt) * sin(tsdt)), du1 = am * f * cos(aoatdt) * sin(tsdt);
+      double ut0 = am * cr_cos(aoat), ut1 = am * cr_sin(aoat);
+      double du0 = -(am * f * cr_sin(aoatdt) * cr_sin(tsdt)), du1 = am * f * cr_cos(aoatdt) * cr_sin(tsdt);
       umf[0] = ut0 + dtk * du0; umf[1] = ut1 + dtk * du1; umf[2] = 0.0 + dtk * 0.0;
-      double dd0 = -(am * f * sin(aoat) * sin(ts)), dd1 = am * f * cos(aoat) * sin(ts);
-      double st = sin(tsdt);
-      double d20 = -(am * f * (cos(aoatdt) * f * (st * st) + dtsdt * sin(aoatdt) * cos(tsdt)));
-      double d21 = -(am * f * (sin(aoatdt) * f * (st * st) - dtsdt * cos(aoatdt) * cos(tsdt)));
+      double dd0 = -(am * f * cr_sin(aoat) * cr_sin(ts)), dd1 = am * f * cr_cos(aoat) * cr_sin(ts);
+      double st = cr_sin(tsdt);
+      double d20 = -(am * f * (cr_cos(aoatdt) * f * (st * st) + dtsdt * cr_sin(aoatdt) * cr_cos(tsdt)));
+      double d21 = -(am * f * (cr_sin(aoatdt) * f * (st * st) - dtsdt * cr_cos(aoatdt) * cr_cos(tsdt)));
       dudt[0] = dd0 + dtk * d20; dudt[1] = dd1 + dtk * d21; dudt[2] = 0.0 + dtk * 0.0;
       sc[S_AOA] = atan(umf[1] / umf[0]) * 180 / PI_D;
     }
@@ -176,7 +257,7 @@ __global__ void k_sc_dt(const CaseCoef *coef, double *sc, const double *part, co
         s[S_DTE] = c.dto;
       }
     }
-    double sn = sin(0.05 * PI_D * (double)(n - (int)s[S_NDT]));
+    double sn = cr_sin(0.05 * PI_D * (double)(n - (int)s[S_NDT]));
     s[S_DT] = s[S_DTS] + (s[S_DTE] - s[S_DTS]) * (sn * sn);
   }
 }
@@ -208,6 +289,24 @@ __global__ void k_sc_end(double *sc, int ncase, const int *active) {
 struct Prim {
   double d1, d2, d3, d4, d5, p, mu;
 };
+// x^1.5 of the viscosity law (Main3D.f90:582), rounded correctly, so that the
+// result does not depend on a math library's last bit: y = x sqrt(x) is
+// within two units in the last place; the residual x^3 - y^2 comes from
+// error-free fma products (its leading difference is exact, the two terms
+// being within a factor two), and one Newton step y + (x^3 - y^2) / 2y gives
+// the nearest double.  A perfect square x has an exact sqrt, and x sqrt(x)
+// is then rounded once already.  Outside [2^-300, 2^300]: pow's value.
+__device__ inline double pow15(double x) {
+  if (!(x > 0x1p-300 && x < 0x1p300)) return pow(x, 1.5);
+  const double s = sqrt(x), y = x * s;
+  if (fma(s, s, -x) == 0.0) return y;
+  const double P = y * y, p = fma(y, y, -P);
+  const double Q = x * x, q = fma(x, x, -Q);
+  const double S = x * Q, e = fma(x, Q, -S);
+  const double r = (S - P) + ((e - p) + x * q);
+  return y + r / (y + y);
+}
+
 __device__ inline Prim primitives(const CaseCoef &c, const Phys &ph, double q1, double q2, double q3, double q4,
                                   double q5) {
   Prim r;
@@ -215,7 +314,7 @@ __device__ inline Prim primitives(const CaseCoef &c, const Phys &ph, double q1, 
   r.d2 = q2 * r.d1; r.d3 = q3 * r.d1; r.d4 = q4 * r.d1;
   r.p = ph.gamm1 * (q5 - 0.5 * (q2 * r.d2 + q3 * r.d3 + q4 * r.d4));
   r.d5 = GAM * r.p * r.d1;
-  r.mu = c.srefp1dre * pow(r.d5, 1.5) / (r.d5 + c.srefoo);
+  r.mu = c.srefp1dre * pow15(r.d5) / (r.d5 + c.srefoo);
   return r;
 }
 
@@ -477,9 +576,9 @@ __global__ void __launch_bounds__(256) k_force(StageArgs a) {
   const double t = sc[S_TIMO] + sc[S_DTK];
   if (!(c.forcing == 1 && t > c.tsfor && t < c.tefor)) return;
   const double ra0 = t - c.tsfor;
-  const double ra1 = c.amfor * cos(ra0 * 48.76 * c.amachoo) / 3;
-  const double ra2 = c.amfor * cos(ra0 * 53.6 * c.amachoo) / 3;
-  const double ra3 = c.amfor * cos(ra0 * 53.6 * c.amachoo) / 3;
+  const double ra1 = c.amfor * cr_cos(ra0 * 48.76 * c.amachoo) / 3;
+  const double ra2 = c.amfor * cr_cos(ra0 * 53.6 * c.amachoo) / 3;
+  const double ra3 = c.amfor * cr_cos(ra0 * 53.6 * c.amachoo) / 3;
   const int nf = b.nforce;
   for (int q = w.a + threadIdx.x; q < w.a + w.n; q += blockDim.x) {
     const int l = b.lcfor[q];

@@ -196,6 +196,31 @@ module tandem_hip
       integer(c_int32_t), intent(in) :: nbc2(2)
       type(c_ptr), value :: src, dst, stream
     end function
+    integer(c_int32_t) function tp_get_option(ctx, option, value) bind(C, name="tp_get_option")
+      import :: c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: option
+      integer(c_int64_t), intent(out) :: value
+    end function
+    integer(c_int32_t) function tp_server_set_staging(srv, ahead) bind(C, name="tp_server_set_staging")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: srv
+      integer(c_int32_t), value :: ahead
+    end function
+    integer(c_int32_t) function tp_server_stats2(srv, out6) bind(C, name="tp_server_stats2")
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: srv
+      real(c_double), intent(out) :: out6(6)
+    end function
+    ! move set-up cases from other contexts into ctx
+    integer(c_int32_t) function tp_adopt_cases(ctx, n, srcs, case_ids, new_ids) bind(C, name="tp_adopt_cases")
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(in) :: srcs(*)
+      integer(c_int32_t), intent(in) :: case_ids(*)
+      integer(c_int32_t), intent(out) :: new_ids(*)
+    end function
     ! the reference's synthetic eddies (inflow 3)
     integer(c_int32_t) function tp_eddy_params(nits, randnum, deck, out10) bind(C, name="tp_eddy_params")
       import :: c_int32_t, c_double

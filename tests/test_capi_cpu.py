@@ -120,3 +120,18 @@ def test_oracle_refuses_bad_cases():
         o.deriv(4, 1, 1)
     with pytest.raises(po.OracleError, match="out of range"):
         o.stage_part(9, "begin")
+
+
+def test_host_buffers_match_the_header_sizes():
+    """The Python host's parameter and scalar buffers have the sizes the C
+    entry points write (tp_case_params copies all TP_D_NPARAM doubles, a
+    shorter buffer would be overrun), and no output buffer is sized from the
+    name lists, which name fewer entries than the arrays hold."""
+    src = open(os.path.join(ROOT, "include", "tandem_hip.h")).read()
+    size = {k: int(v) for k, v in re.findall(r"\b(TP_[ID]_NPARAM|TP_S_NSCALAR) = (\d+)", src)}
+    assert api.N_IPARAM == size["TP_I_NPARAM"] and api.N_DPARAM == size["TP_D_NPARAM"]
+    assert len(api.I_NAMES) <= api.N_IPARAM and len(api.D_NAMES) <= api.N_DPARAM
+    assert len(api.SCALARS) <= size["TP_S_NSCALAR"]
+    host = open(os.path.join(ROOT, "tandemproject_amd", "api.py")).read()
+    assert not re.search(r"np\.(zeros|empty)\(len\((I_NAMES|D_NAMES|SCALARS)\)", host)
+    assert f"np.zeros({size['TP_S_NSCALAR']})" in host     # tp_get_scalars' buffer

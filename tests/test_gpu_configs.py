@@ -171,15 +171,19 @@ def test_c5_served_stream_equals_each_case_alone():
     kinds = ["tandem" if lay.nb == 24 else "single" for lay, _, _ in cases]
     say("C5: 8 requests,", list(zip(kinds, pts)))
     assert max(pts) >= 1.0e7 and "tandem" in kinds
-    nsteps = 10
+    # a client thread submits into the running loop, 0.08 s apart on
+    # average, 40 steps each (a 2 M point case takes ~0.15 s), so requests
+    # set up meanwhile join running cases
+    nsteps = 40
     srv = api.Server(budget_bytes=0, max_active=8, graph=True, fill_points=24_000_000)
-    t0 = srv.clock() + 0.05
-    arrivals = np.cumsum(np.random.default_rng(3).exponential(0.3, len(cases)))
-    reqs = [srv.submit(lay, p, ss, nsteps, t0 + a) for (lay, ss, p), a in zip(cases, arrivals)]
-    srv.run(max_steps=5)
+    arrivals = np.cumsum(np.random.default_rng(3).exponential(0.08, len(cases)))
+    reqs = srv.run_stream([(lay, p, ss, nsteps, a) for (lay, ss, p), a in zip(cases, arrivals)], max_steps=5)
     st = srv.stats()
     say("  p50", st["p50_s"], "p99", st["p99_s"], "joined running cases", st["joined_running"],
         "peak cases", st["peak_cases"])
+    for r, k, n in zip(reqs, kinds, pts):
+        i = srv.result(r, with_state=False)[0]
+        say(f"  {k:6s} {n:9d} points: arrival {i['arrival']:.3f} admitted {i['admitted']:.3f} done {i['done']:.3f}")
     assert st["finished"] == len(cases) and st["joined_running"] >= 1
     results = [srv.result(r)[1] for r in reqs]
     srv.close()

@@ -40,32 +40,29 @@ def oracle_deriv(dims, nn, ends, f):
 
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_field_deriv_matches_oracle(case):
-    import torch
     dims, nn, ends = CASES[case]
     print("field deriv", dims, "direction", nn, "ends", ends, flush=True)
     shape = (dims[2] + 1, dims[1] + 1, dims[0] + 1)
     f = seeded_field(shape, 31 + case)
     ref = oracle_deriv(dims, nn, ends, f)
-    src = torch.tensor(f, dtype=torch.float64, device="cuda")
-    dst = torch.full_like(src, float("nan"))
-    torch.cuda.synchronize()
-    api.field_deriv(dims, ends, nn, src.data_ptr(), dst.data_ptr())
-    torch.cuda.synchronize()
-    got = dst.cpu().numpy()
+    src = api.DeviceArray(f.size).upload(f)
+    dst = api.DeviceArray(f.size).upload(np.full(f.size, np.nan))
+    api.field_deriv(dims, ends, nn, src.address, dst.address)
+    api.device_synchronize()
+    got = dst.download()
     err = float(np.abs(got - ref).max() / np.abs(ref).max())
     print("  max rel diff", err, flush=True)
     assert np.isfinite(got).all()
     assert err <= 1e-13
     # the input is untouched
-    np.testing.assert_array_equal(src.cpu().numpy(), f)
+    np.testing.assert_array_equal(src.download(), f.reshape(-1))
 
 
 def test_field_deriv_refuses_bad_input():
-    import torch
-    x = torch.zeros(8, dtype=torch.float64, device="cuda")
+    x = api.DeviceArray(8).upload(np.zeros(8))
     with pytest.raises(api.TPError, match="end codes"):
-        api.field_deriv((20, 20, 20), (10, 40), 1, x.data_ptr(), x.data_ptr())
+        api.field_deriv((20, 20, 20), (10, 40), 1, x.address, x.address)
     with pytest.raises(api.TPError, match="512 points"):
-        api.field_deriv((600, 8, 8), (10, 10), 1, x.data_ptr(), x.data_ptr())
+        api.field_deriv((600, 8, 8), (10, 10), 1, x.address, x.address)
     with pytest.raises(api.TPError, match="too short"):
-        api.field_deriv((20, 20, 8), (45, 45), 3, x.data_ptr(), x.data_ptr())
+        api.field_deriv((20, 20, 8), (45, 45), 3, x.address, x.address)

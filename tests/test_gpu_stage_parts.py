@@ -80,7 +80,7 @@ def test_parts_compose_to_the_stage(mode):
 # arrays each section writes (compared) -- the rest of its inputs are uploaded
 # from the oracle before the section runs
 OUTPUTS = {
-    "primitives": ["de", "p"],
+    "primitives": ["de", "p", "ss"],
     "viscous": ["tv"],
     "flux_divergence": ["de", "drva1", "drva2", "drva3"],
     "sponge": ["de"],
@@ -107,20 +107,41 @@ def upload_state(o, s, lay, scalars=True):
         s.set_scalars(0, sc)
 
 
-@pytest.mark.parametrize("mode", ["sequential", "chunked"])
-def test_parts_against_oracle_sections(mode):
-    """Lock-step over stages 1 and 2 of a gust-response step: before each
-    section the device gets the oracle's arrays, then both run the section
-    and its outputs are compared."""
-    say("stage sections against the oracle's sections,", mode)
-    lay, ss, p, gmode, slit = gust_case()
-    o = po.Oracle(lay, dict(p, slit=slit, nmodes=1))
-    o.set_inflow_modes(gmode)
-    kw = dict(exact_lines=True) if mode == "sequential" else dict(chunks=3)
-    s = api.Solver([(lay, p)], **kw)
+def les_gcic_case():
+    """Smagorinsky LES with wall slots exchanged through the aerofoil
+    (nextrabc = nextgcic = 1) on a left-handed grid (z flipped, so that the
+    volume term (-1/yaco)**1.5 is real), spanwise split in two."""
+    lay, ss, p = aerofoil_case(lzebk=(31,), npcz=[2])
+    ss = [np.concatenate([x[:2 * (x.size // 3)], -x[2 * (x.size // 3):]]) for x in ss]
+    p = dict(p); p.update(les=1, nextgcic=1, nextrabc=1, wlea=0.4, wlew=0.1)
+    return lay, ss, p
+
+
+@pytest.mark.parametrize("case", ["gust", "les-gcic"])
+def test_parts_against_oracle_sections(case):
+    """Lock-step over stages 1 and 2 of a step, with the sequential line
+    solves: before each section the device gets the oracle's arrays, then
+    both run the section and its outputs are compared.  (The chunked solves
+    round the chunk carries differently; their per-call parity is
+    test_deriv_per_call's, and a flux divergence is a sum of nearly
+    cancelling derivatives, so it is not compared at this bar.)"""
+    say("stage sections against the oracle's sections,", case)
+    if case == "gust":
+        lay, ss, p, gmode, slit = gust_case()
+        o = po.Oracle(lay, dict(p, slit=slit, nmodes=1))
+        o.set_inflow_modes(gmode)
+    else:
+        lay, ss, p = les_gcic_case()
+        o = po.Oracle(lay, p)
+    s = api.Solver([(lay, p)], exact_lines=True)
     for b in range(lay.nb):
         o.set_grid(b, ss[b]); s.set_grid(0, b, ss[b])
     o.metrics(); s.setup_geometry()
+    for b in range(lay.nb):
+        for name in ("xim", "etm", "zem", "yaco"):
+            d = float(np.abs(s.download(0, b, name) - o.array(b, name)).max())
+            if d:
+                say(f"  metrics differ: block {b} {name} {d:.2e}")
     o.initialo(); s.initialo(0)
     o.step(3)
     upload_state(o, s, lay)
@@ -139,13 +160,14 @@ def test_parts_against_oracle_sections(mode):
                     g = s.download(0, b, name)
                     scale = max(float(np.abs(r).max()), 1e-300)
                     d = float(np.abs(g - r).max()) / scale
-                    worst[(part, name)] = max(worst.get((part, name), 0.0), d)
-                    assert d <= 1e-13, (nk, part, name, b, d)
+                    worst[(nk, part, name)] = max(worst.get((nk, part, name), 0.0), d)
             if part == "primitives":
+                # the stage's scalars (frame velocity and acceleration, time
+                # step) use correctly rounded sin, cos on both sides
                 so, sg = o.scalars(), s.scalars(0)
-                for k in ("dt", "dtk", "umf1", "umf2", "dudt1"):
-                    assert abs(so[k] - sg[k]) <= 1e-14 * max(abs(so[k]), 1e-300), (nk, k, so[k], sg[k])
+                for k in ("dt", "dtk", "dtko", "umf1", "umf2", "umf3", "dudt1", "dudt2", "dudt3"):
+                    assert so[k] == sg[k], (nk, k, so[k], sg[k])
     for k, v in sorted(worst.items()):
-        say(f"  {k[0]:16s} {k[1]:6s} {v:.2e}")
-    # the sections did something: the sponge saw the gust, the walls and junctions moved the state
-    assert any(v > 0 for v in worst.values()) or mode == "sequential"
+        say(f"  stage {k[0]} {k[1]:16s} {k[2]:6s} {v:.2e}")
+    bad = {k: v for k, v in worst.items() if v > 1e-13}
+    assert not bad, bad

@@ -2,8 +2,9 @@
 
 Each case plants one plausible mistake (a sign, a constant, a swapped index,
 a wrong weight) in a copy of oracle/tp_oracle.c, builds it into a temporary
-directory and runs the stage pins (tests/test_oracle_stage_pins.py) against
-that copy in a child process: at least one pin must fail.  CPU only.
+directory and runs the pins of that part (tests/test_oracle_stage_pins.py,
+tests/test_rounding_cpu.py) against that copy in a child process: at least one
+pin must fail.  CPU only.
 """
 import os
 import subprocess
@@ -23,8 +24,8 @@ MUTATIONS = {
     "stress transposed": ("double ntxy = r1 * (txy + hzz), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);\n          txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;\n        }",
                           "double ntxy = r1 * (txy + txy), ntyz = r1 * (tyz + hxx), ntzx = r1 * (tzx + hyy);\n          txx = ntxx; tyy = ntyy; tzz = ntzz; txy = ntxy; tyz = ntyz; tzx = ntzx;\n        }"),
     "viscous time-step limit": ("if (c->nviscous == 1) ra1 = half / fv;", "if (c->nviscous == 1) ra1 = 0.4 / fv;"),
-    "ramp period": ("double s = sin(0.05 * PI * (double)(n - (int)sc[ORC_S_NDT]));",
-                    "double s = sin(0.055 * PI * (double)(n - (int)sc[ORC_S_NDT]));"),
+    "ramp period": ("double s = crsin(0.05 * PI * (double)(n - (int)sc[ORC_S_NDT]));",
+                    "double s = crsin(0.055 * PI * (double)(n - (int)sc[ORC_S_NDT]));"),
     "incoming wave index": ("if (ra0 * (vv + e.ao) > zero) cha[3] = zero;\n              if (ra0 * (vv - e.ao) > zero) cha[4] = zero;",
                             "if (ra0 * (vv + e.ao) > zero) cha[4] = zero;\n              if (ra0 * (vv - e.ao) > zero) cha[3] = zero;"),
     "wall reflection sign": ("cha[3 + ipp] = cha[4 - ipp] + two * ra0", "cha[3 + ipp] = -cha[4 - ipp] + two * ra0"),
@@ -36,7 +37,17 @@ MUTATIONS = {
     "averaging weight": ("double r2 = one / rr[l];\n              rr[N + l] = r2;\n              for (int v = 0; v < 5; v++)\n                qa[(size_t)v * N + l] = r2 * ((rr[l] - one)",
                          "double r2 = 0.4;\n              rr[N + l] = r2;\n              for (int v = 0; v < 5; v++)\n                qa[(size_t)v * N + l] = r2 * ((rr[l] - one)"),
     "GCBC face weight": ("cnt[l] = cnt[l] + one; wgt[ll] = res; lst[ll] = l;", "cnt[l] = cnt[l] + one; wgt[ll] = 0.5 * res; lst[ll] = l;"),
+    # x**1.5 of the viscosity law: pinned by tests/test_rounding_cpu.py
+    "x**1.5 residual term dropped": ("double r = (S - P) + ((e - p) + x * q);", "double r = (S - P) + (e - p);"),
+    "x**1.5 Newton step doubled": ("return y + r / (y + y);", "return y + r / y;"),
+    # sin and cos of the per-stage scalars: pinned by tests/test_rounding_cpu.py
+    "sin/cos reduction without the third part of pi/2": ("t.h = -(k * P3); t.l = 0.0;", "t.h = 0.0; t.l = 0.0;"),
+    "sin/cos series cut at 1/13!": ("for (int n = 13; n >= 0; n--) {\n    ps = dd_add",
+                                    "for (int n = 5; n >= 0; n--) {\n    ps = dd_add"),
+    "sin/cos quadrant order": ("case 1: *sn = C; *cs = -S; break;", "case 1: *sn = -C; *cs = S; break;"),
 }
+# the pins a mutation is run against (default: the stage pins)
+PINS = {name: "test_rounding_cpu.py" for name in MUTATIONS if name.startswith(("x**1.5", "sin/cos"))}
 
 
 @pytest.mark.parametrize("name", list(MUTATIONS))
@@ -51,7 +62,7 @@ def test_planted_mistake_fails_a_pin(name, tmp_path):
                            "-I" + os.path.join(ROOT, "oracle"), "-shared", "-o", str(lib), str(mut), "-lm"])
     env = dict(os.environ, ORACLE_LIB=str(lib), OMP_NUM_THREADS="2")
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider",
-                        os.path.join(ROOT, "tests", "test_oracle_stage_pins.py")],
+                        os.path.join(ROOT, "tests", PINS.get(name, "test_oracle_stage_pins.py"))],
                        capture_output=True, text=True, env=env, cwd=ROOT, timeout=300)
     tail = r.stdout.strip().splitlines()[-1] if r.stdout.strip() else r.stderr[-500:]
     assert r.returncode != 0 and "failed" in tail, f"no pin caught '{name}': {tail}"

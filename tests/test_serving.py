@@ -36,6 +36,17 @@ def alone(lay, ss, p, nsteps, exact=True):
     return out
 
 
+def wait_staged(srv, n, limit_s=120.0):
+    """Wait until the staging thread has set n requests up, so that the
+    admissions of the loop that follows do not depend on how fast set-up
+    runs beside the steps."""
+    import time
+    t0 = time.monotonic()
+    while srv.stats()["staged"] < n:
+        assert time.monotonic() - t0 < limit_s, "set-up did not finish"
+        time.sleep(0.005)
+
+
 def state_diff(qs, rs):
     """Max difference of the conserved variables over the case, each variable
     relative to the largest of rho, |rho u| and E (rho a_inf scale)."""
@@ -150,13 +161,17 @@ def test_server_mix_equals_alone(mode):
     srv = api.Server(budget_bytes=0, max_active=2, graph=mode.endswith("graph"))
     if exact:
         srv.set_option(0, 1)
+    srv.set_staging(4)
     nsteps = [12, 7, 9, 10]
     reqs = [srv.submit(lay, p, ss, n, 0.0) for (lay, ss, p), n in zip(CASES, nsteps)]
+    wait_staged(srv, 4)
     srv.run(max_steps=5)
     st = srv.stats()
     say("  stats", {k: round(v, 4) for k, v in st.items()})
     assert st["finished"] == 4 and st["failed"] == 0
     assert st["peak_cases"] == 2 and st["joined_running"] >= 1
+    # every case was set up by the staging thread, at most four ahead
+    assert st["staged"] == 4 and st["stage_failed"] == 0 and 1 <= st["staged_peak"] <= 4
     assert 0 < st["p50_s"] <= st["p99_s"] <= st["max_s"]
     worst = 0.0
     for r, (lay, ss, p), n in zip(reqs, CASES, nsteps):
