@@ -165,6 +165,17 @@ int32_t tp_filte(tp_context *ctx, int32_t nn, int32_t nz);
  * once the solve has run. */
 int32_t tp_field_deriv(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc2, int32_t nn,
                        const double *src, double *dst, void *stream);
+/* tp_field_deriv without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc); no counterpart in the reference.  The tables go
+ * to a work area the library holds on each GPU (1 MiB, kept until the
+ * process ends): calls needing more (over about 4 M lines) fail with
+ * TP_E_SIZE.  The solve is only enqueued on stream, and the next call on
+ * the same GPU reuses the area: calls that overlap on one GPU (other
+ * streams, other host threads) race; calls in turn on one stream do not.
+ * Same results as tp_field_deriv, bit for bit. */
+int32_t tp_field_deriv_noalloc(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc2, int32_t nn,
+                               const double *src, double *dst, void *stream);
 
 /* ---- coarse entry points ---- */
 /* filtering of qa and qo = qa: src/Main3D.f90:554-561 */

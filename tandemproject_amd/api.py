@@ -43,7 +43,7 @@ EXPORTS = ["tp_create", "tp_destroy", "tp_last_error", "tp_add_case", "tp_finali
            "tp_case_dims", "tp_case_params", "tp_write_restart", "tp_read_restart", "tp_write_grid",
            "tp_read_grid", "tp_write_plot3d", "tp_get_grid", "tp_check_case",
            "tp_primitives", "tp_viscous", "tp_flux_divergence", "tp_sponge", "tp_gcbc", "tp_rk_update",
-           "tp_wall", "tp_junction", "tp_interface_avg", "tp_field_deriv", "tp_adopt_cases", "tp_get_option", "tp_server_set_staging",
+           "tp_wall", "tp_junction", "tp_interface_avg", "tp_field_deriv", "tp_field_deriv_noalloc", "tp_adopt_cases", "tp_get_option", "tp_server_set_staging",
            "tp_server_stats2", "tp_eddy_params",
            "tp_set_inflow_eddies"]
 # the sections of a stage, in the order tp_stage runs them (name, takes nk)
@@ -101,6 +101,7 @@ def load(path=LIB_PATH):
         "tp_gcbc": [P], "tp_rk_update": [P, i32], "tp_wall": [P], "tp_junction": [P],
         "tp_interface_avg": [P],
         "tp_field_deriv": [i32, i32, i32, pi32, i32, P, P, P],
+        "tp_field_deriv_noalloc": [i32, i32, i32, pi32, i32, P, P, P],
         "tp_eddy_params": [i32, pd, pd, pd],
         "tp_set_inflow_eddies": [P, i32, i32, pd],
         "tp_adopt_cases": [P, i32, ctypes.POINTER(P), pi32, pi32],
@@ -314,6 +315,15 @@ def field_deriv(dims3, nbc2, nn, src_ptr, dst_ptr, stream=None):
     nbc = np.ascontiguousarray(nbc2, np.int32)
     _check(load().tp_field_deriv(int(dims3[0]), int(dims3[1]), int(dims3[2]), _ip(nbc), int(nn),
                                  ctypes.c_void_p(src_ptr), ctypes.c_void_p(dst_ptr), ctypes.c_void_p(stream)))
+
+
+def field_deriv_noalloc(dims3, nbc2, nn, src_ptr, dst_ptr, stream=None):
+    """tp_field_deriv_noalloc on device addresses: no allocation, work area
+    in the library's per-GPU array; enqueued only (synchronise the stream)."""
+    nbc = np.ascontiguousarray(nbc2, np.int32)
+    _check(load().tp_field_deriv_noalloc(int(dims3[0]), int(dims3[1]), int(dims3[2]), _ip(nbc), int(nn),
+                                         ctypes.c_void_p(src_ptr), ctypes.c_void_p(dst_ptr),
+                                         ctypes.c_void_p(stream)))
 
 
 def check_case(layout, params):

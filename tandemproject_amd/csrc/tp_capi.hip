@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1578,6 +1579,8 @@ int32_t tp_set_inflow_eddies(tp_context *c, int32_t cs, int32_t nits, const doub
 const char *tp_last_error(void) { return g_err.c_str(); }
 
 namespace {
+int scratch_address(char **out);
+
 // the HIP runtime the process has loaded must be no older than the one the
 // library is built against: a process can hold another runtime first (an
 // application's bundled copy answers to the same soname), and this library
@@ -1609,6 +1612,14 @@ int32_t tp_create(tp_context **out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(TP_E_HIP, "hipStreamCreate failed");
+  }
+  // the work area of tp_field_deriv_noalloc on this device, looked up (and
+  // placed by the runtime) now rather than at that entry point's first call
+  char *ws = nullptr;
+  if (int rc = scratch_address(&ws)) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
   }
   *out = c;
   return 0;
@@ -2828,6 +2839,51 @@ int32_t tp_field_deriv(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc
   if (rc) return rc;
   if (e != hipSuccess) return fail(TP_E_HIP, std::string("tp_field_deriv: ") + hipGetErrorString(e));
   return 0;
+}
+
+// ---- the same without allocating: the work area in a device array ----
+// One array per GPU the library runs on, held until the process ends.  Its
+// name is only a host-side stand-in: host code uses the address that
+// hipGetSymbolAddress gives for the current device, looked up on the first
+// call on each device.  Calls that overlap on one GPU (other streams, other
+// host threads) share it and race; calls in turn on one stream are ordered.
+// Nothing in it needs to start at zero: every call writes the parts it reads.
+constexpr size_t kScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch[kMaxDevices] = {nullptr};
+
+int scratch_address(char **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(TP_E_HIP, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(TP_E_HIP, "hipGetSymbolAddress gave no address for the work area");
+    g_scratch[dev] = static_cast<char *>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+}  // namespace
+
+int32_t tp_field_deriv_noalloc(int32_t lxi, int32_t let, int32_t lze, const int32_t *nbc2, int32_t nn,
+                               const double *src, double *dst, void *stream) {
+  FieldCall f;
+  if (int rc = field_prepare(lxi, let, lze, nbc2, nn, src, dst, f)) return rc;
+  if (int rc = runtime_check()) return rc;
+  if (f.ws.total > kScratchBytes)
+    return fail(TP_E_SIZE, "tp_field_deriv_noalloc: the call needs " + std::to_string(f.ws.total) +
+                               " bytes of work area, the library holds " + std::to_string(kScratchBytes) +
+                               " (at most about 4 M lines); use tp_field_deriv");
+  char *ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  return field_launch(f, ws, static_cast<hipStream_t>(stream));
 }
 
 int32_t tp_gust_mode(double tu, double kr, const double *uoo3, double *out8, double *slit) {

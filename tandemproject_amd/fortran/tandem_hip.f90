@@ -196,6 +196,14 @@ module tandem_hip
       integer(c_int32_t), intent(in) :: nbc2(2)
       type(c_ptr), value :: src, dst, stream
     end function
+    ! the same without allocating (work area held by the library per GPU)
+    integer(c_int32_t) function tp_field_deriv_noalloc(lxi, let, lze, nbc2, nn, src, dst, stream) &
+        bind(C, name="tp_field_deriv_noalloc")
+      import :: c_ptr, c_int32_t
+      integer(c_int32_t), value :: lxi, let, lze, nn
+      integer(c_int32_t), intent(in) :: nbc2(2)
+      type(c_ptr), value :: src, dst, stream
+    end function
     integer(c_int32_t) function tp_get_option(ctx, option, value) bind(C, name="tp_get_option")
       import :: c_ptr, c_int32_t, c_int64_t
       type(c_ptr), value :: ctx
