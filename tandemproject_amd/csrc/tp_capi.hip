@@ -623,8 +623,9 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
               int acc, int nf = 1, int in_step = 0, int out_step = 0, int copy_arr = -1, int copy_col = 0,
               bool direct = false) {
   if (!c->L.lines[d].n) return;
-  // algorithmic bytes: one field in, one out; an accumulated output is also read
-  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, (acc ? 24.0 : 16.0) * (double)c->npts * nf);
+  // algorithmic bytes: one field in, one out; an accumulated output is also
+  // read, a copy of the output (the last filter's qo) also written
+  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, ((acc ? 24.0 : 16.0) + (copy_arr >= 0 ? 8.0 : 0.0)) * (double)c->npts * nf);
   LineArgs a;
   a.blocks = c->d_blocks; a.work = c->L.lines[d].d; a.coef = c->d_coef;
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
@@ -1748,6 +1749,8 @@ int32_t tp_add_case(tp_context *c, int32_t nb, const int32_t *dims, const int32_
                                 {fi.beta, fi.alpha, 1, fi.alpha, fi.beta}};
     std::memcpy(h.albef[1], inner, sizeof(inner));
   }
+  std::memcpy(k.flhs, h.albef, sizeof(k.flhs));
+  k.fal = fi.alpha; k.fbe = fi.beta;
   closure_rows(fltk, k);
   h.cbca_ra0 = gcbc_weights(k.sbci);
   k.fex[0] = 45 / 30.0; k.fex[1] = -9 / 30.0; k.fex[2] = 1 / 30.0;

@@ -441,33 +441,75 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     if (DIRECT) rows(fsum[0], fsum[1]);
     else rows(RcvFace{rcv, b.F[D], jk, 0}, RcvFace{rcv, b.F[D], jk, 1});
   }
-  // interior stencil on every row, then the closure rows patched in by the
-  // (at most two) chunks at each end that hold them, then the padding rows
-  // zeroed
+  // derivative: interior stencil on every row, then the closure rows patched
+  // in by the (at most two) chunks at each end that hold them; then the
+  // padding rows zeroed
   double r[C];
+  if (KIND == 1) {
+    // filter: the solve yields the filtered field itself, A out = R + A in
+    // (A the filter's left-hand side: interior rows (fbe, fal, 1, fal, fbe),
+    // closure rows of each end from flhs, mirrored at the far end), rather
+    // than the increment A^-1 R added to a second read of the input.  The
+    // closure rows' A in is formed first, from the window (its rows found by
+    // selects, not by indexing the register array), then every row's
+    // R + A in with the window dying row by row.
+    if (n > 0 && s0 <= 2) {
+      const double *hd = &c.flhs[b.ndf[D][0]][0][0];
+      double ah[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < C; k++) {
-    if (KIND == 0) {
-      const double aa = 0.6431406736919156, ab = 0.2586011023495066, ac = 0.007140953479797375;
-      r[k] = aa * (v[k + 4] - v[k + 2]) + ab * (v[k + 5] - v[k + 1]) + ac * (v[k + 6] - v[k]);
-    } else {
-      const double res = 2.0 * v[k + 3];
-      r[k] = c.fa * (v[k + 2] + v[k + 4] - res) + c.fb * (v[k + 1] + v[k + 5] - res) +
-             c.fc * (v[k] + v[k + 6] - res);
+      for (int k = 0; k < 3 && k < C; k++) {
+        const int i = s0 + k;   // row i sits at k; entries at columns i-2 .. i+2
+        if (i <= 2) {
+          const double *e = hd + 5 * i;
+          const double s = e[0] * v[k + 1] + e[1] * v[k + 2] + v[k + 3] + e[3] * v[k + 4] + e[4] * v[k + 5];
+          ah[0] = i == 0 ? s : ah[0]; ah[1] = i == 1 ? s : ah[1]; ah[2] = i == 2 ? s : ah[2];
+        }
+      }
+      rb0 = rb0 + ah[0]; rb1 = rb1 + ah[1]; rb2 = rb2 + ah[2];
     }
-  }
-  if (n > 0 && s0 <= 2) {
+    if (n > 0 && s0 + n - 1 >= L - 3) {
+      const double *tl = &c.flhs[b.ndf[D][1]][0][0];
+      double at0 = 0.0, at1 = 0.0, at2 = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3 && k < C; k++) {
-      const int i = s0 + k;
-      r[k] = i == 0 ? rb0 : i == 1 ? rb1 : i == 2 ? rb2 : r[k];
+      for (int k = 0; k < C; k++) {
+        const int i = s0 + k;   // mirrored: entry 4-q at column i+q-2
+        if (i >= L - 3 && i <= L - 1) {
+          const double *e = tl + 5 * (L - 1 - i);
+          const double s = e[4] * v[k + 1] + e[3] * v[k + 2] + v[k + 3] + e[1] * v[k + 4] + e[0] * v[k + 5];
+          at0 = i == L - 1 ? s : at0; at1 = i == L - 2 ? s : at1; at2 = i == L - 3 ? s : at2;
+        }
+      }
+      re0 = re0 + at0; re1 = re1 + at1; re2 = re2 + at2;
     }
-  }
-  if (n > 0 && s0 + n - 1 >= L - 3) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      const int i = s0 + k;
-      r[k] = i == L - 1 ? re0 : i == L - 2 ? re1 : i == L - 3 ? re2 : r[k];
+      const double res = 2.0 * v[k + 3];
+      r[k] = c.fa * (v[k + 2] + v[k + 4] - res) + c.fb * (v[k + 1] + v[k + 5] - res) +
+             c.fc * (v[k] + v[k + 6] - res) + (v[k + 3] + c.fal * (v[k + 2] + v[k + 4]) + c.fbe * (v[k + 1] + v[k + 5]));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      const double aa = 0.6431406736919156, ab = 0.2586011023495066, ac = 0.007140953479797375;
+      r[k] = aa * (v[k + 4] - v[k + 2]) + ab * (v[k + 5] - v[k + 1]) + ac * (v[k + 6] - v[k]);
+    }
+  }
+  // the closure rows patched in by the (at most two) chunks at each end that
+  // hold them
+  {
+    if (n > 0 && s0 <= 2) {
+#pragma unroll
+      for (int k = 0; k < 3 && k < C; k++) {
+        const int i = s0 + k;
+        r[k] = i == 0 ? rb0 : i == 1 ? rb1 : i == 2 ? rb2 : r[k];
+      }
+    }
+    if (n > 0 && s0 + n - 1 >= L - 3) {
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        const int i = s0 + k;
+        r[k] = i == L - 1 ? re0 : i == L - 2 ? re1 : i == L - 3 ? re2 : r[k];
+      }
     }
   }
   if (n < C) {
@@ -542,9 +584,9 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
     }
   }
 
-  // ---- store: filter adds the increment to the input, deriv may accumulate ----
-  const bool add = KIND == 1 || a.acc;
-  const double *ob = KIND == 1 ? src : dst;
+  // ---- store: the filtered field, or the derivative (which may accumulate) ----
+  const bool add = KIND == 0 && a.acc;
+  const double *ob = dst;
   double *cpy = KIND == 1 && a.copy_arr >= 0 ? field_ptr(b, a.copy_arr, a.copy_col + fi * a.out_step) : nullptr;
   // xi: the work item's lines are consecutive rows of the block, one
   // contiguous span of nl * L doubles.  Once every wave is done with the

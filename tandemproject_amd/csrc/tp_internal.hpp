@@ -42,6 +42,10 @@ struct CaseCoef {
   double pbco[2][2][LMP + 1];
   double pbcot[2][2];
   double fa, fb, fc, fam[3], fbm[3], fcm[3];
+  // the filter's left-hand side: rows 0..2 of a one-sided end (flhs[0]) and
+  // of an interface or periodic end (flhs[1]), entries at columns i-2..i+2;
+  // interior rows (fbe, fal, 1, fal, fbe)
+  double flhs[2][3][5], fal, fbe;
   double sbci[MBCI];
   double fex[3], ffx[3];
   double srefoo, srefp1dre, cfl, dto, wtemp, smago1, smago2;
