@@ -67,6 +67,22 @@ def state_diff(qs, rs):
     return max(float(np.abs(q[v] - r[v]).max() / scale[v]) for q, r in zip(qs, rs) for v in range(5))
 
 
+def run_logged(srv, max_steps, label, every_s=20.0):
+    """The request loop in pieces of at most max_steps steps, with a progress
+    line every every_s seconds (a long ensemble run prints nothing else)."""
+    t0 = last = time.perf_counter()
+    while True:
+        st = srv.iterate(max_steps)
+        if st["active"] == 0 and st["queued"] == 0 and st["waiting"] == 0:
+            return
+        if st["active"] == 0:
+            time.sleep(0.001)
+        now = time.perf_counter()
+        if now - last > every_s:
+            log(f"{label}: {st['done']} done, {st['active']} on the device, {now - t0:.0f} s")
+            last = now
+
+
 def ensemble_bench(ncase, nsteps, nplain):
     """cases/h of the batched ensemble (every case in one request loop,
     advanced in shared launches) against the plain path (one case at a time,
@@ -79,7 +95,7 @@ def ensemble_bench(ncase, nsteps, nplain):
     srv = api.Server(budget_bytes=0, max_active=ncase, graph=True)
     t0 = time.perf_counter()
     reqs = [srv.submit(lay, p, ss, nsteps, 0.0) for lay, ss, p in cases]
-    srv.run(max_steps=nsteps)
+    run_logged(srv, min(nsteps, 50), "ensemble")
     wall_b = time.perf_counter() - t0
     st = srv.stats()
     step_s = st["stepping_s"] / nsteps
