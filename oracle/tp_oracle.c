@@ -23,6 +23,14 @@
  *                                                                  interior transfer function
  *   eleme/xtq2r/xtr2q   src/Subroutines3D.f90:235-324              X_R * X_L = I, X_L A X_R diagonal
  *   metrics, normals    src/Main3D.f90:365-439                     exact metrics of affine grids, yaco = 1/J
+ *   sponge mask         src/SingleTurbulence3D.f90:209-249         zone edges from the domain geometry (the
+ *                                                                  single-precision 0.7e0 literal included),
+ *                                                                  1 - sigma/szco = product of cos^2(pi r/2),
+ *                                                                  lambda's ends; mutations caught
+ *   extrabcc            src/SingleTurbulence3D.f90:447-464         slot points = the wall points inside slots
+ *                                                                  of width 0.4 wlea centred a quarter pitch
+ *                                                                  into each whole pitch of the span, along
+ *                                                                  the first 2 wlea of chord; mutations caught
  *   stage sections (orc_stage_part; tests/test_oracle_stage_pins.py, every pin checked to fail on a
  *   planted mistake in tests/test_oracle_mutations.py):
  *     BEGIN             src/Main3D.f90:571-622                     CFL and viscous limits of a Cartesian grid
@@ -60,10 +68,19 @@
  *                                                                  decided in exact integer arithmetic; libm
  *                                                                  pow within one unit in the last place
  *
- * Parity unpinned (nothing outside this file fixes it): the order in which
- * orc_stage composes its sections (src/Main3D.f90:567-958, the reference's
- * statement order), the sponge mask geometry (SingleTurbulence3D.f90:209-249,
- * a profile the deck defines) and the wall-slot flags (extrabcc, :447-464).
+ *   orc_stage (order)   src/Main3D.f90:567-958                     after one whole stage: the interior residual of
+ *                                                                  a viscous linear field (stresses before fluxes),
+ *                                                                  zero incoming amplitudes at open faces (faces
+ *                                                                  after fluxes), state = qo - dtk yaco de (update
+ *                                                                  after the residual), wall conditions and
+ *                                                                  continuous interfaces (wall, averaging after
+ *                                                                  the update); mutations caught
+ *
+ * Parity unpinned (nothing outside this file fixes it): in orc_stage, the
+ * sponge before the characteristic section and the junction before the
+ * interface averaging (the reference's statement order).  The first pair
+ * commutes except where two copies of an interface point differ; the second
+ * differs only at the corner points the two share.
  * The reference program itself cannot be built here (it needs an MPI
  * Fortran module).
  *
@@ -2363,10 +2380,15 @@ static void stage_avg(orc *c, int nk) {
   }
 }
 
-/* One stage, src/Main3D.f90:567-958: its sections in the reference's order. */
+/* The stage's sections, indexed by ORC_P_*. */
 static void (*const stage_parts[ORC_P_NPART])(orc *, int) = {
     stage_begin, stage_viscous, stage_flux, stage_sponge, stage_gcbc,
     stage_rk, stage_wall, stage_junction, stage_avg};
+
+/* One stage, src/Main3D.f90:567-958: the sections in the reference's order. */
+static const int stage_order[ORC_P_NPART] = {
+    ORC_P_BEGIN, ORC_P_VISCOUS, ORC_P_FLUX, ORC_P_SPONGE, ORC_P_GCBC,
+    ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};
 
 int orc_stage_part(orc *c, int nk, int part) {
   if (nk < 1 || nk > c->nkrk || part < 0 || part >= ORC_P_NPART) {
@@ -2378,7 +2400,7 @@ int orc_stage_part(orc *c, int nk, int part) {
 }
 
 void orc_stage(orc *c, int nk) {
-  for (int part = 0; part < ORC_P_NPART; part++) stage_parts[part](c, nk);
+  for (int q = 0; q < ORC_P_NPART; q++) stage_parts[stage_order[q]](c, nk);
 }
 
 void orc_step(orc *c, int nsteps) {

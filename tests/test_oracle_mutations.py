@@ -37,14 +37,26 @@ MUTATIONS = {
     "averaging weight": ("double r2 = one / rr[l];\n              rr[N + l] = r2;\n              for (int v = 0; v < 5; v++)\n                qa[(size_t)v * N + l] = r2 * ((rr[l] - one)",
                          "double r2 = 0.4;\n              rr[N + l] = r2;\n              for (int v = 0; v < 5; v++)\n                qa[(size_t)v * N + l] = r2 * ((rr[l] - one)"),
     "GCBC face weight": ("cnt[l] = cnt[l] + one; wgt[ll] = res; lst[ll] = l;", "cnt[l] = cnt[l] + one; wgt[ll] = 0.5 * res; lst[ll] = l;"),
-    # x**1.5 of the viscosity law: pinned by tests/test_rounding_cpu.py
-    "x**1.5 residual term dropped": ("double r = (S - P) + ((e - p) + x * q);", "double r = (S - P) + (e - p);"),
-    "x**1.5 Newton step doubled": ("return y + r / (y + y);", "return y + r / y;"),
-    # sin and cos of the per-stage scalars: pinned by tests/test_rounding_cpu.py
-    "sin/cos reduction without the third part of pi/2": ("t.h = -(k * P3); t.l = 0.0;", "t.h = 0.0; t.l = 0.0;"),
-    "sin/cos series cut at 1/13!": ("for (int n = 13; n >= 0; n--) {\n    ps = dd_add",
-                                    "for (int n = 5; n >= 0; n--) {\n    ps = dd_add"),
-    "sin/cos quadrant order": ("case 1: *sn = C; *cs = -S; break;", "case 1: *sn = -C; *cs = S; break;"),
+    "sponge outflow zone without the extension": ("szr[1][0] = 1 / (c->szth1 + c->szxt);", "szr[1][0] = 1 / c->szth1;"),
+    "sponge edge literal taken as double": ("double p07 = (double)0.7f;", "double p07 = 0.7;"),
+    "sponge upper eta edge": ("szp[1][1] = c->domlen + c->szxt - c->szth2;", "szp[1][1] = c->domlen - c->szth2;"),
+    "sponge profile factor": ("(one - 0.125 * (one + cos(PI * r[0]))", "(one - 0.25 * (one + cos(PI * r[0]))"),
+    "sponge streamwise weight period": ("double ra1 = 0.25, ra2 = PI / (two * c->domlen);", "double ra1 = 0.25, ra2 = PI / c->domlen;"),
+    "sponge energy weight": ("B->bsz[q] = res * hamm1 * sg[l];", "B->bsz[q] = res * ham * sg[l];"),
+    "wall slots: one pitch too many": ("int ie = (int)((c->span + sml) / c->wlew) - 1;", "int ie = (int)((c->span + sml) / c->wlew);"),
+    "wall slots: quarter-pitch offset dropped": ("double ra1 = -half * c->span + 0.25 * c->wlew - half * slitw, ra2",
+                                                 "double ra1 = -half * c->span - half * slitw, ra2"),
+    "wall slots: chordwise extent": ("B->xbc_ra0 = -half + c->wlea + slitl;", "B->xbc_ra0 = -half + c->wlea;"),
+    "wall slots: width": ("double slitl = c->wlea, slitw = 0.4 * slitl;", "double slitl = c->wlea, slitw = 0.2 * slitl;"),
+    "wall slots: switch ignored": ("if (res < zero && c->nextrabc == 1) flag = one;", "if (res < zero) flag = one;"),
+    # the order of the sections in a whole stage (orc_stage only; the
+    # sections themselves are unchanged)
+    "stage: stresses after the fluxes": ("ORC_P_BEGIN, ORC_P_VISCOUS, ORC_P_FLUX,", "ORC_P_BEGIN, ORC_P_FLUX, ORC_P_VISCOUS,"),
+    "stage: characteristic faces before the fluxes": ("ORC_P_FLUX, ORC_P_SPONGE, ORC_P_GCBC,\n    ORC_P_RK,",
+                                                      "ORC_P_GCBC, ORC_P_FLUX, ORC_P_SPONGE,\n    ORC_P_RK,"),
+    "stage: update before the characteristic faces": ("ORC_P_GCBC,\n    ORC_P_RK, ORC_P_WALL,", "ORC_P_RK,\n    ORC_P_GCBC, ORC_P_WALL,"),
+    "stage: wall before the update": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_WALL, ORC_P_RK, ORC_P_JUNCTION, ORC_P_AVG};"),
+    "stage: averaging before the update": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_AVG, ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION};"),
 }
 # the pins a mutation is run against (default: the stage pins)
 PINS = {name: "test_rounding_cpu.py" for name in MUTATIONS if name.startswith(("x**1.5", "sin/cos"))}

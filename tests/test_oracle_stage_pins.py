@@ -106,10 +106,14 @@ def linear_field_expected(X, Y, G, u0, p0, gp, umf, mu):
     return out
 
 
-@pytest.mark.parametrize("viscous,nsmf", [(0, 0), (0, 1), (1, 0), (1, 1)],
-                         ids=["euler", "euler-frame", "viscous", "viscous-frame"])
-def test_flux_divergence_linear_fields(viscous, nsmf):
-    """yaco * de after the flux section == div F of the analytic fluxes."""
+@pytest.mark.parametrize("viscous,nsmf,whole", [(0, 0, False), (0, 1, False), (1, 0, False), (1, 1, False),
+                                                (1, 0, True)],
+                         ids=["euler", "euler-frame", "viscous", "viscous-frame", "viscous-whole-stage"])
+def test_flux_divergence_linear_fields(viscous, nsmf, whole):
+    """yaco * de after the flux section == div F of the analytic fluxes.
+    After a whole stage (orc_stage) the same holds away from the faces: the
+    stresses are computed before the fluxes that use them, and nothing after
+    the fluxes changes the interior residual."""
     nx, ny, nz = 22, 19, 13
     lay, X, Y, Z, M = affine_block(nx, ny, nz, [[10, 10], [10, 10], [45, 45]])
     o, p = make(lay, (X, Y, Z), nviscous=viscous, nsmf=nsmf)
@@ -122,8 +126,11 @@ def test_flux_divergence_linear_fields(viscous, nsmf):
     u = u0[:, None] + G @ r
     pr = p0 + gp[0] * X + gp[1] * Y
     set_state(o, conserved(rho, u[0], u[1], u[2], pr))
-    for part in ("begin", "viscous", "flux"):
-        o.stage_part(1, part)
+    if whole:
+        o.stage(1)
+    else:
+        for part in ("begin", "viscous", "flux"):
+            o.stage_part(1, part)
     umf = np.array([o.scalars()[k] for k in ("umf1", "umf2", "umf3")])
     if nsmf:
         assert np.abs(umf).max() > 0.1   # the frame moves at the free-stream speed
@@ -567,3 +574,255 @@ def test_sponge_isentropic_target():
             d = dq[(b, l)]
             np.testing.assert_allclose(de[0:4], a * d[0:4], rtol=1e-9, atol=1e-15 * a)
             np.testing.assert_allclose(de[4], bb * d[4], rtol=1e-8, atol=1e-15 * bb)
+
+
+@pytest.mark.parametrize("nbc_eta", [(10, 10), (25, 10)], ids=["eta-open", "eta-wall-below"])
+def test_sponge_mask_geometry(nbc_eta):
+    """The sponge strength sigma and the streamwise weight lambda of
+    src/SingleTurbulence3D.f90:209-249, from the zone geometry alone:
+
+    * a zone lies along each open face (code 10) of the xi and eta
+      directions, none along zeta;
+    * the inflow zone reaches full strength at x = -0.7 domlen and starts
+      szth1 downstream of it, the outflow zone starts szth1 before x = domlen
+      and reaches full strength at domlen + szxt; in eta the zones are
+      szth2 thick, full at y = -0.7 domlen and at y = domlen + szxt.  The
+      literal 0.7e0 is default real: its single-precision value
+      0.699999988079071 places the edges;
+    * with depth r in [0, 1] into each zone, 1 - sigma / szco is the product
+      of cos^2(pi r / 2) over the directions;
+    * lambda = 1/4 + 3/4 cos^2(pi (min(x, domlen) + domlen) / (4 domlen)),
+      1 at x = -domlen and 1/4 from x = domlen on;
+    * the sponge list holds the points with sigma > 0, asz = sigma lambda /
+      yaco and bsz = sigma / ((gamma - 1) yaco)."""
+    dl, t1, t2, xt, szco = 1.0, 0.25, 0.2, 0.3, 7.0
+    p07 = float(np.float32(0.7))
+    x_in = -0.7 * dl + t1          # the inflow edge, were the literal double
+    h, nx, ny, nz = 0.05, 50, 45, 13
+    i0 = 12
+    x1 = x_in + h * (np.arange(nx) - i0)      # x from -1.05, one column exactly on that edge
+    y1 = -0.8 + h * np.arange(ny)
+    lay = L.single_block(nx - 1, ny - 1, nz - 1, [[10, 10], list(nbc_eta), [10, 10]])
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    X, Y, Z = x1[i].ravel(), y1[j].ravel(), (0.03 * k).ravel()
+    o, _ = make(lay, (X, Y, Z), domlen=dl, szth1=t1, szth2=t2, szxt=xt, szco=szco)
+    # depth into each zone, from its start (0) to full strength (1)
+    full_in, start_in = -p07 * dl, -p07 * dl + t1
+    start_out, full_out = dl - t1, dl + xt
+    rx = np.clip((start_in - X) / (start_in - full_in), 0, None) + \
+        np.clip((X - start_out) / (full_out - start_out), 0, None)
+    lo_open = nbc_eta[0] == 10
+    start_lo, full_lo = -p07 * dl + t2, -p07 * dl
+    start_hi, full_hi = dl + xt - t2, dl + xt
+    ry = lo_open * np.clip((start_lo - Y) / (start_lo - full_lo), 0, None) + \
+        np.clip((Y - start_hi) / (full_hi - start_hi), 0, None)
+    c2 = lambda r: np.cos(0.5 * np.pi * np.minimum(r, 1.0)) ** 2
+    # past full strength (r > 1, up to 1.4 on this grid) the profile is not
+    # held at szco: the cosine turns back; those points are checked apart
+    inside = (rx <= 1) & (ry <= 1)
+    sigma = szco * (1 - c2(rx) * c2(ry))
+    lam = 0.25 + 0.75 * np.cos(np.pi * (np.minimum(X, dl) + dl) / (4 * dl)) ** 2
+    ls, _ = o.sponge_list(0)
+    yaco = o.array(0, "yaco")
+    asz, bsz = o.array(0, "asz"), o.array(0, "bsz")
+    # membership, away from the zone starts (where round-off decides)
+    member = np.zeros(X.size, bool); member[ls] = True
+    near = (np.abs(X - start_in) < 1e-9) | (np.abs(X - start_out) < 1e-9) | \
+        (np.abs(Y - start_lo) < 1e-9) | (np.abs(Y - start_hi) < 1e-9)
+    np.testing.assert_array_equal(member[~near], (sigma > 0)[~near])
+    # the column on the double-literal edge is 1.2e-8 deep in the zone
+    col = np.abs(X - x_in) == 0
+    assert col.sum() == ny * nz
+    assert member[col].all() and not member[(np.abs(X - (x_in + h)) == 0) & (ry == 0)].any()
+    np.testing.assert_array_equal(np.diff(ls) > 0, True)
+    # strengths and weights
+    s_got = asz * yaco[ls] / lam[ls]
+    sel = inside[ls]
+    np.testing.assert_allclose(s_got[sel], sigma[ls][sel], rtol=1e-13, atol=1e-14 * szco)
+    np.testing.assert_allclose(bsz * yaco[ls] * (GAM - 1), s_got, rtol=1e-13, atol=1e-14 * szco)
+    # full strength at the full-strength positions, the reference's cosine
+    # profile past them
+    at_full = (np.abs(X - full_in) < 1e-9) | (np.abs(X - full_out) < 1e-9)
+    np.testing.assert_allclose(s_got[at_full[ls]], szco, rtol=1e-13)
+    past = ~inside[ls]
+    assert past.any()
+    cr = lambda r: 0.5 * (1 + np.cos(np.pi * r))
+    np.testing.assert_allclose(s_got[past], (szco * (1 - cr(rx) * cr(ry)))[ls][past], rtol=1e-13)
+    assert (s_got <= 2 * szco).all() and (s_got > 0).all()
+    # lambda's ends
+    assert lam.min() == 0.25 and abs(lam[np.abs(X + dl) < 1e-9] - 1).max() < 1e-15
+
+
+def test_wall_slot_flags():
+    """Wall slots (src/SingleTurbulence3D.f90:447-464) with the leading
+    edge at x = -1/2: slots of width 0.4 wlea along the first 2 wlea of the
+    chord, centred at z = -span/2 + wlew/4 + m wlew for each whole slot pitch
+    wlew that fits in the span (m = 0, 1, ...).  A wall point is a slot point
+    when it lies inside one (the chordwise test uses the limit the previous
+    call left, so the first point of a block is checked for the z position
+    only).  With nextrabc = 0 there are none."""
+    wlea, wlew, span = 0.1, 0.05, 0.2
+    lay, ss, p = aerofoil_case(nextrabc=1, wlea=wlea, wlew=wlew, span=span)
+    ss = [np.concatenate([x[:x.size // 3] - 0.5, x[x.size // 3:]]) for x in ss]
+    centres = -span / 2 + wlew / 4 + wlew * np.arange(int(span / wlew + 1e-9))
+    half_w = 0.2 * wlea
+    for nextrabc in (1, 0):
+        o = po.Oracle(lay, dict(p, nextrabc=nextrabc))
+        for b in range(lay.nb):
+            o.set_grid(b, ss[b])
+        o.metrics()
+        slots = 0
+        for b in range(lay.nb):
+            nx, ny, nz = lay.dims[b] + 1
+            rp = o.array(b, "rpex")
+            ls = np.concatenate([face_points(nx, ny, nz, d, s)[0] for d in range(3) for s in range(2)
+                                 if lay.nbc[b][d][s] in (20, 25)] or [np.zeros(0, int)])
+            assert rp.size == ls.size
+            if not ls.size:
+                continue
+            x, y, z = ss[b].reshape(3, -1)[:, ls]
+            dz = np.abs(z[:, None] - centres[None, :]).min(1)
+            assert (np.abs(dz - half_w) > 1e-9).all()
+            in_z = dz < half_w
+            want = nextrabc * (in_z & (x < -0.5 + 2 * wlea))
+            np.testing.assert_array_equal(rp[1:], want[1:])
+            assert rp[0] in (0, nextrabc * in_z[0])
+            slots += int(rp.sum())
+        if nextrabc:
+            assert slots > 20
+        else:
+            assert slots == 0
+
+
+@pytest.mark.parametrize("case", ["open-faces", "wall-in-sponge"])
+def test_whole_stage_residual_then_update(case):
+    """The order of a whole stage (orc_stage, src/Main3D.f90:567-958) before
+    the update, from what holds after one stage:
+
+    * open-faces: no sponge; the incoming characteristic amplitude of the
+      residual is zero at the open xi faces (the characteristic treatment
+      follows the fluxes);
+    * wall-in-sponge: a reflecting wall (code 25) partly inside the sponge
+      zones, with no normal velocity at rest: the incoming amplitude of the
+      residual equals the outgoing one at the wall, sponge term included.
+      (Whether the sponge comes before or after the characteristic section
+      cannot be seen here: at a wall with no normal velocity the sponge
+      term's two acoustic amplitudes are equal, and at open faces the
+      section works on the stored flux derivative alone);
+    * both: the state is qo - dtk yaco de with that final residual away from
+      the wall (the update follows the complete residual)."""
+    nx, ny, nz = 18, 16, 13
+    wall = case == "wall-in-sponge"
+    nbc = [[10, 10], [25 if wall else 10, 10], [45, 45]]
+    lay, X, Y, Z, M = affine_block(nx, ny, nz, nbc, shear=0.0, y0=0.0 if wall else -0.3)
+    if wall:
+        o, p = make(lay, (X, Y, Z), nviscous=0, szco=5.0, domlen=0.5, szth1=0.3, szth2=0.3, szxt=0.2)
+        rho, u, v, w, pr = smooth_state(X, Y, Z)
+        v = 0.4 * Y * (1 + 0.2 * np.sin(2 * X))   # v = 0 on the wall y = 0
+        pr = pr + 0.03 * np.cos(3.0 * Y)
+    else:
+        o, p = make(lay, (X, Y, Z), nviscous=0)
+        rho = 1.0 + 0.05 * np.sin(2.1 * X + 0.3)
+        u = 0.3 + 0.04 * np.cos(1.3 * X)
+        v = 0.03 * np.sin(1.1 * X)
+        pr = 1 / GAM + 0.02 * np.cos(1.9 * X)
+        w = 0.02 * np.cos(0.9 * X + 0.4)
+    q0 = conserved(rho, u, v, w, pr)
+    set_state(o, q0)
+    for k, val in (("n", 20), ("ndt", 10), ("dts", 2e-3), ("dte", 2e-3)):
+        o.set_scalar(k, val)   # a step of 2e-3, past the start-up ramp
+    o.stage(1)
+    sc = o.scalars()
+    # a frame at rest whose start-up acceleration is along x, tangent to the wall
+    assert sc["umf1"] == sc["umf2"] == sc["umf3"] == 0 and sc["dudt2"] == sc["dudt3"] == 0 and sc["dtk"] > 0
+    yaco = o.array(0, "yaco")
+    de = o.array(0, "de").reshape(5, -1)
+    q0 = q0.reshape(5, -1)
+    pr0 = (GAM - 1) * (q0[4] - 0.5 * (q0[1] ** 2 + q0[2] ** 2 + q0[3] ** 2) / q0[0])
+    off_wall = np.ones(X.size, bool)
+    if wall:
+        off_wall[face_points(nx, ny, nz, 1, 0)[0]] = False
+    np.testing.assert_allclose(o.array(0, "qa").reshape(5, -1)[:, off_wall],
+                               (q0 - sc["dtk"] * yaco * de)[:, off_wall], rtol=1e-14, atol=1e-15)
+    ls_sponge, _ = o.sponge_list(0)
+    d, sides, other, n_o = (1, (0,), 0, nx) if wall else (0, (0, 1), 1, ny)
+    F = nz * nx if wall else ny * nz
+    cm = o.array(0, f"cm{d + 1}").reshape(2, 3, F)
+    checked = in_sponge = 0
+    for s_ in sides:
+        ls, jk, ii = face_points(nx, ny, nz, d, s_)
+        for l, f, io in zip(ls, jk, ii[other]):
+            if io <= 3 or io >= n_o - 4:
+                continue   # in the band of another face too
+            XL = po.xtq2r(q0[:, l], pr0[l], cm[s_, :, f], np.zeros(3))
+            wv = XL @ (yaco[l] * de[:, l])
+            if wall:
+                assert abs(wv[3] - wv[4]) <= 1e-12 * np.abs(wv).max(), wv
+                assert abs(wv[4]) > 1e-3 * np.abs(wv).max()
+            else:
+                inc = 3 if s_ == 0 else 4
+                assert abs(wv[inc]) <= 1e-12 * np.abs(wv).max(), (s_, l, wv)
+            checked += 1
+            in_sponge += int(l in ls_sponge)
+    assert checked > 50
+    assert (in_sponge > 20) if wall else (in_sponge == 0)
+
+
+def test_whole_stage_boundary_sections_last():
+    """The order of a whole stage after the update: the wall conditions
+    (no-slip, gamma p / rho = T_wall) and continuous interfaces hold after
+    one whole stage, so the wall and averaging sections follow the update
+    (src/Main3D.f90:873-952)."""
+    lay, ss, p = aerofoil_case(wtemp=1.07)
+    o = po.Oracle(lay, p)
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b])
+    o.metrics()
+    rng = np.random.default_rng(5)
+    for b in range(lay.nb):
+        x, y, z = ss[b].reshape(3, -1)
+        rho, u, v, w, pr = smooth_state(x, y, z)
+        q = conserved(rho, u, v, w, pr).reshape(5, -1)
+        q = q * (1 + 0.002 * rng.standard_normal(q.shape))
+        o.array(b, "qa")[:] = q.ravel()
+        o.array(b, "qo")[:] = q.ravel()
+    for k, val in (("n", 20), ("ndt", 10), ("dts", 1e-3), ("dte", 1e-3)):
+        o.set_scalar(k, val)
+    o.stage(1)
+    sc = o.scalars()
+    assert sc["dtk"] > 0
+    umf = np.array([sc["umf1"], sc["umf2"], sc["umf3"]])
+    dims = lay.dims + 1
+    walls = 0
+    for b in range(lay.nb):
+        for d in range(3):
+            for s in range(2):
+                if lay.nbc[b][d][s] not in (20, 25):
+                    continue
+                ls, jk, ii = face_points(*dims[b], d, s)
+                inner = np.ones(ls.size, bool)
+                for e in range(3):
+                    if e != d:
+                        inner &= (ii[e] > 0) & (ii[e] < dims[b][e] - 1)
+                q = o.array(b, "qa").reshape(5, -1)[:, ls[inner]]
+                pr = (GAM - 1) * (q[4] - 0.5 * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2) / q[0])
+                np.testing.assert_allclose(GAM * pr / q[0], 1.07, rtol=1e-13)
+                np.testing.assert_allclose(q[1:4] / q[0] + umf[:, None], 0, atol=1e-15)
+                walls += inner.sum()
+    assert walls > 100
+    pairs = 0
+    for b in range(lay.nb):
+        for d in range(3):
+            if lay.nbc[b][d][1] not in (30, 45):
+                continue
+            nb = lay.ncd[b][d][1]
+            la, _, ia = face_points(*dims[b], d, 1)
+            lb, _, _ = face_points(*dims[nb], d, 0)
+            inner = np.ones(la.size, bool)
+            for e in range(3):
+                if e != d:
+                    inner &= (ia[e] > 0) & (ia[e] < dims[b][e] - 1)
+            np.testing.assert_array_equal(o.array(b, "qa").reshape(5, -1)[:, la[inner]],
+                                          o.array(nb, "qa").reshape(5, -1)[:, lb[inner]])
+            pairs += 1
+    assert pairs >= 8
