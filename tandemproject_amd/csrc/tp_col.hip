@@ -609,14 +609,38 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
       }
       __syncthreads();
       const int rows = min(R, nl - h * R);
+      const int tot = rows * L;
       const unsigned base0 = (unsigned)(w.a + h * R) * (unsigned)b.nx;
-      for (int idx = threadIdx.x; idx < rows * L; idx += blockDim.x) {
-        const int row = idx / L;
-        const unsigned o = 8u * (base0 + (unsigned)idx);
-        const double x = tile[row * pitch + idx - row * L];
-        const double y = add ? ld_stream(&at(ob, o)) + x : x;
-        st(dst, o, y);
-        if (KIND == 1 && cpy) st(cpy, o, y);
+      // element idx of the run sits at (row, col) = (idx / L, idx % L) of the
+      // tile; each thread steps by blockDim.x = dq L + dr elements, so
+      // (row, col) advance without a division, four elements at a time
+      const int dq = (int)blockDim.x / L, dr = (int)blockDim.x - dq * L;
+      int row = (int)threadIdx.x / L, col = (int)threadIdx.x - row * L;
+      for (int i0 = threadIdx.x; i0 < tot; i0 += 4 * (int)blockDim.x) {
+        double x4[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          x4[u] = i0 + u * (int)blockDim.x < tot ? tile[row * pitch + col] : 0.0;
+          col += dr;
+          row += dq;
+          if (col >= L) { col -= L; row++; }
+        }
+        double o4[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int idx = i0 + u * (int)blockDim.x;
+          o4[u] = add && idx < tot ? ld_stream(&at(ob, 8u * (base0 + (unsigned)idx))) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int idx = i0 + u * (int)blockDim.x;
+          if (idx < tot) {
+            const unsigned o = 8u * (base0 + (unsigned)idx);
+            const double y = add ? o4[u] + x4[u] : x4[u];
+            st(dst, o, y);
+            if (KIND == 1 && cpy) st(cpy, o, y);
+          }
+        }
       }
       __syncthreads();   // the tile is read before the next rows overwrite it
     }
