@@ -801,12 +801,14 @@ void enqueue_filter_step(tp_context *c) {
 
 // stage start: frame velocity and inflow weights (:571-572), primitives
 // (:576-583), time step at the first stage (:587-622)
-void enqueue_begin(tp_context *c, int nk) {
+// prim_ready: the previous stage's update already left this stage's
+// primitives (enqueue_rk with prim)
+void enqueue_begin(tp_context *c, int nk, bool prim_ready = false) {
   const int ncase = (int)c->linked.size();
   launch_sc_stage(c->d_coef, c->d_sc, ncase, nk, c->stream);
   StageArgs a = base_args(c);
   a.nk = nk;
-  run_points(c, PK_PRIM, a, K_POINT, 96.0);
+  if (!prim_ready) run_points(c, PK_PRIM, a, K_POINT, 96.0);
   if (nk == 1) {
     run_points(c, PK_DTRED, a, K_POINT, 0.0);
     launch_sc_dt(c->d_coef, c->d_sc, c->d_part, c->d_wr, ncase, 1, c->stream);
@@ -872,34 +874,49 @@ void enqueue_gcbc(tp_context *c) {
 }
 
 // frame velocity for the update and the RK update, src/Main3D.f90:873-881
-void enqueue_rk(tp_context *c, int nk) {
+// prim: the update also leaves the next stage's primitives (de(:,1:5), p,
+// ss(:,1) of the new state, k_primitives' arithmetic), and the wall, junction
+// and averaging passes after it form them again where they change the state,
+// so that the next stage starts without the primitives pass and its read of
+// qa (the residual in de is read first)
+void enqueue_rk(tp_context *c, int nk, bool prim = false) {
   launch_sc_rk(c->d_coef, c->d_sc, (int)c->linked.size(), nk, c->stream);
   StageArgs a = base_args(c);
   a.nk = nk;
-  run_points(c, PK_RK, a, K_RK, 128.0);
+  if (prim) run_points(c, PK_RK_PRIM, a, K_RK, 184.0);
+  else run_points(c, PK_RK, a, K_RK, 128.0);
 }
 
 // wall temperature and velocity, src/Main3D.f90:884-895
-void enqueue_wall(tp_context *c) { run_faces(c, c->L.wall, FK_WALL, base_args(c), K_GCBC); }
+void enqueue_wall(tp_context *c, bool prim = false) {
+  StageArgs a = base_args(c);
+  a.opt = prim ? 1 : 0;
+  run_faces(c, c->L.wall, FK_WALL, a, K_GCBC);
+}
 
 // corner junctions, src/SingleTurbulence3D.f90:378-443
-void enqueue_junction(tp_context *c) {
-  if (c->njct) launch_junction(c->d_blocks, c->d_jct, c->d_jvel, c->njct, c->maxk, c->stream);
+void enqueue_junction(tp_context *c, bool prim = false) {
+  if (c->njct)
+    launch_junction(c->d_blocks, c->d_coef, c->d_jct, c->d_jvel, c->njct, c->maxk, prim ? 1 : 0, c->stream);
 }
 
 // interface averaging, src/Main3D.f90:897-952
-void enqueue_avg(tp_context *c) {
-  const StageArgs a = base_args(c);
+void enqueue_avg(tp_context *c, bool prim = false) {
+  StageArgs a = base_args(c);
   run_faces(c, c->L.acap, FK_AVG_CAP, a, K_GCBC);
   run_faces(c, c->L.acap, FK_COPY, a, K_GCBC);
+  a.opt = prim ? 1 : 0;
   for (int d = 0; d < 3; d++) run_faces(c, c->L.aapply[d], FK_AVG_APPLY, a, K_GCBC);
 }
 
 // one stage, src/Main3D.f90:567-958: the stresses and heat fluxes stay in
 // registers of the pass that forms the fluxes (tv is written only where it
 // is read, or everywhere with TP_OPT_STRESSES)
-void enqueue_stage(tp_context *c, int nk) {
-  enqueue_begin(c, nk);
+// prim_ready: the previous stage left this one's primitives; prim_next: leave
+// the next stage's (inside a time step only: after a single stage the state
+// is the reference's, with the residual in de)
+void enqueue_stage(tp_context *c, int nk, bool prim_ready = false, bool prim_next = false) {
+  enqueue_begin(c, nk, prim_ready);
   enqueue_gradients(c);
   {
     StageArgs f = base_args(c);
@@ -911,16 +928,18 @@ void enqueue_stage(tp_context *c, int nk) {
   enqueue_flux_lines(c);
   enqueue_sponge(c);
   enqueue_gcbc(c);
-  enqueue_rk(c, nk);
-  enqueue_wall(c);
-  enqueue_junction(c);
-  enqueue_avg(c);
+  enqueue_rk(c, nk, prim_next);
+  enqueue_wall(c, prim_next);
+  enqueue_junction(c, prim_next);
+  enqueue_avg(c, prim_next);
 }
 
+// a time step: the filter step, then the stages, each but the last leaving
+// the next one's primitives
 void enqueue_step(tp_context *c) {
   if (c->linked.empty()) return;
   enqueue_filter_step(c);
-  for (int nk = 1; nk <= c->nkrk; nk++) enqueue_stage(c, nk);
+  for (int nk = 1; nk <= c->nkrk; nk++) enqueue_stage(c, nk, nk > 1, nk < c->nkrk);
   launch_sc_end(c->d_sc, (int)c->linked.size(), nullptr, c->stream);
 }
 

@@ -256,7 +256,8 @@ struct StageArgs {
   double *sc;        // [ncase][S_STRIDE]
   double *part;      // reduction partials [nwork][2]
   double *wpart;     // wall-force partials [nwallwork][4]
-  int nk, m, d, opt; // stage number, field, direction, option (k_flux_all: write tv everywhere)
+  int nk, m, d, opt; // stage number, field, direction, option (k_flux_all: write tv everywhere;
+                     // k_wall, k_avg_apply: form the primitives of the points they change)
   int arr, col;      // generic field selector
   int o0, o1, o2;    // tv slots for gradient combination (-1..-3 = ss columns)
 };
@@ -281,15 +282,15 @@ void launch_sc_dt(const CaseCoef *coef, double *sc, const double *part, const in
                   int reduced, hipStream_t st);
 void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st);
 void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st);
-void launch_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk,
-                     hipStream_t st);
+void launch_junction(const Block *blocks, const CaseCoef *coef, const int *jct, const int *jvel, int njct, int maxk,
+                     int prim, hipStream_t st);
 void launch_surf_reduce(const CaseCoef *coef, double *sc, const double *wpart, const int *wwr, int ncase,
                         hipStream_t st);
 
 // host: record the message of a failure (tp_last_error) and return its code
 int set_error(int code, const std::string &msg);
 
-enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_FORCE = 5, PK_SPONGE = 6, PK_RK, PK_COPYF = 9, PK_INIT, PK_METRICS,
+enum PointKernel { PK_PRIM = 0, PK_DTRED, PK_FORCE = 5, PK_SPONGE = 6, PK_RK, PK_RK_PRIM, PK_COPYF = 9, PK_INIT, PK_METRICS,
                    PK_FLUXALL = 14, PK_STRESS = 15, PK_FLUX_TV = 16 };
 enum FaceKernel { FK_GCBC_PREP = 0, FK_COPY, FK_GCBC_APPLY, FK_WALL, FK_AVG_CAP, FK_AVG_APPLY, FK_NORMALS,
                   FK_SURF_SETUP, FK_SURF_SAMPLE, FK_GCBC_APPLY_XI };

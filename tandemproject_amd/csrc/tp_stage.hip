@@ -318,6 +318,16 @@ __device__ inline Prim primitives(const CaseCoef &c, const Phys &ph, double q1, 
   return r;
 }
 
+// the primitives of the point's conserved state q into de(:,1:5), p and the
+// viscosity ss(:,1) (plain stores: the face kernels' points)
+__device__ inline void put_primitives(const Block &b, const CaseCoef &c, const Phys &ph, int l) {
+  const int N = b.N;
+  const Prim r = primitives(c, ph, b.qa[l], b.qa[N + l], b.qa[2 * N + l], b.qa[3 * N + l], b.qa[4 * N + l]);
+  b.de[l] = r.d1; b.de[N + l] = r.d2; b.de[2 * N + l] = r.d3; b.de[3 * N + l] = r.d4; b.de[4 * N + l] = r.d5;
+  b.p[l] = r.p;
+  b.ss[l] = r.mu;
+}
+
 __global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
@@ -911,13 +921,33 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
 }
 
 // ---- RK update, Main3D.f90:876-881 ----
+// PRIM: also the next stage's primitives of the updated state (k_primitives'
+// arithmetic on the same values), into de, p and ss once the residual is
+// read; the wall, junction and averaging passes that follow form them again
+// at the points they change
+template <bool PRIM>
 __global__ void __launch_bounds__(256) k_rk(StageArgs a) {
   WORK_POINTS
   const double dtk = a.sc[b.cs * S_STRIDE + S_DTK];
+  const CaseCoef &c = a.coef[b.cs];
+  const Phys ph = phys();
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
     double r1 = dtk * ld_stream(&b.yaco[l]);
-    for (int v = 0; v < 5; v++)
-      st_stream(&b.qa[(size_t)v * N + l], ld_stream(&b.qo[(size_t)v * N + l]) - r1 * ld_stream(&b.de[(size_t)v * N + l]));
+    double q[5];
+    for (int v = 0; v < 5; v++) {
+      q[v] = ld_stream(&b.qo[(size_t)v * N + l]) - r1 * ld_stream(&b.de[(size_t)v * N + l]);
+      st_stream(&b.qa[(size_t)v * N + l], q[v]);
+    }
+    if (PRIM) {
+      const Prim r = primitives(c, ph, q[0], q[1], q[2], q[3], q[4]);
+      st_stream(&b.de[l], r.d1);
+      st_stream(&b.de[N + l], r.d2);
+      st_stream(&b.de[2 * N + l], r.d3);
+      st_stream(&b.de[3 * N + l], r.d4);
+      st_stream(&b.de[4 * N + l], r.d5);
+      st_stream(&b.p[l], r.p);
+      st_stream(&b.ss[l], r.mu);
+    }
   }
 }
 
@@ -944,11 +974,15 @@ __global__ void __launch_bounds__(256) k_wall(StageArgs a) {
   for (int v = 0; v < 3; v++) qa[(1 + v) * N + l] = ra3 * qa[(1 + v) * N + l] - fctr * umf[v];
   double q1 = qa[l], q2 = qa[N + l], q3 = qa[2 * N + l], q4 = qa[3 * N + l];
   qa[4 * N + l] = rp * qa[4 * N + l] + ra2 * (ra0 * q1 + 0.5 * (q2 * q2 + q3 * q3 + q4 * q4) / q1);
+  if (a.opt) put_primitives(b, c, ph, l);   // the next stage's primitives, formed in the update
 }
 
 // ---- junction averaging, SingleTurbulence3D.f90:378-443 ----
 // jct[j*12 + mbr*3 + {block, ip, jp}], jvel[j*2 + {0,1}]; one thread per (junction, k)
-__global__ void k_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk) {
+// prim: also the primitives of the new corner states (the next stage's,
+// formed in the update)
+__global__ void k_junction(const Block *blocks, const CaseCoef *coef, const int *jct, const int *jvel, int njct,
+                           int maxk, int prim) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int jn = t / maxk, k = t % maxk;
   if (jn >= njct) return;
@@ -970,6 +1004,7 @@ __global__ void k_junction(const Block *blocks, const int *jct, const int *jvel,
   for (int q = 0; q < 4; q++) {
     const Block &b = blocks[J[q * 3]];
     for (int m = 0; m < 5; m++) b.qa[(size_t)m * b.N + lq[q]] = r[m];
+    if (prim) put_primitives(b, coef[b.cs], phys(), lq[q]);
   }
 }
 
@@ -1044,6 +1079,7 @@ __global__ void __launch_bounds__(256) k_avg_apply(StageArgs a) {
       qa[(size_t)v * N + l] = rp * r2 * ((r1 - 1.0) * qa[(size_t)v * N + l] + drvb[(ip * 5 + v) * F + jk]) +
                               ra1 * qa[(size_t)v * N + l];
   }
+  if (a.opt) put_primitives(b, c, phys(), l);   // the next stage's primitives, formed in the update
 }
 
 // ---- copies ----
@@ -1233,7 +1269,8 @@ void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a) {
     case 1: LAUNCH(k_dt_reduce, nwg, 256, st, a); break;
     case 5: LAUNCH(k_force, nwg, 256, st, a); break;
     case 6: LAUNCH(k_sponge, nwg, 256, st, a); break;
-    case 7: LAUNCH(k_rk, nwg, 256, st, a); break;
+    case 7: LAUNCH(k_rk<false>, nwg, 256, st, a); break;
+    case 8: LAUNCH(k_rk<true>, nwg, 256, st, a); break;
     case 9: LAUNCH(k_copy_field, nwg, 256, st, a); break;
     case 10: LAUNCH(k_initialo, nwg, 256, st, a); break;
     case 11: LAUNCH(k_metrics, nwg, 256, st, a); break;
@@ -1273,9 +1310,10 @@ void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream
 void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st) {
   hipLaunchKernelGGL(k_sc_end, dim3((ncase + 63) / 64), dim3(64), 0, st, sc, ncase, active);
 }
-void launch_junction(const Block *blocks, const int *jct, const int *jvel, int njct, int maxk, hipStream_t st) {
+void launch_junction(const Block *blocks, const CaseCoef *coef, const int *jct, const int *jvel, int njct, int maxk,
+                     int prim, hipStream_t st) {
   int tot = njct * maxk;
-  hipLaunchKernelGGL(k_junction, dim3((tot + 63) / 64), dim3(64), 0, st, blocks, jct, jvel, njct, maxk);
+  hipLaunchKernelGGL(k_junction, dim3((tot + 63) / 64), dim3(64), 0, st, blocks, coef, jct, jvel, njct, maxk, prim);
 }
 
 }  // namespace tp
