@@ -547,7 +547,10 @@ __global__ void __launch_bounds__(256) k_sponge(StageArgs a) {
   const int ng = b.ngust, nt1 = c.ntz + 1;
   for (int q = w.a + threadIdx.x; q < w.a + w.n; q += blockDim.x) {
     const int l = b.lcsz[q];
-    double rt = 1.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    // rest-state target: rho = 1, no momentum, gamma p = rho^gamma = 1
+    // (pow(1, gamma) is 1 exactly, so pt = ham * 1 is the reference's value
+    // without the pow, which the sponge points' lanes would all evaluate)
+    double rt = 1.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, pt = ph.ham;
     const int ll = c.inflow ? b.gix[q] : -1;
     if (ll >= 0) {
       // target of an inflow point: isentropic density and momentum of the
@@ -566,13 +569,14 @@ __global__ void __launch_bounds__(256) k_sponge(StageArgs a) {
       rt = pow(1.0 - fctr * (v0 * v0 + v1 * v1 + v2 * v2), ph.hamm1);
       const double at = b.atz[ll] * rt;
       m0 = at * v0; m1 = at * v1; m2 = at * v2;
+      pt = ph.ham * pow(rt, GAM);
     }
     const double as = b.asz[q];
     b.de[l] = b.de[l] + as * (b.qa[l] - rt);
     b.de[N + l] = b.de[N + l] + as * (b.qa[N + l] - m0);
     b.de[2 * N + l] = b.de[2 * N + l] + as * (b.qa[2 * N + l] - m1);
     b.de[3 * N + l] = b.de[3 * N + l] + as * (b.qa[3 * N + l] - m2);
-    b.de[4 * N + l] = b.de[4 * N + l] + b.bsz[q] * (b.p[l] - ph.ham * pow(rt, GAM));
+    b.de[4 * N + l] = b.de[4 * N + l] + b.bsz[q] * (b.p[l] - pt);
   }
 }
 
