@@ -38,13 +38,17 @@
  *     VISCOUS + FLUX    src/Main3D.f90:626-768                     analytic divergence of the Euler and
  *                                                                  Navier-Stokes fluxes of linear fields
  *                                                                  (moving frame too), quadratic shear,
- *                                                                  heat flux of a linear temperature
+ *                                                                  heat flux of a linear temperature; the
+ *                                                                  LES subgrid stress by the model's form
+ *                                                                  (its constants through the GPU parity)
  *     SPONGE            src/SingleTurbulence3D.f90:332-374         isentropic target with total enthalpy
  *                                                                  1/(gamma-1); relaxation linear in the
  *                                                                  departure; forcing (rpt.f90:758-769) in
  *                                                                  its window, closed-form coefficients
  *     GCBC              src/Main3D.f90:778-869                     incoming amplitudes zero after a far-field
- *                                                                  face; R+ = R- after a reflecting wall
+ *                                                                  face; R+ = R- after a reflecting wall; the
+ *                                                                  two copies of an interface point carry the
+ *                                                                  same amplitudes after a whole stage
  *     RK                src/Main3D.f90:873-881                     4th-order Taylor polynomial on y' = ly
  *     WALL              src/Main3D.f90:884-895                     no-slip, gamma p / rho = T_wall, slots kept
  *     JUNCTION          src/SingleTurbulence3D.f90:378-443         copies agree: mean rho, E of four, momentum
@@ -74,13 +78,11 @@
  *                                                                  after fluxes), state = qo - dtk yaco de (update
  *                                                                  after the residual), wall conditions and
  *                                                                  continuous interfaces (wall, averaging after
- *                                                                  the update); mutations caught
- *
- * Parity unpinned (nothing outside this file fixes it): in orc_stage, the
- * sponge before the characteristic section and the junction before the
- * interface averaging (the reference's statement order).  The first pair
- * commutes except where two copies of an interface point differ; the second
- * differs only at the corner points the two share.
+ *                                                                  the update), junction corners with the wall
+ *                                                                  pair's momentum (wall, junction, averaging
+ *                                                                  in that order), interface amplitudes with
+ *                                                                  the interface in the sponge (sponge before
+ *                                                                  the characteristic section); mutations caught
  * The reference program itself cannot be built here (it needs an MPI
  * Fortran module).
  *

@@ -57,6 +57,25 @@ MUTATIONS = {
     "stage: update before the characteristic faces": ("ORC_P_GCBC,\n    ORC_P_RK, ORC_P_WALL,", "ORC_P_RK,\n    ORC_P_GCBC, ORC_P_WALL,"),
     "stage: wall before the update": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_WALL, ORC_P_RK, ORC_P_JUNCTION, ORC_P_AVG};"),
     "stage: averaging before the update": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_AVG, ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION};"),
+    "stage: sponge after the characteristic faces": ("ORC_P_SPONGE, ORC_P_GCBC,\n    ORC_P_RK,", "ORC_P_GCBC, ORC_P_SPONGE,\n    ORC_P_RK,"),
+    "interface: the entering acoustic wave not taken": (
+        "              if (ra0 * (vv - e.ao) > zero) cha[4] = dha[4];\n            }\n            xtr2q(&e, cmv, xt);",
+        "            }\n            xtr2q(&e, cmv, xt);"),
+    "interface: the entering entropy and vorticity waves not taken": (
+        "            } else {\n              if (ra0 * vv > zero) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }",
+        "            } else {\n              if (ra0 * vv > zero) { cha[0] = dha[0]; }"),
+    # the Smagorinsky subgrid stress: pinned by tests/test_oracle_stage_pins.py::test_les_subgrid_stress_form
+    "LES: a strain-rate term dropped": ("(hyy + tzx) * (hyy + tzx) + (hxx + tyz) * (hxx + tyz);", "(hyy + tzx) * (hyy + tzx);"),
+    "LES: eddy viscosity linear in smago1": ("double r3 = qa[l] * (c->smago1 * c->smago1) * varr * sqrt(2 * d1);",
+                                             "double r3 = qa[l] * c->smago1 * varr * sqrt(2 * d1);"),
+    "LES: eddy viscosity without the density": ("double r3 = qa[l] * (c->smago1 * c->smago1) * varr * sqrt(2 * d1);",
+                                                "double r3 = (c->smago1 * c->smago1) * varr * sqrt(2 * d1);"),
+    "LES: strain-rate norm not rooted": ("double r3 = qa[l] * (c->smago1 * c->smago1) * varr * sqrt(2 * d1);",
+                                         "double r3 = qa[l] * (c->smago1 * c->smago1) * varr * (2 * d1);"),
+    "LES: isotropic part added": ("double ntxx = r1 * (2 * txx - d5) - r3, ntyy = r1 * (2 * tyy - d5) - r3,",
+                                  "double ntxx = r1 * (2 * txx - d5) + r3, ntyy = r1 * (2 * tyy - d5) + r3,"),
+    "stage: averaging before the junction": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_RK, ORC_P_WALL, ORC_P_AVG, ORC_P_JUNCTION};"),
+    "stage: junction before the wall": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_RK, ORC_P_JUNCTION, ORC_P_WALL, ORC_P_AVG};"),
 }
 # the pins a mutation is run against (default: the stage pins)
 PINS = {name: "test_rounding_cpu.py" for name in MUTATIONS if name.startswith(("x**1.5", "sin/cos"))}

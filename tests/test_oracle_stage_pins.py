@@ -26,7 +26,15 @@ closed form that does not reuse the oracle's own formulas:
   point shared by two blocks takes the mean of the two;
 * sponge target (src/SingleTurbulence3D.f90:360-372): isentropic from the
   rest state with constant total enthalpy 1/(gamma-1); a state equal to the
-  target is left alone, and the relaxation is linear in the departure.
+  target is left alone, and the relaxation is linear in the departure;
+* sponge mask (:209-249) from the zone geometry, and the wall-slot flags
+  (:447-464) from the slot positions;
+* the subgrid stress of the LES model (src/Main3D.f90:656-680) by the form
+  of the model (an eddy viscosity added to the molecular one, its scalings,
+  the axes alike, the isotropic part);
+* the interface conditions (code 30): the two copies of an interface point
+  carry the same characteristic amplitudes after a whole stage;
+* the order of a whole stage (orc_stage) by what holds after it.
 """
 import numpy as np
 import pytest
@@ -772,7 +780,12 @@ def test_whole_stage_boundary_sections_last():
     """The order of a whole stage after the update: the wall conditions
     (no-slip, gamma p / rho = T_wall) and continuous interfaces hold after
     one whole stage, so the wall and averaging sections follow the update
-    (src/Main3D.f90:873-952)."""
+    (src/Main3D.f90:873-952).  At the junction corners away from the span's
+    periodic ends the four copies carry the mean density of the four updated
+    copies and the mean momentum of the two wall copies after the wall
+    section, -umf rho (src/SingleTurbulence3D.f90:378-443): the junction
+    follows the wall section and precedes the averaging, which then finds
+    equal copies there."""
     lay, ss, p = aerofoil_case(wtemp=1.07)
     o = po.Oracle(lay, p)
     for b in range(lay.nb):
@@ -826,3 +839,152 @@ def test_whole_stage_boundary_sections_last():
                                           o.array(nb, "qa").reshape(5, -1)[:, lb[inner]])
             pairs += 1
     assert pairs >= 8
+    corners = 0
+    for jn in range(lay.njct):
+        mem = lay.jct[jn]
+        va, vb = lay.jvel[jn]
+        for k in range(1, dims[mem[0][0]][2] - 1):
+            vals, rho = [], []
+            for (b, ip, jp) in mem:
+                nx, ny, _ = dims[b]
+                l = (k * ny + jp * (ny - 1)) * nx + ip * (nx - 1)
+                vals.append(o.array(b, "qa").reshape(5, -1)[:, l])
+                # the updated density: qo - dtk yaco de with the stage's residual
+                rho.append(o.array(b, "qo").reshape(5, -1)[0, l]
+                           - sc["dtk"] * o.array(b, "yaco")[l] * o.array(b, "de").reshape(5, -1)[0, l])
+            for x in vals[1:]:
+                np.testing.assert_allclose(x, vals[0], rtol=1e-14, atol=1e-16)
+            np.testing.assert_allclose(vals[0][0], np.mean(rho), rtol=1e-14)
+            np.testing.assert_allclose(vals[0][1:4], -umf * 0.5 * (rho[va] + rho[vb]), rtol=1e-14, atol=1e-15)
+            corners += 1
+    assert corners >= 20
+
+
+@pytest.mark.parametrize("sponge", [False, True], ids=["no-sponge", "interface-in-sponge"])
+def test_gcic_interface_amplitudes(sponge):
+    """Characteristic interface conditions (code 30, src/Main3D.f90:778-869):
+    each side of an interface takes the amplitudes of the waves entering it
+    from the other side's residual, so after one whole stage the five
+    characteristic amplitudes X_L yaco de of the two copies of an interface
+    point agree (each copy in the basis of its own stage-start state) at the
+    points no other characteristic face's band reaches.  The two copies start
+    from different states.  With the interface inside the sponge zone the
+    sponge term is part of the residual the sides exchange, so this also
+    fixes the sponge before the characteristic section."""
+    over = dict(nviscous=0)
+    if sponge:
+        over.update(szco=5.0, domlen=0.3, szth1=1.0, szth2=1.0, szxt=1.0)
+    else:
+        over.update(szco=0.0)
+    lay, ss, p = aerofoil_case(**over)
+    o = po.Oracle(lay, p)
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b])
+    o.metrics()
+    rng = np.random.default_rng(11)
+    q0 = []
+    for b in range(lay.nb):
+        x, y, z = ss[b].reshape(3, -1)
+        rho, u, v, w, pr = smooth_state(x, y, z)
+        q = conserved(rho, u, v, w, pr).reshape(5, -1)
+        q = q * (1 + 0.01 * rng.standard_normal(q.shape))   # the copies of a shared point differ
+        o.array(b, "qa")[:] = q.ravel()
+        o.array(b, "qo")[:] = q.ravel()
+        q0.append(q.copy())
+    for k, val in (("n", 20), ("ndt", 10), ("dts", 1e-3), ("dte", 1e-3)):
+        o.set_scalar(k, val)
+    o.stage(1)
+    sc = o.scalars()
+    assert sc["umf1"] == sc["umf2"] == sc["umf3"] == 0
+    dims = lay.dims + 1
+    A, B = 0, 1                         # xi end of block 0 against the xi start of block 1
+    assert lay.nbc[A][0][1] == 30 and lay.ncd[A][0][1] == B and lay.nbc[B][0][0] == 30
+    nxa, ny, nz = dims[A]
+    la, jka, ia = face_points(*dims[A], 0, 1)
+    lb, jkb, ib = face_points(*dims[B], 0, 0)
+    assert np.array_equal(jka, jkb)
+    F = ny * nz
+    cma, cmb = o.array(A, "cm1").reshape(2, 3, F), o.array(B, "cm1").reshape(2, 3, F)
+    dea, deb = o.array(A, "de").reshape(5, -1), o.array(B, "de").reshape(5, -1)
+    ya, yb = o.array(A, "yaco"), o.array(B, "yaco")
+    spa = set(o.sponge_list(A)[0].tolist())
+    checked = in_sponge = 0
+    for t in range(la.size):
+        j = ia[1][t]
+        if j <= 3 or j >= ny - 4:
+            continue   # in an eta face's band too
+        l1, l2, f = la[t], lb[t], jka[t]
+        qa_, qb_ = q0[A][:, l1], q0[B][:, l2]
+        assert np.abs(qa_ - qb_).max() > 1e-4          # the copies differ
+        pa = (GAM - 1) * (qa_[4] - 0.5 * (qa_[1:4] @ qa_[1:4]) / qa_[0])
+        pb = (GAM - 1) * (qb_[4] - 0.5 * (qb_[1:4] @ qb_[1:4]) / qb_[0])
+        wa = po.xtq2r(qa_, pa, cma[1, :, f], np.zeros(3)) @ (ya[l1] * dea[:, l1])
+        wb = po.xtq2r(qb_, pb, cmb[0, :, f], np.zeros(3)) @ (yb[l2] * deb[:, l2])
+        np.testing.assert_allclose(wa, wb, rtol=0, atol=1e-12 * np.abs(wb).max())
+        checked += 1
+        in_sponge += int(l1 in spa)
+    assert checked > 100
+    assert (in_sponge == checked) if sponge else (in_sponge == 0)
+
+
+def les_stresses(G, rho0=1.0, smago1=0.0, smago2=0.0, les=1):
+    """tv (the 9 stress and energy-flux terms) after the viscous section for
+    the velocity u = G r, density rho0 and pressure 1/gamma (constant), on a
+    left-handed affine grid (z flipped, so that the subgrid length scale
+    (-1/yaco)^1.5 of src/MainVar3D.f90:105 is real)."""
+    nx, ny, nz = 18, 16, 13
+    # unit cells, so that the eddy viscosity is comparable to the molecular one
+    lay, X, Y, Z, M = affine_block(nx, ny, nz, [[10, 10], [10, 10], [45, 45]], hx=1.0, hy=0.8, hz=-0.9)
+    o, p = make(lay, (X, Y, Z), nviscous=1, reoo=100.0, les=les, smago1=smago1, smago2=smago2)
+    r = np.stack([X, Y, 0 * X])
+    u = np.asarray(G) @ r
+    rho = np.full_like(X, rho0)
+    set_state(o, conserved(rho, u[0], u[1], u[2], np.full_like(X, 1 / GAM)))
+    for part in ("begin", "viscous"):
+        o.stage_part(1, part)
+    return o.array(0, "tv").reshape(9, -1)[:, interior(nx, ny, nz)].copy()
+
+
+def test_les_subgrid_stress_form():
+    """The Smagorinsky subgrid stress of src/Main3D.f90:656-680 (les = 1), by
+    the form of the model rather than its formula: with smago2 = 0 the eddy
+    viscosity adds to the molecular one, so every stress component is the
+    laminar one times the same factor 1 + mu_t / mu; mu_t is proportional to
+    rho and to smago1^2 and to the size of the velocity gradient (the subgrid
+    stress is homogeneous of degree 2 in it), and the same for a shear of u
+    along y as for the shear of w along y (the model treats the axes alike);
+    smago2 adds an isotropic stress, the same negative amount on the three
+    normal components, proportional to rho, smago2 and the square of the
+    gradient.  (The model's constants, smago1^2 (-1/yaco)^1.5 sqrt(2 d1),
+    are the deck's calibration; they are checked through the GPU's parity
+    with the oracle only.)"""
+    G = np.array([[0.3, -0.2, 0.0], [0.15, -0.1, 0.0], [0.05, 0.25, 0.0]])
+    lam = les_stresses(G, les=0)
+    sgs = les_stresses(G, smago1=0.16) - lam
+    nz_ = np.abs(lam[:6]).max(axis=1) > 1e-9
+    ratio = sgs[:6][nz_] / lam[:6][nz_]
+    assert ratio.min() > 1e-3
+    np.testing.assert_allclose(ratio, ratio.flat[0], rtol=1e-9)
+    # degree 2 in the gradient, proportional to rho and smago1^2
+    sgs2 = les_stresses(2 * G, smago1=0.16) - les_stresses(2 * G, les=0)
+    np.testing.assert_allclose(sgs2[:6], 4 * sgs[:6], rtol=1e-9, atol=1e-14)
+    sgs_r = les_stresses(G, rho0=2.0, smago1=0.16) - les_stresses(G, rho0=2.0, les=0)
+    np.testing.assert_allclose(sgs_r[:6], 2 * sgs[:6], rtol=1e-9, atol=1e-14)
+    sgs_c = les_stresses(G, smago1=0.32) - lam
+    np.testing.assert_allclose(sgs_c[:6], 4 * sgs[:6], rtol=1e-9, atol=1e-14)
+    # the axes alike: u = a y against w = a y
+    a = 0.4
+    Gu = np.zeros((3, 3)); Gu[0, 1] = a
+    Gw = np.zeros((3, 3)); Gw[2, 1] = a
+    fu = (les_stresses(Gu, smago1=0.16) - les_stresses(Gu, les=0))[3] / les_stresses(Gu, les=0)[3]
+    fw = (les_stresses(Gw, smago1=0.16) - les_stresses(Gw, les=0))[4] / les_stresses(Gw, les=0)[4]
+    np.testing.assert_allclose(fu, fw, rtol=1e-9)
+    assert fu.min() > 1e-3
+    # the isotropic part
+    iso = les_stresses(G, smago2=0.0066) - lam
+    assert np.abs(iso[3:6]).max() < 1e-14
+    np.testing.assert_allclose(iso[1], iso[0], rtol=1e-12)
+    np.testing.assert_allclose(iso[2], iso[0], rtol=1e-12)
+    assert iso[0].max() < 0
+    iso2 = les_stresses(2 * G, rho0=2.0, smago2=2 * 0.0066) - les_stresses(2 * G, rho0=2.0, les=0)
+    np.testing.assert_allclose(iso2[:3], 16 * iso[:3], rtol=1e-9)
