@@ -10,6 +10,8 @@
 * A context emptied by removing its only case: stepping it (eager, captured,
   a single stage, the filter sweep) does nothing and reports no error, and a
   case admitted into it afterwards computes what it computes alone.
+* A request loop with nothing submitted returns at once and counts nothing;
+  a request submitted afterwards is served as alone.
 """
 import sys
 
@@ -112,3 +114,26 @@ def test_emptied_context_keeps_working():
     for k in ("n", "dt", "timo"):
         assert sc[k] == rsc[k], k
     s.close()
+
+
+def test_empty_request_loop():
+    """A request loop with nothing submitted: run and iterate return at once,
+    nothing is counted; a request submitted afterwards is served and equals
+    its run alone."""
+    say("request loop with no requests, then one")
+    srv = api.Server(budget_bytes=0, max_active=4, graph=True)
+    srv.run(5)
+    st = srv.iterate(5)
+    assert st["active"] == 0 and st["queued"] == 0 and st["done"] == 0 and st["steps"] == 0
+    s0 = srv.stats()
+    assert s0["finished"] == 0 and s0["failed"] == 0
+    lay, ss, p = aerofoil_case(lxibk=(16, 20, 16), letbk=(14, 14))
+    req = srv.submit(lay, p, ss, 3)
+    srv.run(5)
+    info, qa = srv.result(req)
+    assert info["state"] == "done" and info["steps"] == 3
+    ref, _ = run_alone(lay, ss, p, 3)
+    for q, r in zip(qa, ref):
+        np.testing.assert_array_equal(q, r)
+    assert srv.stats()["finished"] == 1
+    srv.close()
