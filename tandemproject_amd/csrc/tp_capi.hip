@@ -1024,7 +1024,7 @@ namespace {
 
 // sub-array offsets of one block's arena (256-byte aligned)
 struct Arena {
-  size_t qa, qo, de, rr, ss, xi, et, ze, p, ya, tv, fx, f[3][4], lud, luf, rp, sa, sn, ss2, va, total;
+  size_t qa, qo, de, rr, ss, xi, et, ze, p, ya, tv, fx, f[3][4], gq, lud, luf, rp, sa, sn, ss2, va, total;
 };
 
 // les: the case needs the single-precision Smagorinsky volume term per point
@@ -1040,6 +1040,7 @@ Arena arena_layout(const Block &b, int nwall, bool les) {
     A.f[d][0] = take(10 * (size_t)b.F[d]); A.f[d][1] = take(10 * (size_t)b.F[d]);
     A.f[d][2] = take(6 * RCV_SLOTS * (size_t)b.F[d]); A.f[d][3] = take(6 * (size_t)b.F[d]);
   }
+  A.gq = take(14 * (size_t)b.F[0]);
   A.lud = take(5 * (size_t)b.nlu); A.luf = take(5 * (size_t)b.nlu); A.rp = take(std::max(nwall, 1));
   A.sa = take(b.nsurf); A.sn = take(3 * (size_t)b.nsurf); A.ss2 = take(3 * (size_t)b.nsurf);
   A.va = les ? take((N + 1) / 2) : 0;   // N floats
@@ -1237,6 +1238,7 @@ int build_case(HostCase &h) {
     for (int d = 0; d < 3; d++) {
       b.drva[d] = P(A.f[d][0]); b.drvb[d] = P(A.f[d][1]); b.recv[d] = P(A.f[d][2]); b.cm[d] = P(A.f[d][3]);
     }
+    b.gq = P(A.gq);
     b.lud = P(A.lud); b.luf = P(A.luf); b.rpex = P(A.rp);
     b.sarea = P(A.sa); b.snor = P(A.sn); b.sst = P(A.ss2);
     b.varr = les ? reinterpret_cast<float *>(base + A.va) : nullptr;

@@ -77,6 +77,9 @@ struct Block {
                               // reference's real(4) work array, src/Main3D.f90:664), formed
                               // once from the metrics on the host; null otherwise
   double *drva[3], *drvb[3], *cm[3];
+  double *gq;                 // [2][7][F0]: the stage-start q (5), p and yaco at each xi face
+                              // point, gathered by the characteristic preparation for its
+                              // application (contiguous reads instead of one row each)
   double *recv[3];            // [RCV_SLOTS][6][F]: closure sums per field of a merged launch
   double *lud, *luf;          // [5][nlu]: l1, l2, u1, u2, u3 for deriv / filter
   double *rpex;

@@ -795,6 +795,12 @@ __global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
   for (int c = 0; c < 5; c++) v[c] = ya * cha[c];
   xtq2r_apply(e, cm, v, o);
   for (int c = 0; c < 5; c++) drva[(ip * 5 + c) * F + jk] = o[c];
+  if (d == 0) {
+    // the xi faces' state for k_gcbc_apply<true>, one face array each
+    for (int c = 0; c < 5; c++) b.gq[(ip * 7 + c) * F + jk] = q[c];
+    b.gq[(ip * 7 + 5) * F + jk] = b.p[l];
+    b.gq[(ip * 7 + 6) * F + jk] = ya;
+  }
 }
 
 // ---- face copy from the neighbour: drvb(:,:,ip) <- drva(:,:,1-ip) of ncd
@@ -850,11 +856,20 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
     face_jk(b, d, jk, j, k);
     const double ra0 = iq;
     int l = indx3(b, i, j, k, d);
-    double q[5], cm[3], cha[5], dha[5], dv[5], dq[5];
-    for (int v = 0; v < 5; v++) q[v] = b.qa[(size_t)v * N + l];
+    double q[5], cm[3], cha[5], dha[5], dv[5], dq[5], pl, yl;
+    if (XI) {
+      // gathered by k_gcbc_prep: the values at l, from one face array each
+      for (int v = 0; v < 5; v++) q[v] = b.gq[(ip * 7 + v) * F + jk];
+      pl = b.gq[(ip * 7 + 5) * F + jk];
+      yl = b.gq[(ip * 7 + 6) * F + jk];
+    } else {
+      for (int v = 0; v < 5; v++) q[v] = b.qa[(size_t)v * N + l];
+      pl = b.p[l];
+      yl = b.yaco[l];
+    }
     for (int v = 0; v < 3; v++) cm[v] = b.cm[d][(ip * 3 + v) * F + jk];
     Elem e;
-    eleme(q, b.p[l], cm, sc + S_UMF1, e);
+    eleme(q, pl, cm, sc + S_UMF1, e);
     const double *drva = b.drva[d], *drvb = b.drvb[d];
     for (int v = 0; v < 5; v++) { cha[v] = drva[(ip * 5 + v) * F + jk]; dha[v] = drvb[(ip * 5 + v) * F + jk]; }
     double vv = e.vn + e.vs;
@@ -879,7 +894,7 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
     }
     for (int v = 0; v < 5; v++) dv[v] = cha[v] - drva[(ip * 5 + v) * F + jk];
     xtr2q_apply(e, cm, dv, dq);
-    const double res = 1.0 / b.yaco[l];
+    const double res = 1.0 / yl;
     if (XI) {
       for (int v = 0; v < 5; v++) sdq[threadIdx.x][v] = dq[v];
       sdq[threadIdx.x][5] = res;
