@@ -48,7 +48,9 @@
  *     GCBC              src/Main3D.f90:778-869                     incoming amplitudes zero after a far-field
  *                                                                  face; R+ = R- after a reflecting wall; the
  *                                                                  two copies of an interface point carry the
- *                                                                  same amplitudes after a whole stage
+ *                                                                  same amplitudes after a whole stage, and so
+ *                                                                  do partner wall slots (extrabcs) for the
+ *                                                                  waves entering one and leaving the other
  *     RK                src/Main3D.f90:873-881                     4th-order Taylor polynomial on y' = ly
  *     WALL              src/Main3D.f90:884-895                     no-slip, gamma p / rho = T_wall, slots kept
  *     JUNCTION          src/SingleTurbulence3D.f90:378-443         copies agree: mean rho, E of four, momentum

@@ -74,6 +74,10 @@ MUTATIONS = {
                                          "double r3 = qa[l] * (c->smago1 * c->smago1) * varr * (2 * d1);"),
     "LES: isotropic part added": ("double ntxx = r1 * (2 * txx - d5) - r3, ntyy = r1 * (2 * tyy - d5) - r3,",
                                   "double ntxx = r1 * (2 * txx - d5) + r3, ntyy = r1 * (2 * tyy - d5) + r3,"),
+    "wall slots: the entering acoustic waves not taken": (
+        "/* extrabcs, src/SingleTurbulence3D.f90:468-474 */\n                if (ra0 * vv > zero) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }\n                if (ra0 * (vv + e.ao) > zero) cha[3] = dha[3];\n                if (ra0 * (vv - e.ao) > zero) cha[4] = dha[4];",
+        "/* extrabcs, src/SingleTurbulence3D.f90:468-474 */\n                if (ra0 * vv > zero) { cha[0] = dha[0]; cha[1] = dha[1]; cha[2] = dha[2]; }"),
+    "wall slots: reflected like the wall": ("if (B->rpex[lp] == zero) {", "if (B->rpex[lp] == zero || 1) {"),
     "stage: averaging before the junction": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_RK, ORC_P_WALL, ORC_P_AVG, ORC_P_JUNCTION};"),
     "stage: junction before the wall": ("ORC_P_RK, ORC_P_WALL, ORC_P_JUNCTION, ORC_P_AVG};", "ORC_P_RK, ORC_P_JUNCTION, ORC_P_WALL, ORC_P_AVG};"),
 }

@@ -33,7 +33,8 @@ closed form that does not reuse the oracle's own formulas:
   of the model (an eddy viscosity added to the molecular one, its scalings,
   the axes alike, the isotropic part);
 * the interface conditions (code 30): the two copies of an interface point
-  carry the same characteristic amplitudes after a whole stage;
+  carry the same characteristic amplitudes after a whole stage, and so do
+  partner wall slots for the waves entering one and leaving the other;
 * the order of a whole stage (orc_stage) by what holds after it.
 """
 import numpy as np
@@ -988,3 +989,72 @@ def test_les_subgrid_stress_form():
     assert iso[0].max() < 0
     iso2 = les_stresses(2 * G, rho0=2.0, smago2=2 * 0.0066) - les_stresses(2 * G, rho0=2.0, les=0)
     np.testing.assert_allclose(iso2[:3], 16 * iso[:3], rtol=1e-9)
+
+
+def test_wall_slot_characteristic_exchange():
+    """Wall slots with the exchange through the aerofoil (nextrabc =
+    nextgcic = 1, extrabcs, src/SingleTurbulence3D.f90:466-474): a slot point
+    takes the amplitudes of the waves entering it from its partner slot point
+    on the other surface, so after one whole stage every amplitude that
+    enters one side and leaves the other agrees between the two (each in the
+    basis of its own stage-start state)."""
+    lay, ss, p = aerofoil_case(nextrabc=1, nextgcic=1, wlea=0.1, wlew=0.05, nviscous=0)
+    ss = [np.concatenate([x[:x.size // 3] - 0.5, x[x.size // 3:]]) for x in ss]   # leading edge at -1/2
+    o = po.Oracle(lay, p)
+    for b in range(lay.nb):
+        o.set_grid(b, ss[b])
+    o.metrics()
+    rng = np.random.default_rng(13)
+    q0 = []
+    for b in range(lay.nb):
+        x, y, z = ss[b].reshape(3, -1)
+        rho, u, v, w, pr = smooth_state(x, y, z)
+        q = conserved(rho, u, v, w, pr).reshape(5, -1)
+        q = q * (1 + 0.01 * rng.standard_normal(q.shape))
+        o.array(b, "qa")[:] = q.ravel()
+        o.array(b, "qo")[:] = q.ravel()
+        q0.append(q.copy())
+    for k, val in (("n", 20), ("ndt", 10), ("dts", 1e-3), ("dte", 1e-3)):
+        o.set_scalar(k, val)
+    o.stage(1)
+    sc = o.scalars()
+    assert sc["umf1"] == sc["umf2"] == sc["umf3"] == 0
+    dims = lay.dims + 1
+    A = [b for b in range(lay.nb) if lay.nbc[b][1][1] in (20, 25)][0]     # wall at its eta end
+    B = lay.ncd[A][1][1]                                                  # the block across the aerofoil
+    assert lay.nbc[B][1][0] in (20, 25) and lay.ncd[B][1][0] == A
+    nx, ny, nz = dims[A]
+    la, jka, ia = face_points(*dims[A], 1, 1)
+    lb, jkb, ib = face_points(*dims[B], 1, 0)
+    assert np.array_equal(jka, jkb)
+    F = nz * nx
+    rpa, rpb = o.array(A, "rpex"), o.array(B, "rpex")
+    cma, cmb = o.array(A, "cm2").reshape(2, 3, F), o.array(B, "cm2").reshape(2, 3, F)
+    dea, deb = o.array(A, "de").reshape(5, -1), o.array(B, "de").reshape(5, -1)
+    ya, yb = o.array(A, "yaco"), o.array(B, "yaco")
+
+    def side(q, cm, ra0):
+        pr = (GAM - 1) * (q[4] - 0.5 * (q[1:4] @ q[1:4]) / q[0])
+        vv = cm @ (q[1:4] / q[0])
+        ao = np.sqrt(GAM * pr / q[0])
+        inc = np.array([ra0 * vv > 0] * 3 + [ra0 * (vv + ao) > 0, ra0 * (vv - ao) > 0])
+        return pr, inc
+
+    checked = 0
+    for t in range(la.size):
+        i = ia[0][t]
+        if i <= 3 or i >= nx - 4:
+            continue   # in a xi face's band too
+        if rpa[t] == 0:
+            continue   # not a slot point
+        assert rpb[t] == 1
+        l1, l2, f = la[t], lb[t], jka[t]
+        pa, inca = side(q0[A][:, l1], cma[1, :, f], -1.0)
+        pb, incb = side(q0[B][:, l2], cmb[0, :, f], 1.0)
+        wa = po.xtq2r(q0[A][:, l1], pa, cma[1, :, f], np.zeros(3)) @ (ya[l1] * dea[:, l1])
+        wb = po.xtq2r(q0[B][:, l2], pb, cmb[0, :, f], np.zeros(3)) @ (yb[l2] * deb[:, l2])
+        sel = inca ^ incb          # enters one side, leaves the other
+        scale = max(np.abs(wa).max(), np.abs(wb).max())
+        np.testing.assert_allclose(wa[sel], wb[sel], rtol=0, atol=1e-12 * scale)
+        checked += int(sel.sum())
+    assert checked > 50
