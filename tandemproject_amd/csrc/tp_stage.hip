@@ -1084,19 +1084,24 @@ __global__ void __launch_bounds__(256) k_avg_apply(StageArgs a) {
   // (lines have >= 7 points), so only earlier directions contribute
   double *qa = b.qa;
   const double *drvb = b.drvb[d];
+  // the point's state: on xi faces (the first direction) the value captured
+  // for this face, from one face array each instead of a row per point (no
+  // earlier direction has changed it); later directions read qa, which an
+  // earlier face of the point may have averaged
+  double q[5];
+  for (int v = 0; v < 5; v++) q[v] = d == 0 ? b.drva[0][(ip * 5 + v) * F + jk] : qa[(size_t)v * N + l];
   if (avg_code(np)) {
     double r1 = avg_count(b, c, x, y, z, d, 1.0);
     double r2 = 1.0 / r1;
     for (int v = 0; v < 5; v++)
-      qa[(size_t)v * N + l] = r2 * ((r1 - 1.0) * qa[(size_t)v * N + l] + drvb[(ip * 5 + v) * F + jk]);
+      qa[(size_t)v * N + l] = r2 * ((r1 - 1.0) * q[v] + drvb[(ip * 5 + v) * F + jk]);
   } else {
     double rp = b.rpex[b.wall_off[d][ip] + jk];
     double ra1 = 1.0 - rp;
     double r1 = avg_count(b, c, x, y, z, d, rp);
     double r2 = 1.0 / r1;
     for (int v = 0; v < 5; v++)
-      qa[(size_t)v * N + l] = rp * r2 * ((r1 - 1.0) * qa[(size_t)v * N + l] + drvb[(ip * 5 + v) * F + jk]) +
-                              ra1 * qa[(size_t)v * N + l];
+      qa[(size_t)v * N + l] = rp * r2 * ((r1 - 1.0) * q[v] + drvb[(ip * 5 + v) * F + jk]) + ra1 * q[v];
   }
   if (a.opt) put_primitives(b, c, phys(), l);   // the next stage's primitives, formed in the update
 }
