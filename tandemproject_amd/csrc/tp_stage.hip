@@ -125,29 +125,40 @@ __device__ void movef_dev(const CaseCoef &c, double *sc, double dtko, double dtk
   double *umf = sc + S_UMF1, *dudt = sc + S_DUDT1;
   const double timo = sc[S_TIMO];
   if (c.nsmf == 0) {
+    // each angle's sine and cosine from one evaluation (cr_sin and cr_cos of
+    // an angle are the two results of cr_sincos)
     double ra0 = PI_D / c.timf;
     double ra1 = ra0 * fmin(timo, c.timf), ra2 = ra0 * fmin(timo + dtko, c.timf);
-    double fctr = 1.0 - cr_cos(ra1), dfdt = ra0 * cr_sin(ra2);
+    double s1, c1, s2, c2;
+    cr_sincos(ra1, &s1, &c1);
+    cr_sincos(ra2, &s2, &c2);
+    double fctr = 1.0 - c1, dfdt = ra0 * s2;
     double progmf = 0.5 * (fctr + dtk * dfdt);
     for (int d = 0; d < 3; d++) umf[d] = progmf * c.uoo[d];
-    fctr = cr_sin(ra1);
-    dfdt = ra0 * cr_cos(ra2);
+    fctr = s1;
+    dfdt = ra0 * c2;
     progmf = 0.5 * ra0 * (fctr + dtk * dfdt);
     for (int d = 0; d < 3; d++) dudt[d] = progmf * c.uoo[d];
     if (c.tp > 0 && timo + dtk > c.tps) {
       double daoa = (c.aoa1 - c.aoa0) * PI_D / 180, raoa0 = c.aoa0 * PI_D / 180;
       double ra3 = timo - c.tps, dtsdt = PI_D / c.tp;
       double ts = fmin(dtsdt * ra3, PI_D), tsdt = fmin(dtsdt * (ra3 + dtko), PI_D);
-      double aoat = raoa0 + 0.5 * daoa * (1 - cr_cos(ts));
-      double aoatdt = raoa0 + 0.5 * daoa * (1 - cr_cos(tsdt));
+      double sts, cts, stsdt, ctsdt;
+      cr_sincos(ts, &sts, &cts);
+      cr_sincos(tsdt, &stsdt, &ctsdt);
+      double aoat = raoa0 + 0.5 * daoa * (1 - cts);
+      double aoatdt = raoa0 + 0.5 * daoa * (1 - ctsdt);
+      double sa, ca, sad, cad;
+      cr_sincos(aoat, &sa, &ca);
+      cr_sincos(aoatdt, &sad, &cad);
       double f = 0.5 * daoa * dtsdt, am = c.amachoo;
-      double ut0 = am * cr_cos(aoat), ut1 = am * cr_sin(aoat);
-      double du0 = -(am * f * cr_sin(aoatdt) * cr_sin(tsdt)), du1 = am * f * cr_cos(aoatdt) * cr_sin(tsdt);
+      double ut0 = am * ca, ut1 = am * sa;
+      double du0 = -(am * f * sad * stsdt), du1 = am * f * cad * stsdt;
       umf[0] = ut0 + dtk * du0; umf[1] = ut1 + dtk * du1; umf[2] = 0.0 + dtk * 0.0;
-      double dd0 = -(am * f * cr_sin(aoat) * cr_sin(ts)), dd1 = am * f * cr_cos(aoat) * cr_sin(ts);
-      double st = cr_sin(tsdt);
-      double d20 = -(am * f * (cr_cos(aoatdt) * f * (st * st) + dtsdt * cr_sin(aoatdt) * cr_cos(tsdt)));
-      double d21 = -(am * f * (cr_sin(aoatdt) * f * (st * st) - dtsdt * cr_cos(aoatdt) * cr_cos(tsdt)));
+      double dd0 = -(am * f * sa * sts), dd1 = am * f * ca * sts;
+      double st = stsdt;
+      double d20 = -(am * f * (cad * f * (st * st) + dtsdt * sad * ctsdt));
+      double d21 = -(am * f * (sad * f * (st * st) - dtsdt * cad * ctsdt));
       dudt[0] = dd0 + dtk * d20; dudt[1] = dd1 + dtk * d21; dudt[2] = 0.0 + dtk * 0.0;
       sc[S_AOA] = atan(umf[1] / umf[0]) * 180 / PI_D;
     }
