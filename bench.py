@@ -272,6 +272,30 @@ def measured_traffic_ratio():
         return None
 
 
+# Bytes per point and time step of a minimal viscous schedule (FP64, O(face)
+# and sponge-zone traffic left out), in doubles per stage: three gradient
+# passes reading Q (5) and writing 4 derivatives, 3 x 9 = 27; stresses and
+# heat fluxes reading 12 gradients, 9 metrics, Q and the Jacobian and writing
+# 9, 36; three flux passes reading Q, one metric row (3) and the 9 stresses
+# and heat fluxes, plus the residual accumulated (25), 3 x 17 + 25 = 76; the
+# update reading Q0, the residual and the Jacobian and writing Q, 16.  155
+# doubles per stage, four stages, plus the filter sweep's 3 x 10 doubles:
+# 1240 x 4 + 240 = 5200 B.
+STEP_BYTES_PER_POINT = 5200
+
+
+def measured_step_bytes():
+    """HBM bytes per point moved by every kernel of one time step of the
+    default workload (FETCH_SIZE/WRITE_SIZE passes summarised under
+    profiles/); None when no such summary is present."""
+    path = os.path.join(ROOT, "profiles", "step_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_point_step_measured"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def copy_probe(mib):
     """One device-to-device copy of mib MiB through the HIP runtime (a blit
     kernel): a reference of known traffic for scaling hardware byte counters."""
@@ -404,6 +428,13 @@ def main():
                 "kernel": "compact derivative line solves (k_col / k_deriv, 3 directions)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl,
                 "bytes_per_launch": round(by / nl)}
+    # the whole step against the same peak: the minimal schedule's bytes per
+    # point-step at the measured throughput, and the bytes the kernels were
+    # measured to move
+    step_roof = {"bound": "hbm", "bytes_per_point_step": STEP_BYTES_PER_POINT,
+                 "achieved": round(value * 1e6 * STEP_BYTES_PER_POINT / 1e9, 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": round(value * 1e6 * STEP_BYTES_PER_POINT / 1e9 / HBM_PEAK_GBS, 4),
+                 "measured_bytes_per_point_step": measured_step_bytes()}
     breakdown = {k: round(v[0] / breakdown_steps, 3) for k, v in prof_all.items() if v[1]}
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
@@ -433,7 +464,7 @@ def main():
                    "points": npts, "sub_blocks": lay.nb, "graph": bool(args.graph),
                    "line_chunks": line_chunks,
                    "ms_per_step_by_class": breakdown},
-        "roofline": roof, "cpu_baseline": cpu, "graph_replay": graph_replay, "tandem": tandem, "ensemble": ens,
+        "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu, "graph_replay": graph_replay, "tandem": tandem, "ensemble": ens,
         "serving": srv,
     }
     print(json.dumps(out))
