@@ -216,7 +216,11 @@ int32_t tp_wall(tp_context *ctx);
 int32_t tp_junction(tp_context *ctx);
 /* interface averaging of qa (src/Main3D.f90:897-952) */
 int32_t tp_interface_avg(tp_context *ctx);
-/* nsteps whole time steps (filter + nkrk stages + n, t advance): src/Main3D.f90:552-965 */
+/* nsteps whole time steps (filter + nkrk stages + n, t advance): src/Main3D.f90:552-965.
+ * qa, qo, p, de and the scalars end as the reference leaves them.  ss(:,1),
+ * the viscosity (stage scratch), does not: the flux pass forms it without
+ * storing it, and the step stores it only on the steps whose time-step limit
+ * reads it (n = 1, 11, 21, ...). */
 int32_t tp_step(tp_context *ctx, int32_t nsteps);
 /* the same, replaying one step captured in a HIP graph */
 int32_t tp_step_graph(tp_context *ctx, int32_t nsteps);

@@ -803,12 +803,16 @@ void enqueue_filter_step(tp_context *c) {
 // (:576-583), time step at the first stage (:587-622)
 // prim_ready: the previous stage's update already left this stage's
 // primitives (enqueue_rk with prim)
-void enqueue_begin(tp_context *c, int nk, bool prim_ready = false) {
+// lean (inside a whole time step): the primitives pass leaves 1/rho and the
+// viscosity only on the steps whose time-step limit reads them
+void enqueue_begin(tp_context *c, int nk, bool prim_ready = false, bool lean = false) {
   const int ncase = (int)c->linked.size();
   launch_sc_stage(c->d_coef, c->d_sc, ncase, nk, c->stream);
   StageArgs a = base_args(c);
   a.nk = nk;
-  if (!prim_ready) run_points(c, PK_PRIM, a, K_POINT, 96.0);
+  a.opt = lean ? 1 : 0;
+  if (!prim_ready) run_points(c, PK_PRIM, a, K_POINT, lean ? 80.0 : 96.0);
+  a.opt = 0;
   if (nk == 1) {
     run_points(c, PK_DTRED, a, K_POINT, 0.0);
     launch_sc_dt(c->d_coef, c->d_sc, c->d_part, c->d_wr, ncase, 1, c->stream);
@@ -874,8 +878,8 @@ void enqueue_gcbc(tp_context *c) {
 }
 
 // frame velocity for the update and the RK update, src/Main3D.f90:873-881
-// prim: the update also leaves the next stage's primitives (de(:,1:5), p,
-// ss(:,1) of the new state, k_primitives' arithmetic), and the wall, junction
+// prim: the update also leaves the primitives the next stage reads
+// (de(:,2:5) and p of the new state, k_primitives' arithmetic), and the wall, junction
 // and averaging passes after it form them again where they change the state,
 // so that the next stage starts without the primitives pass and its read of
 // qa (the residual in de is read first)
@@ -883,7 +887,7 @@ void enqueue_rk(tp_context *c, int nk, bool prim = false) {
   launch_sc_rk(c->d_coef, c->d_sc, (int)c->linked.size(), nk, c->stream);
   StageArgs a = base_args(c);
   a.nk = nk;
-  if (prim) run_points(c, PK_RK_PRIM, a, K_RK, 184.0);
+  if (prim) run_points(c, PK_RK_PRIM, a, K_RK, 168.0);
   else run_points(c, PK_RK, a, K_RK, 128.0);
 }
 
@@ -915,8 +919,8 @@ void enqueue_avg(tp_context *c, bool prim = false) {
 // prim_ready: the previous stage left this one's primitives; prim_next: leave
 // the next stage's (inside a time step only: after a single stage the state
 // is the reference's, with the residual in de)
-void enqueue_stage(tp_context *c, int nk, bool prim_ready = false, bool prim_next = false) {
-  enqueue_begin(c, nk, prim_ready);
+void enqueue_stage(tp_context *c, int nk, bool prim_ready = false, bool prim_next = false, bool lean = false) {
+  enqueue_begin(c, nk, prim_ready, lean);
   enqueue_gradients(c);
   {
     StageArgs f = base_args(c);
@@ -939,7 +943,7 @@ void enqueue_stage(tp_context *c, int nk, bool prim_ready = false, bool prim_nex
 void enqueue_step(tp_context *c) {
   if (c->linked.empty()) return;
   enqueue_filter_step(c);
-  for (int nk = 1; nk <= c->nkrk; nk++) enqueue_stage(c, nk, nk > 1, nk < c->nkrk);
+  for (int nk = 1; nk <= c->nkrk; nk++) enqueue_stage(c, nk, nk > 1, nk < c->nkrk, true);
   launch_sc_end(c->d_sc, (int)c->linked.size(), nullptr, c->stream);
 }
 

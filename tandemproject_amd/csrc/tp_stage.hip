@@ -329,31 +329,39 @@ __device__ inline Prim primitives(const CaseCoef &c, const Phys &ph, double q1, 
   return r;
 }
 
-// the primitives of the point's conserved state q into de(:,1:5), p and the
-// viscosity ss(:,1) (plain stores: the face kernels' points)
+// the primitives of the point's conserved state q that a later stage of the
+// same time step reads: u, v, w, T into de(:,2:5) and p (plain stores: the
+// face kernels' points).  1/rho (de(:,1)) and the viscosity (ss(:,1)) are
+// read only by the time-step limit, which follows the primitives pass of a
+// step's first stage; the flux pass forms the viscosity again from q.
 __device__ inline void put_primitives(const Block &b, const CaseCoef &c, const Phys &ph, int l) {
   const int N = b.N;
   const Prim r = primitives(c, ph, b.qa[l], b.qa[N + l], b.qa[2 * N + l], b.qa[3 * N + l], b.qa[4 * N + l]);
-  b.de[l] = r.d1; b.de[N + l] = r.d2; b.de[2 * N + l] = r.d3; b.de[3 * N + l] = r.d4; b.de[4 * N + l] = r.d5;
+  b.de[N + l] = r.d2; b.de[2 * N + l] = r.d3; b.de[3 * N + l] = r.d4; b.de[4 * N + l] = r.d5;
   b.p[l] = r.p;
-  b.ss[l] = r.mu;
 }
 
+// opt (inside a whole time step): 1/rho (de(:,1)) and the viscosity
+// (ss(:,1)) only on the steps whose time-step limit reads them (the test of
+// k_dt_reduce); nothing else in the step reads them
 __global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
   WORK_POINTS
   const CaseCoef &c = a.coef[b.cs];
   const Phys ph = phys();
+  const bool all = !a.opt || (int)a.sc[b.cs * S_STRIDE + S_N] % 10 == 1;
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
     const double *qa = b.qa;
     const Prim r = primitives(c, ph, ld_stream(&qa[l]), ld_stream(&qa[N + l]), ld_stream(&qa[2 * N + l]),
                               ld_stream(&qa[3 * N + l]), ld_stream(&qa[4 * N + l]));
-    st_stream(&b.de[l], r.d1);
     st_stream(&b.de[N + l], r.d2);
     st_stream(&b.de[2 * N + l], r.d3);
     st_stream(&b.de[3 * N + l], r.d4);
     st_stream(&b.de[4 * N + l], r.d5);
     st_stream(&b.p[l], r.p);
-    st_stream(&b.ss[l], r.mu);
+    if (all) {
+      st_stream(&b.de[l], r.d1);
+      st_stream(&b.ss[l], r.mu);
+    }
   }
 }
 
@@ -952,9 +960,9 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
 
 // ---- RK update, Main3D.f90:876-881 ----
 // PRIM: also the next stage's primitives of the updated state (k_primitives'
-// arithmetic on the same values), into de, p and ss once the residual is
-// read; the wall, junction and averaging passes that follow form them again
-// at the points they change
+// arithmetic on the same values) that the stage reads, u, v, w, T into
+// de(:,2:5) and p, once the residual is read; the wall, junction and
+// averaging passes that follow form them again at the points they change
 template <bool PRIM>
 __global__ void __launch_bounds__(256) k_rk(StageArgs a) {
   WORK_POINTS
@@ -970,13 +978,11 @@ __global__ void __launch_bounds__(256) k_rk(StageArgs a) {
     }
     if (PRIM) {
       const Prim r = primitives(c, ph, q[0], q[1], q[2], q[3], q[4]);
-      st_stream(&b.de[l], r.d1);
       st_stream(&b.de[N + l], r.d2);
       st_stream(&b.de[2 * N + l], r.d3);
       st_stream(&b.de[3 * N + l], r.d4);
       st_stream(&b.de[4 * N + l], r.d5);
       st_stream(&b.p[l], r.p);
-      st_stream(&b.ss[l], r.mu);
     }
   }
 }
