@@ -496,6 +496,8 @@ struct tp_context {
   size_t lds[3] = {0, 0, 0};
   bool exact_lines = false;        // sequential one-lane-per-line sweeps
   bool keep_stress = false;        // tv written at every point of every stage
+  bool qin = false;                // kernels enqueued now read the stage's state from qo
+                                   // (StageArgs::qin): set by enqueue_step only
   int col_fixed = 0;               // requested chunks per line (0: chosen per direction)
   // autotuned chunks per line for each direction, kind and line-length class (-1: none)
   int tuned[3][2][kNClass];
@@ -601,6 +603,7 @@ StageArgs base_args(tp_context *c) {
   a.sc = c->d_sc;
   a.part = c->d_part;
   a.wpart = c->d_wpart;
+  a.qin = c->qin ? 1 : 0;
   return a;
 }
 
@@ -783,7 +786,10 @@ bool direct_ok(const tp_context *c, int kind) {
   return true;
 }
 
-void enqueue_filter_step(tp_context *c) {
+// lean (inside a whole time step): the last filter leaves the result in qo
+// only, and the step's first stage reads its state from there until its
+// update writes qa from qo
+void enqueue_filter_step(tp_context *c, bool lean = false) {
   // src/Main3D.f90:555-560: per field m, mpigo then the xi, eta, zeta filters
   // in place; the five fields are independent, so each of these steps runs
   // over all five in one launch; the last also leaves the result in qo
@@ -793,7 +799,8 @@ void enqueue_filter_step(tp_context *c) {
   run_exchange(c, 1, 1, 0, F_QA, 0, 5, 1);
   run_line(c, 1, 0, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
   run_line(c, 1, 1, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1);
-  run_line(c, 1, 2, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1, F_QO, 0);
+  if (lean) run_line(c, 1, 2, F_QA, 0, F_QO, 0, 0, 0, 5, 1, 1);
+  else run_line(c, 1, 2, F_QA, 0, F_QA, 0, 0, 0, 5, 1, 1, F_QO, 0);
 }
 
 // ---- the sections of a stage (src/Main3D.f90:567-958), shared by the fused
@@ -932,6 +939,7 @@ void enqueue_stage(tp_context *c, int nk, bool prim_ready = false, bool prim_nex
   enqueue_flux_lines(c);
   enqueue_sponge(c);
   enqueue_gcbc(c);
+  c->qin = false;   // the update reads qo and writes qa in any case
   enqueue_rk(c, nk, prim_next);
   enqueue_wall(c, prim_next);
   enqueue_junction(c, prim_next);
@@ -942,8 +950,10 @@ void enqueue_stage(tp_context *c, int nk, bool prim_ready = false, bool prim_nex
 // the next one's primitives
 void enqueue_step(tp_context *c) {
   if (c->linked.empty()) return;
-  enqueue_filter_step(c);
+  enqueue_filter_step(c, true);
+  c->qin = true;    // until the first stage's update
   for (int nk = 1; nk <= c->nkrk; nk++) enqueue_stage(c, nk, nk > 1, nk < c->nkrk, true);
+  c->qin = false;
   launch_sc_end(c->d_sc, (int)c->linked.size(), nullptr, c->stream);
 }
 

@@ -263,6 +263,9 @@ struct StageArgs {
                      // k_wall, k_avg_apply: form the primitives of the points they change)
   int arr, col;      // generic field selector
   int o0, o1, o2;    // tv slots for gradient combination (-1..-3 = ss columns)
+  int qin;           // 1: the stage's state is in qo, not qa, until its update (the
+                     // first stage of a whole time step: the filter sweep leaves the
+                     // filtered state in qo only, and the update writes qa from qo)
 };
 
 

@@ -52,15 +52,18 @@ __device__ inline void tile_load(const Block &b, const double *__restrict__ src,
   }
 }
 
+// ADD: dst = (addsrc, or dst itself) + the tile
 template <int D, int ADD>
 __device__ inline void tile_store(const Block &b, double *__restrict__ dst, int q0, int nl, int L,
-                                  int nlp, const double *lds, double *copy = nullptr) {
+                                  int nlp, const double *lds, double *copy = nullptr,
+                                  const double *addsrc = nullptr) {
   if (D == 0) {
     for (int r = 0; r < nl; r++) {
       double *row = dst + (size_t)(q0 + r) * b.nx;
+      const double *arow = addsrc ? addsrc + (size_t)(q0 + r) * b.nx : row;
       for (int p = threadIdx.x; p < L; p += blockDim.x) {
         double v = lds[p * nlp + r];
-        v = ADD ? row[p] + v : v;
+        v = ADD ? arow[p] + v : v;
         row[p] = v;
         if (copy) copy[(size_t)(q0 + r) * b.nx + p] = v;
       }
@@ -73,7 +76,7 @@ __device__ inline void tile_store(const Block &b, double *__restrict__ dst, int 
       line_addr<D>(b, q0 + r, base, stride, jk);
       const size_t o = base + (size_t)p * stride;
       double v = lds[p * nlp + r];
-      v = ADD ? dst[o] + v : v;
+      v = ADD ? (addsrc ? addsrc[o] : dst[o]) + v : v;
       dst[o] = v;
       if (copy) copy[o] = v;
     }
@@ -186,7 +189,8 @@ __global__ void __launch_bounds__(64) k_deriv(LineArgs a) {
   else tile_store<D, 0>(b, dst, q0, nl, L, nlp, lds);
 }
 
-// ---- compact filter along direction D (in place), Subroutines3D.f90:145-206 ----
+// ---- compact filter along direction D, Subroutines3D.f90:145-206: in place,
+// or into another field (out_arr, out_col) when the caller asks ----
 template <int D>
 __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
   extern __shared__ double lds[];
@@ -279,8 +283,10 @@ __global__ void __launch_bounds__(64) k_filter(LineArgs a) {
 #undef S
   }
   __syncthreads();
-  tile_store<D, 1>(b, io, q0, nl, L, nlp, lds,
-                   a.copy_arr >= 0 ? field_ptr(b, a.copy_arr, a.copy_col + f * a.out_step) : nullptr);
+  double *out = field_ptr(b, a.out_arr, a.out_col + f * a.out_step);
+  tile_store<D, 1>(b, out, q0, nl, L, nlp, lds,
+                   a.copy_arr >= 0 ? field_ptr(b, a.copy_arr, a.copy_col + f * a.out_step) : nullptr,
+                   out != io ? io : nullptr);
 }
 
 // ---- closure sums received from the neighbour, Subroutines3D.f90:16-80 ----

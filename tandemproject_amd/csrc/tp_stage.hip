@@ -329,6 +329,10 @@ __device__ inline Prim primitives(const CaseCoef &c, const Phys &ph, double q1, 
   return r;
 }
 
+// the conserved state a stage reads before its update (qo on the first stage
+// of a whole time step, whose filter sweep leaves it there only)
+__device__ inline const double *stage_q(const StageArgs &a, const Block &b) { return a.qin ? b.qo : b.qa; }
+
 // the primitives of the point's conserved state q that a later stage of the
 // same time step reads: u, v, w, T into de(:,2:5) and p (plain stores: the
 // face kernels' points).  1/rho (de(:,1)) and the viscosity (ss(:,1)) are
@@ -349,8 +353,8 @@ __global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
   const CaseCoef &c = a.coef[b.cs];
   const Phys ph = phys();
   const bool all = !a.opt || (int)a.sc[b.cs * S_STRIDE + S_N] % 10 == 1;
+  const double *qa = stage_q(a, b);
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
-    const double *qa = b.qa;
     const Prim r = primitives(c, ph, ld_stream(&qa[l]), ld_stream(&qa[N + l]), ld_stream(&qa[2 * N + l]),
                               ld_stream(&qa[3 * N + l]), ld_stream(&qa[4 * N + l]));
     st_stream(&b.de[N + l], r.d2);
@@ -442,7 +446,7 @@ __global__ void __launch_bounds__(256) k_flux_all(StageArgs a) {
   const int visc = c.nviscous == 1;
   // rows of this block whose stresses the surface forces read
   const int wall_row = a.nk == 1 && b.nsurf > 0 ? (b.sside ? b.ny - 1 : 0) : -1;
-  const double *qa = b.qa, *xi = b.xim, *et = b.etm, *ze = b.zem;
+  const double *qa = stage_q(a, b), *xi = b.xim, *et = b.etm, *ze = b.zem;
   const double *ya = b.yaco;
   double *fx = b.fx, *tv = b.tv;
   for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {
@@ -591,10 +595,11 @@ __global__ void __launch_bounds__(256) k_sponge(StageArgs a) {
       pt = ph.ham * pow(rt, GAM);
     }
     const double as = b.asz[q];
-    b.de[l] = b.de[l] + as * (b.qa[l] - rt);
-    b.de[N + l] = b.de[N + l] + as * (b.qa[N + l] - m0);
-    b.de[2 * N + l] = b.de[2 * N + l] + as * (b.qa[2 * N + l] - m1);
-    b.de[3 * N + l] = b.de[3 * N + l] + as * (b.qa[3 * N + l] - m2);
+    const double *qs = stage_q(a, b);
+    b.de[l] = b.de[l] + as * (qs[l] - rt);
+    b.de[N + l] = b.de[N + l] + as * (qs[N + l] - m0);
+    b.de[2 * N + l] = b.de[2 * N + l] + as * (qs[2 * N + l] - m1);
+    b.de[3 * N + l] = b.de[3 * N + l] + as * (qs[3 * N + l] - m2);
     b.de[4 * N + l] = b.de[4 * N + l] + b.bsz[q] * (b.p[l] - pt);
   }
 }
@@ -616,8 +621,9 @@ __global__ void __launch_bounds__(256) k_force(StageArgs a) {
   for (int q = w.a + threadIdx.x; q < w.a + w.n; q += blockDim.x) {
     const int l = b.lcfor[q];
     const double f = ra1 * b.xafor[q] + ra2 * b.xafor[nf + q] + ra3 * b.xafor[2 * nf + q];
-    b.de[N + l] = b.de[N + l] + b.qa[l] * f;
-    b.de[2 * N + l] = b.de[2 * N + l] - b.qa[l] * f;
+    const double rho = stage_q(a, b)[l];
+    b.de[N + l] = b.de[N + l] + rho * f;
+    b.de[2 * N + l] = b.de[2 * N + l] - rho * f;
   }
 }
 
@@ -804,7 +810,8 @@ __global__ void __launch_bounds__(256) k_gcbc_prep(StageArgs a) {
   int l = indx3(b, ip * (b.L[d] - 1), j, k, d);
   double ra0 = (double)((20 - np) * (25 - np) * (30 - np) / 3000), ra1 = 1.0 - ra0;
   double q[5], cm[3], cha[5], v[5], o[5];
-  for (int c = 0; c < 5; c++) q[c] = b.qa[(size_t)c * N + l];
+  const double *qs = stage_q(a, b);
+  for (int c = 0; c < 5; c++) q[c] = qs[(size_t)c * N + l];
   for (int c = 0; c < 3; c++) cm[c] = b.cm[d][(ip * 3 + c) * F + jk];
   Elem e;
   eleme(q, b.p[l], cm, sc + S_UMF1, e);
@@ -882,7 +889,8 @@ __global__ void __launch_bounds__(256) k_gcbc_apply(StageArgs a) {
       pl = b.gq[(ip * 7 + 5) * F + jk];
       yl = b.gq[(ip * 7 + 6) * F + jk];
     } else {
-      for (int v = 0; v < 5; v++) q[v] = b.qa[(size_t)v * N + l];
+      const double *qs = stage_q(a, b);
+      for (int v = 0; v < 5; v++) q[v] = qs[(size_t)v * N + l];
       pl = b.p[l];
       yl = b.yaco[l];
     }
@@ -1235,7 +1243,7 @@ __global__ void __launch_bounds__(256) k_surf_sample(StageArgs a) {
     f0 = p * n1 * ar;
     f1 = p * n2 * ar;
     if (c.nviscous == 1) {
-      const double *tv = b.tv, rho = b.qa[l];
+      const double *tv = b.tv, rho = stage_q(a, b)[l];
       const double tw1 = rho * (tv[l] * n1 + tv[3 * N + l] * n2 + tv[5 * N + l] * n3);
       const double tw2 = rho * (tv[3 * N + l] * n1 + tv[N + l] * n2 + tv[4 * N + l] * n3);
       f2 = tw1 * ar;
