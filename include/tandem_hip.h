@@ -412,15 +412,18 @@ int32_t tp_case_info(tp_context *ctx, int32_t case_id, int64_t *out4);
 
 /* ---- request loop (no counterpart in the reference) ----
  * A server owns a context.  Requests -- whole cases with their grids and a
- * step count -- queue on the host with an arrival time.  A staging thread
- * sets arrived requests up, in arrival order, in contexts of their own
- * (device memory, metrics, sponge, inflow table, the uniform initial state)
+ * step count -- queue on the host with an arrival time.  Two staging
+ * threads take arrived requests in arrival order and set each up in a
+ * context of its own (device memory, metrics, sponge, inflow table, the
+ * uniform initial state)
  * while their device bytes fit the HBM budget beside the cases on the device
  * and fewer than tp_server_set_staging requests wait set up; this runs on
  * its own streams, beside the stepping.  Each iteration
- *   1. admits set-up requests in arrival order while a slot is free and the
- *      cases on the device hold fewer points than the fill target
- *      (tp_server_set_fill), joining them to the cases on the device
+ *   1. admits set-up requests in arrival order (a request whose set-up
+ *      finished first may go before an earlier one still being set up)
+ *      while a slot is free and the cases on the device hold fewer points
+ *      than the fill target (tp_server_set_fill), joining them to the cases
+ *      on the device
  *      (tp_adopt_cases: no copy, no recomputation); an idle device first
  *      waits until the requests that have arrived are set up, or until those
  *      set up fill the slots, the fill target or the staging window, so that

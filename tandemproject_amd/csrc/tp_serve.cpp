@@ -227,10 +227,10 @@ int retire(tp_server *s, const std::vector<int> &fin) {
   return 0;
 }
 
-// the staging thread: sets arrived requests up, in arrival order, each in
-// its own context (build, geometry, initial state), while the device bytes of
-// the cases on the device and of those set up fit the budget and fewer than
-// `ahead` requests wait set up
+// a staging thread (there are nstagers): takes arrived requests in arrival
+// order and sets each up in its own context (build, geometry, initial
+// state), while the device bytes of the cases on the device and of those set
+// up fit the budget and fewer than `ahead` requests wait set up
 void stager_main(tp_server *s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);

@@ -370,13 +370,16 @@ __global__ void __launch_bounds__(256) k_primitives(StageArgs a) {
 }
 
 // ---- time-step maxima, Main3D.f90:590-603 (partials per workgroup) ----
+// The limit is formed on every tenth step (n = 1, 11, ...); k_sc_dt reads
+// the partials only then, so on the other steps a workgroup returns at once
+// (uniformly: a work item lies in one block of one case).
 __global__ void __launch_bounds__(256) k_dt_reduce(StageArgs a) {
   WORK_POINTS
   const double *sc = a.sc + b.cs * S_STRIDE;
+  if ((int)sc[S_N] % 10 != 1) return;
   __shared__ double mi[256], mv[256];
   double fi = -INFINITY, fv = -INFINITY;
-  int n = (int)sc[S_N];
-  if (n % 10 == 1) {
+  {
     const double u0 = sc[S_UMF1], u1 = sc[S_UMF2], u2 = sc[S_UMF3];
     const double *xi = b.xim, *et = b.etm, *ze = b.zem, *de = b.de;
     for (int l = w.a + threadIdx.x; l < w.a + w.n; l += blockDim.x) {

@@ -233,7 +233,10 @@ def test_server_admits_by_hbm_budget():
 
 def test_server_fill_target_runs_cases_in_turn():
     """With a fill target below one case's points the loop admits one case at
-    a time, in arrival order, and each result equals the case run alone."""
+    a time, and each result equals the case run alone.  (Two staging threads
+    set requests up side by side, so an idle device may take a later arrival
+    whose set-up finished first: the admissions are compared in the order
+    they happened.)"""
     say("request loop with a one-point fill target")
     srv = api.Server(budget_bytes=0, max_active=8, graph=True, fill_points=1)
     srv.set_option(0, 1)
@@ -242,7 +245,7 @@ def test_server_fill_target_runs_cases_in_turn():
     srv.run(max_steps=4)
     st = srv.stats()
     assert st["finished"] == 3 and st["peak_cases"] == 1
-    infos = [srv.result(r, with_state=False)[0] for r in reqs]
+    infos = sorted((srv.result(r, with_state=False)[0] for r in reqs), key=lambda d: d["admitted"])
     for a, b in zip(infos, infos[1:]):
         assert b["admitted"] >= a["done"]
     for r, (lay, ss, p), n in zip(reqs, CASES, nsteps):
