@@ -814,7 +814,9 @@ void enqueue_filter_step(tp_context *c, bool lean = false) {
 // viscosity only on the steps whose time-step limit reads them
 void enqueue_begin(tp_context *c, int nk, bool prim_ready = false, bool lean = false) {
   const int ncase = (int)c->linked.size();
-  launch_sc_stage(c->d_coef, c->d_sc, ncase, nk, c->stream);
+  // (with prim_ready the previous stage's k_sc_rk left this stage's start
+  // scalars too)
+  if (!prim_ready) launch_sc_stage(c->d_coef, c->d_sc, ncase, nk, c->stream);
   StageArgs a = base_args(c);
   a.nk = nk;
   a.opt = lean ? 1 : 0;
@@ -891,7 +893,7 @@ void enqueue_gcbc(tp_context *c) {
 // so that the next stage starts without the primitives pass and its read of
 // qa (the residual in de is read first)
 void enqueue_rk(tp_context *c, int nk, bool prim = false) {
-  launch_sc_rk(c->d_coef, c->d_sc, (int)c->linked.size(), nk, c->stream);
+  launch_sc_rk(c->d_coef, c->d_sc, (int)c->linked.size(), nk, prim ? 1 : 0, c->stream);
   StageArgs a = base_args(c);
   a.nk = nk;
   if (prim) run_points(c, PK_RK_PRIM, a, K_RK, 168.0);

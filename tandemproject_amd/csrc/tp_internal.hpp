@@ -286,7 +286,8 @@ void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a);
 void launch_sc_stage(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st);
 void launch_sc_dt(const CaseCoef *coef, double *sc, const double *part, const int *wr, int ncase,
                   int reduced, hipStream_t st);
-void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st);
+// next: also the next stage's start scalars (inside a whole time step)
+void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, int next, hipStream_t st);
 void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st);
 void launch_junction(const Block *blocks, const CaseCoef *coef, const int *jct, const int *jvel, int njct, int maxk,
                      int prim, hipStream_t st);

@@ -273,8 +273,12 @@ __global__ void k_sc_dt(const CaseCoef *coef, double *sc, const double *part, co
   }
 }
 
-// before the RK update: Main3D.f90:873-874
-__global__ void k_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk) {
+// before the RK update: Main3D.f90:873-874.  Its frame arguments (dtko, dtk)
+// are those of the next stage's start (k_sc_stage(nk + 1): the same time,
+// the same step), so with next set (inside a whole time step) it also forms
+// the next stage's inflow weights and that stage needs no k_sc_stage; this
+// stage's sponge, their only reader, has run.
+__global__ void k_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, int next) {
   int cs = blockIdx.x * blockDim.x + threadIdx.x;
   if (cs >= ncase) return;
   const CaseCoef &c = coef[cs];
@@ -284,6 +288,7 @@ __global__ void k_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk) {
   double dtko = dt * (double)f2 / (c.nkrk - nk + 2);
   double dtk = dt / (c.nkrk - nk + 1);
   movef_dev(c, s, dtko, dtk);
+  if (next && c.inflow) inflow_weights(c, s);
 }
 
 // end of step: Main3D.f90:964-965
@@ -1356,8 +1361,8 @@ void launch_sc_dt(const CaseCoef *coef, double *sc, const double *part, const in
                   hipStream_t st) {
   hipLaunchKernelGGL(k_sc_dt, dim3(ncase), dim3(256), 0, st, coef, sc, part, wr, ncase, reduced);
 }
-void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, hipStream_t st) {
-  hipLaunchKernelGGL(k_sc_rk, dim3((ncase + 63) / 64), dim3(64), 0, st, coef, sc, ncase, nk);
+void launch_sc_rk(const CaseCoef *coef, double *sc, int ncase, int nk, int next, hipStream_t st) {
+  hipLaunchKernelGGL(k_sc_rk, dim3((ncase + 63) / 64), dim3(64), 0, st, coef, sc, ncase, nk, next);
 }
 void launch_sc_end(double *sc, int ncase, const int *active, hipStream_t st) {
   hipLaunchKernelGGL(k_sc_end, dim3((ncase + 63) / 64), dim3(64), 0, st, sc, ncase, active);
