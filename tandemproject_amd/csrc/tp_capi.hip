@@ -558,24 +558,51 @@ hipEvent_t take_event(tp_context *c) {
   return e;
 }
 
+// Times the launches enqueued in its scope (profiling on, class selected).
+// Plain: event packets recorded on the stream before and after them.  ext:
+// the two events are handed to the first and the last launch (start() and
+// stop()), which carry them in their own dispatches: no packets between the
+// launches, so timing costs the stream next to nothing.
 struct Timed {
   tp_context *c;
   int cls;
   double bytes;
-  hipEvent_t a = nullptr;
-  Timed(tp_context *cc, int k, double by) : c(cc), cls(k), bytes(by) {
+  bool ext;
+  hipEvent_t a = nullptr, b = nullptr;
+  bool used = false;
+  Timed(tp_context *cc, int k, double by, bool ext_ = false) : c(cc), cls(k), bytes(by), ext(ext_) {
     if (c->prof && (c->prof_mask >> cls & 1)) {
       a = take_event(c);
-      if (a) (void)hipEventRecord(a, c->stream);
+      if (a && ext) {
+        b = take_event(c);
+        if (!b) { c->pool.push_back(a); a = nullptr; }
+      } else if (a) {
+        (void)hipEventRecord(a, c->stream);
+      }
     }
   }
+  // the events of launch i of n (ext)
+  hipEvent_t start(int i) { return a && i == 0 ? a : nullptr; }
+  hipEvent_t stop(int i, int n) {
+    if (!a) return nullptr;
+    used = true;
+    return i == n - 1 ? b : nullptr;
+  }
   ~Timed() {
-    if (a) {
-      hipEvent_t b = take_event(c);
-      if (b) {
-        (void)hipEventRecord(b, c->stream);
+    if (!a) return;
+    if (ext) {
+      if (used) {
         c->pend[cls].push_back({a, b, bytes});
+      } else {
+        c->pool.push_back(a);
+        c->pool.push_back(b);
       }
+      return;
+    }
+    hipEvent_t e = take_event(c);
+    if (e) {
+      (void)hipEventRecord(e, c->stream);
+      c->pend[cls].push_back({a, e, bytes});
     }
   }
 };
@@ -628,7 +655,8 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   if (!c->L.lines[d].n) return;
   // algorithmic bytes: one field in, one out; an accumulated output is also
   // read, a copy of the output (the last filter's qo) also written
-  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, ((acc ? 24.0 : 16.0) + (copy_arr >= 0 ? 8.0 : 0.0)) * (double)c->npts * nf);
+  Timed t(c, kind == 0 ? K_DERIV : K_FILTER, ((acc ? 24.0 : 16.0) + (copy_arr >= 0 ? 8.0 : 0.0)) * (double)c->npts * nf,
+          true);
   LineArgs a;
   a.blocks = c->d_blocks; a.work = c->L.lines[d].d; a.coef = c->d_coef;
   a.in_arr = in_arr; a.in_col = in_col; a.out_arr = out_arr; a.out_col = out_col;
@@ -638,17 +666,22 @@ void run_line(tp_context *c, int kind, int d, int in_arr, int in_col, int out_ar
   a.direct = direct ? 1 : 0;
   a.n45 = 0;
   if (c->exact_lines) {
-    if (kind == 0) launch_deriv(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
-    else launch_filter(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a);
+    hipEvent_t e0 = t.start(0), e1 = t.stop(0, 1);
+    if (kind == 0) launch_deriv(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a, e0, e1);
+    else launch_filter(d, c->L.lines[d].n, nf, c->lds[d], c->stream, a, e0, e1);
     return;
   }
+  int ng = 0, i = 0;
+  for (const ColGroup &g : c->L.cg[d][kind]) ng += g.work.n ? 1 : 0;
   for (const ColGroup &g : c->L.cg[d][kind]) {
     if (!g.work.n) continue;
     a.work = g.work.d;
     a.direct = direct && g.closure ? 1 : 0;
-    if (g.P) launch_col(d, kind, g.C, g.P, g.work.n, nf, c->stream, a);
-    else if (kind == 0) launch_deriv(d, g.work.n, nf, c->lds[d], c->stream, a);
-    else launch_filter(d, g.work.n, nf, c->lds[d], c->stream, a);
+    hipEvent_t e0 = t.start(i), e1 = t.stop(i, ng);
+    if (g.P) launch_col(d, kind, g.C, g.P, g.work.n, nf, c->stream, a, e0, e1);
+    else if (kind == 0) launch_deriv(d, g.work.n, nf, c->lds[d], c->stream, a, e0, e1);
+    else launch_filter(d, g.work.n, nf, c->lds[d], c->stream, a, e0, e1);
+    i++;
   }
 }
 

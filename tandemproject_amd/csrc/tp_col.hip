@@ -698,15 +698,16 @@ __global__ void __launch_bounds__(1024) k_col(LineArgs a) {
 size_t col_lds_bytes(int d, int P) { return (size_t)col_lds_doubles(d, P) * sizeof(double); }
 
 template <int D, int KIND>
-static void launch_col_c(int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a) {
+static void launch_col_c(int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a, hipEvent_t e0,
+                         hipEvent_t e1) {
   const size_t lds = col_lds_bytes(D, P);
   switch (C) {
-#define CASE_C(CV)                                                                                   \
-  case CV:                                                                                         \
-    if (KIND == 0 && a.direct)                                                                     \
-      hipLaunchKernelGGL((k_col<D, KIND, CV, KIND == 0>), dim3(nwg, nf), dim3(64 * P), lds, st, a); \
-    else                                                                                           \
-      hipLaunchKernelGGL((k_col<D, KIND, CV, false>), dim3(nwg, nf), dim3(64 * P), lds, st, a);    \
+#define CASE_C(CV)                                                                                     \
+  case CV:                                                                                           \
+    if (KIND == 0 && a.direct)                                                                       \
+      TP_LAUNCH_EV((k_col<D, KIND, CV, KIND == 0>), dim3(nwg, nf), dim3(64 * P), lds, st, e0, e1, a); \
+    else                                                                                             \
+      TP_LAUNCH_EV((k_col<D, KIND, CV, false>), dim3(nwg, nf), dim3(64 * P), lds, st, e0, e1, a);    \
     break;
     CASE_C(8) CASE_C(12) CASE_C(16) CASE_C(20) CASE_C(24) CASE_C(28) CASE_C(32)
 #undef CASE_C
@@ -720,15 +721,16 @@ int col_capacity(int chunk) {
   return chunk <= 8 ? 8 : (chunk + 3) / 4 * 4;
 }
 
-void launch_col(int d, int kind, int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a) {
+void launch_col(int d, int kind, int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a, hipEvent_t e0,
+                hipEvent_t e1) {
   if (kind == 0) {
-    if (d == 0) launch_col_c<0, 0>(C, P, nwg, nf, st, a);
-    else if (d == 1) launch_col_c<1, 0>(C, P, nwg, nf, st, a);
-    else launch_col_c<2, 0>(C, P, nwg, nf, st, a);
+    if (d == 0) launch_col_c<0, 0>(C, P, nwg, nf, st, a, e0, e1);
+    else if (d == 1) launch_col_c<1, 0>(C, P, nwg, nf, st, a, e0, e1);
+    else launch_col_c<2, 0>(C, P, nwg, nf, st, a, e0, e1);
   } else {
-    if (d == 0) launch_col_c<0, 1>(C, P, nwg, nf, st, a);
-    else if (d == 1) launch_col_c<1, 1>(C, P, nwg, nf, st, a);
-    else launch_col_c<2, 1>(C, P, nwg, nf, st, a);
+    if (d == 0) launch_col_c<0, 1>(C, P, nwg, nf, st, a, e0, e1);
+    else if (d == 1) launch_col_c<1, 1>(C, P, nwg, nf, st, a, e0, e1);
+    else launch_col_c<2, 1>(C, P, nwg, nf, st, a, e0, e1);
   }
 }
 

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 
 #include <string>
@@ -277,9 +278,21 @@ void launch_eddy_table(const double *eddies, int nits, const double *pos, int ng
 // chunked line solves: P waves per 64 lines, chunk capacity C points per lane
 int col_capacity(int chunk);
 size_t col_lds_bytes(int d, int P);
-void launch_col(int d, int kind, int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a);
-void launch_deriv(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a);
-void launch_filter(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a);
+// e0, e1 (optional): timing events carried in the launch's own dispatch
+// (hipExtLaunchKernel), recorded as the kernel starts and ends, instead of
+// event packets of their own between the launches
+void launch_col(int d, int kind, int C, int P, int nwg, int nf, hipStream_t st, const LineArgs &a,
+                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void launch_deriv(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a,
+                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void launch_filter(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a,
+                   hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// a launch with the timing events e0, e1 in its dispatch when either is set
+#define TP_LAUNCH_EV(K, G, B, LDS, ST, E0, E1, ...)                                        \
+  do {                                                                                  \
+    if ((E0) || (E1)) hipExtLaunchKernelGGL(K, G, B, (uint32_t)(LDS), ST, E0, E1, 0, __VA_ARGS__); \
+    else hipLaunchKernelGGL(K, G, B, LDS, ST, __VA_ARGS__);                             \
+  } while (0)
 void launch_exchange(int nwg, int nf, hipStream_t st, const ExchArgs &a);
 void launch_point(int which, int nwg, hipStream_t st, const StageArgs &a);
 void launch_face(int which, int nwg, hipStream_t st, const StageArgs &a);

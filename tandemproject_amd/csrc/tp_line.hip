@@ -314,16 +314,18 @@ __global__ void __launch_bounds__(256) k_exchange(ExchArgs a) {
 }
 
 // ---- host launchers ----
-void launch_deriv(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a) {
-  if (d == 0) hipLaunchKernelGGL(k_deriv<0>, dim3(nwg, nf), dim3(64), lds, st, a);
-  else if (d == 1) hipLaunchKernelGGL(k_deriv<1>, dim3(nwg, nf), dim3(64), lds, st, a);
-  else hipLaunchKernelGGL(k_deriv<2>, dim3(nwg, nf), dim3(64), lds, st, a);
+void launch_deriv(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a, hipEvent_t e0,
+                  hipEvent_t e1) {
+  if (d == 0) TP_LAUNCH_EV(k_deriv<0>, dim3(nwg, nf), dim3(64), lds, st, e0, e1, a);
+  else if (d == 1) TP_LAUNCH_EV(k_deriv<1>, dim3(nwg, nf), dim3(64), lds, st, e0, e1, a);
+  else TP_LAUNCH_EV(k_deriv<2>, dim3(nwg, nf), dim3(64), lds, st, e0, e1, a);
 }
 
-void launch_filter(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a) {
-  if (d == 0) hipLaunchKernelGGL(k_filter<0>, dim3(nwg, nf), dim3(64), lds, st, a);
-  else if (d == 1) hipLaunchKernelGGL(k_filter<1>, dim3(nwg, nf), dim3(64), lds, st, a);
-  else hipLaunchKernelGGL(k_filter<2>, dim3(nwg, nf), dim3(64), lds, st, a);
+void launch_filter(int d, int nwg, int nf, size_t lds, hipStream_t st, const LineArgs &a, hipEvent_t e0,
+                   hipEvent_t e1) {
+  if (d == 0) TP_LAUNCH_EV(k_filter<0>, dim3(nwg, nf), dim3(64), lds, st, e0, e1, a);
+  else if (d == 1) TP_LAUNCH_EV(k_filter<1>, dim3(nwg, nf), dim3(64), lds, st, e0, e1, a);
+  else TP_LAUNCH_EV(k_filter<2>, dim3(nwg, nf), dim3(64), lds, st, e0, e1, a);
 }
 
 void launch_exchange(int nwg, int nf, hipStream_t st, const ExchArgs &a) {
