@@ -479,12 +479,15 @@ def test_surface_statistics_long_run():
     say("  worst relative difference of the statistics", worst)
 
 
-def test_profile_classes():
+@pytest.mark.parametrize("mode", ["chunked", "sequential"])
+def test_profile_classes(mode):
     """tp_profile_classes times only the classes asked for; tp_profile times
-    them all; the launches and algorithmic bytes of a step are counted."""
-    say("profiling by kernel class")
+    them all; the launches and algorithmic bytes of a step are counted.  The
+    line solves carry their timing events in their own dispatches, in the
+    chunked launches (one or more per direction) and the sequential ones."""
+    say("profiling by kernel class:", mode)
     lay, ss, p = aerofoil_case(lxibk=(16, 20, 16), letbk=(14, 14))
-    s = api.Solver([(lay, p)])
+    s = api.Solver([(lay, p)], exact_lines=mode == "sequential")
     for b in range(lay.nb):
         s.set_grid(0, b, ss[b])
     s.setup_geometry(); s.initialo(0)
